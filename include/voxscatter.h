@@ -1,0 +1,142 @@
+/*
+ * voxscatter -- MI355X (gfx950) solve loop of the voxel volume-integral-equation scattering solver.
+ *
+ * C ABI.  Everything is plain pointers and sizes:
+ *   - complex arrays are interleaved (re, im) doubles ("complex128") unless a `precision` argument
+ *     says otherwise for device pointers (VS_FP32: interleaved floats);
+ *   - volumes are column-major L x M x N (x fastest), the component index slowest;
+ *   - index arrays are int64, 0-based (a MATLAB gateway subtracts 1 from MATLAB's indices);
+ *   - the caller owns every host buffer; handles own device memory and are released by *_destroy;
+ *   - each call returns VS_OK (0) or a negative VS_ERR_* code, and vs_last_error() (thread-local)
+ *     describes the failure; the library never aborts;
+ *   - a context is one HIP stream on one device, to be used from one host thread at a time.
+ *
+ * Each entry point names the reference (MATLAB) function it replaces, file:line in
+ * samuelpgroth/VoxScatter.
+ */
+#ifndef VOXSCATTER_H
+#define VOXSCATTER_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VS_OK 0
+#define VS_ERR_ARG (-1)
+#define VS_ERR_HIP (-2)
+#define VS_ERR_OOM (-3)
+#define VS_ERR_NODEV (-4)
+#define VS_ERR_UNSUPPORTED (-5)
+#define VS_ERR_NAN (-6)
+#define VS_ERR_INTERNAL (-7)
+
+/* storage precisions (bit mask where a handle may hold both) */
+#define VS_FP64 1
+#define VS_FP32 2
+
+typedef struct vs_ctx vs_ctx;
+typedef struct vs_op vs_op;
+typedef struct vs_pc vs_pc;
+
+/* ---------------------------------------------------------------------------------------------- */
+/* library / context                                                                               */
+/* ---------------------------------------------------------------------------------------------- */
+const char* vs_last_error(void);
+int vs_version(void);
+int vs_device_count(int* n);
+int vs_ctx_create(int device, vs_ctx** out);
+void vs_ctx_destroy(vs_ctx* ctx);
+int vs_ctx_synchronize(vs_ctx* ctx);
+
+/* ---------------------------------------------------------------------------------------------- */
+/* operator assembly                                                                               */
+/* ---------------------------------------------------------------------------------------------- */
+
+/* LDR polarisability alpha_LDR (replaces getOPERATOR_DDA.m:4-26). */
+int vs_alpha_ldr(const double refInd[2], double ko, double d, const double kvec[3], const double Eo[3],
+                 double alpha_out[2]);
+
+/* Toeplitz Green dyadic, its 2L x 2M x 2N circulant embedding and the FFT of the embedding, all
+ * assembled on the device in FP64 (replaces getOPERATOR_DDA.m:28-122, circulant_nop_const.m:5-40,
+ * fft_operator.m:1-8).  r: L*M*N*3 voxel centres.  nearby_quad: 0 = 'off', 1 = 'on'
+ * (getOPERATOR_DDA.m:47-79).  precisions: VS_FP64 and/or VS_FP32 copies of the tensor to keep. */
+int vs_op_create_dda(vs_ctx* ctx, int L, int M, int N, const double* r, double ko, int nearby_quad,
+                     int precisions, vs_op** out);
+
+/* Operator from a caller-supplied circulant FFT fN (2L*2M*2N*6 complex128), e.g. a Galerkin operator
+ * (the fN argument of MVP.m:1).  Toep (L*M*N*6, nullable) enables vs_pc_create_circ. */
+int vs_op_create_fN(vs_ctx* ctx, int L, int M, int N, const double* fN, const double* Toep,
+                    int precisions, vs_op** out);
+void vs_op_destroy(vs_op* op);
+int vs_op_get_toeplitz(vs_op* op, double* Toep); /* L*M*N*6 complex128 */
+int vs_op_get_fN(vs_op* op, double* fN);         /* 2L*2M*2N*6 complex128 */
+
+/* Unknown set and diagonal of MVP.m:1 (idx, Gram, Mr, Mc).  idx: nidx 0-based indices into the
+ * 3*L*M*N unknowns.  Gram/Mr/Mc each hold 1 complex value (scalar) or L*M*N values. */
+int vs_op_set_domain(vs_op* op, const int64_t* idx, int64_t nidx, const double* Gram, int64_t gram_len,
+                     const double* Mr, int64_t mr_len, const double* Mc, int64_t mc_len);
+int64_t vs_op_nunknowns(const vs_op* op);
+
+/* ---------------------------------------------------------------------------------------------- */
+/* matvec (replaces MVP.m:1-145 / mv_AN_const.m, both transp_flag branches)                        */
+/* ---------------------------------------------------------------------------------------------- */
+/* Host buffers, complex128, length vs_op_nunknowns().  transp: 0 'notransp', 1 'transp'. */
+int vs_mvp(vs_op* op, const double* x, double* y, int transp);
+/* Device buffers of nrhs consecutive vectors in `precision` (VS_FP64 or VS_FP32), on the context
+ * stream, asynchronous. */
+int vs_mvp_dev(vs_op* op, const void* x, void* y, int transp, int precision, int nrhs);
+
+/* ---------------------------------------------------------------------------------------------- */
+/* circulant preconditioners                                                                       */
+/* ---------------------------------------------------------------------------------------------- */
+/* Chan circulant along x + FFT along x (circ_1_level.m:5-90); with blocks != NULL also the L dense
+ * 3MN x 3MN 1-level blocks of circ_1_level.m:92-259 ('on'). */
+int vs_circ_1_level(vs_ctx* ctx, int L, int M, int N, const double* Toep, double* circ_L, double* blocks);
+/* Chan along y + FFT along y, and the L*M blocks of 3N x 3N (circ_2_level.m:1-155).  blocks is
+ * ordered block (i,j) at (i + L*j), each block column-major. */
+int vs_circ_2_level(vs_ctx* ctx, int L, int M, int N, const double* circ_L, double* circ_M, double* blocks);
+/* inv(a*I - diag(w)*B) for nblk column-major n x n blocks (main_DDA.m:105-111,
+ * scatteringHumanHead.m:178-184).  w: n complex values, or NULL to use the scalar wscalar. */
+int vs_invert_blocks(vs_ctx* ctx, int64_t nblk, int n, const double* blocks, const double a[2],
+                     const double* w, const double wscalar[2], double* inv_out);
+
+/* Build the preconditioner of an operator on the device: levels 1 (circ_1_level 'on' + chan_mvp_idx)
+ * or 2 (circ_2_level + mvp_circ_2_level); blocks inv(a I - diag(w) C). */
+int vs_pc_create_circ(vs_op* op, int levels, const double a[2], const double* w, const double wscalar[2],
+                      int precisions, vs_pc** out);
+/* Preconditioner from caller-supplied inverse blocks (the inv_blocks argument of
+ * mvp_circ_2_level.m:2 / chan_mvp_idx.m:2); same block ordering as the builders above. */
+int vs_pc_create_blocks(vs_ctx* ctx, int levels, int L, int M, int N, const double* inv_blocks,
+                        const int64_t* idx, int64_t nidx, int precisions, vs_pc** out);
+void vs_pc_destroy(vs_pc* pc);
+/* apply (replaces mvp_circ_2_level.m:2-22 / chan_mvp_idx.m:2-16) */
+int vs_pc_apply(vs_pc* pc, const double* x, double* y);
+int vs_pc_apply_dev(vs_pc* pc, const void* x, void* y, int precision, int nrhs);
+
+/* ---------------------------------------------------------------------------------------------- */
+/* Krylov solvers (device-resident; host buffers complex128 of length vs_op_nunknowns())            */
+/* ---------------------------------------------------------------------------------------------- */
+/* MATLAB bicgstab(A,b,tol,maxit,M1,[],x0) semantics as called at main_DDA.m:82,120.  pc may be NULL.
+ * resvec must hold 2*maxit+1 doubles; *nres receives its used length (length(resvec)). */
+int vs_bicgstab(vs_op* op, vs_pc* pc, const double* b, double* x, double tol, int maxit, const double* x0,
+                int* flag, double* relres, double* iter, double* resvec, int* nres);
+/* MATLAB gmres(A,b,restart,tol,maxit,M) (main_DDA.m:89,128): left preconditioned, restarted.
+ * resvec holds restart*maxit+1 doubles; iter[2] = (outer, inner). */
+int vs_gmres(vs_op* op, vs_pc* pc, const double* b, double* x, int restart, double tol, int maxit,
+             const double* x0, int* flag, double* relres, int iter[2], double* resvec, int* nres);
+/* pgmres(A,b,restart,tol,maxit,L1,[],R1,[],x0) of pgmres.m:70-180 (side 0: pc as L1, 1: as R1). */
+int vs_pgmres(vs_op* op, vs_pc* pc, int side, const double* b, double* x, int restart, double tol,
+              int maxit, const double* x0, int* flag, double* relres, int iter[2], double* resvec,
+              int* nres);
+
+/* Device-pointer forms for benchmarking: b, x are FP64 device vectors. */
+int vs_bicgstab_dev(vs_op* op, vs_pc* pc, const void* b, void* x, double tol, int maxit, int* flag,
+                    double* relres, double* iter, int* nres);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VOXSCATTER_H */

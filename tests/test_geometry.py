@@ -1,0 +1,49 @@
+"""Grid and mask conventions of geometry.m / generatedomain_new.m / koch_snowflake.m."""
+import math
+
+import numpy as np
+
+from voxscatter_amd import geometry as G
+
+
+def test_matlab_round_half_away_from_zero():
+    assert list(G.matlab_round([0.5, 1.5, 2.5, -0.5, -2.5])) == [1, 2, 3, -1, -3]
+
+
+def test_colon_symmetric():
+    a = G.matlab_colon(-(7 / 2 - 0.5) * 0.1, 0.1, (7 / 2 - 0.5) * 0.1)
+    assert a.size == 7 and np.allclose(a, -a[::-1], atol=0, rtol=0)
+
+
+def test_reference_driver_grid_counts():
+    # main_DDA.m: hex, m = 1.4, size parameter 20, 10 voxels per interior wavelength (z count is 4.5 -> 5)
+    r, idx, res, P, le, li = G.geometry("hex", 1.4, 20, 10)
+    assert r.shape == (90, 78, 5, 3)
+    lam_ext = 2 * math.pi / 20
+    for m, shape in ((1.5, (83, 96, 5, 3)), (1.7, (94, 109, 5, 3))):
+        r, idx, P = G.koch_snowflake(lam_ext / m, 10, 1 / 20, 5)
+        assert r.shape == shape
+
+
+def test_inpolygon_counts_edges_as_inside():
+    sq = np.array([[0, 0], [1, 0], [1, 1], [0, 1]], dtype=float)
+    xq = np.array([0.5, 1.0, 0.0, 1.5, 0.5])
+    yq = np.array([0.5, 0.5, 0.0, 0.5, 1.0])
+    assert list(G.inpolygon(xq, yq, sq[:, 0], sq[:, 1])) == [True, True, True, False, True]
+
+
+def test_koch_polygon_vertex_count_and_symmetry():
+    P = G.koch_polygon(3)
+    # 3 * 4^3 edges; duplicates at the origin placeholder
+    uniq = np.unique(np.round(P, 12), axis=0)
+    assert uniq.shape[0] == 3 * 4 ** 3
+    assert np.isclose(P[:, 1].max(), 1.0) and np.isclose(P[:, 1].min(), -1.0)
+
+
+def test_workload_geometries():
+    s = G.hex_plate()
+    assert s.shape == (32, 32, 8) and 0 < s.idx.size < 32 * 32 * 8
+    assert abs(s.d - 1 / (10 * 1.31)) < 1e-15
+    s = G.koch_prism()
+    assert s.shape == (128, 128, 16)
+    assert abs(s.k0 * s.d - 0.3125) < 1e-12

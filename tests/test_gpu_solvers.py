@@ -1,0 +1,121 @@
+"""GPU parity of the solve loop on the 32x32x8 hexagonal ice plate (m = 1.31, d = lambda/(10|m|),
+incidence +z, polarisation +x): the device BiCGStab / GMRES / pgmres against the oracle running the
+same sequence as main_DDA.m.  Bar: identical iteration count (length of resvec) and solution within
+1e-10 relative."""
+import numpy as np
+import pytest
+
+from oracle import circulant as Ch
+from oracle import krylov as OK
+from oracle import mvp as OM
+from oracle import operator as O
+from voxscatter_amd import api, geometry as G
+from voxscatter_amd.dda import DDASystem
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-6
+
+
+@pytest.fixture(scope="module")
+def c1(ctx):
+    s = G.hex_plate()
+    sysd = DDASystem.build(s, ctx=ctx)
+    L, M, N = s.shape
+    fref, Tref, alpha = O.getOPERATOR_DDA(s.r, s.k0, s.refInd, sysd.kvec, s.Eo)
+    chi = np.zeros(L * M * N)
+    chi[s.idx] = 1
+    chi = chi.reshape((L, M, N), order="F")
+    ones = np.ones((L, M, N))
+
+    def afun(v):
+        return OM.MVP(v, fref, ones, sysd.dV * chi, 1 / alpha, "notransp", sysd.idx3)
+
+    return dict(s=s, sys=sysd, Tref=Tref, alpha=alpha, afun=afun, L=L, M=M, N=N)
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def _oracle_pc(c1, levels):
+    L, M, N = c1["L"], c1["M"], c1["N"]
+    if levels == 2:
+        _, cL = Ch.circ_1_level(c1["Tref"], L, M, N)
+        _, blk = Ch.circ_2_level(cL, L, M, N)
+        inv = Ch.invert_blocks_dda(blk, c1["alpha"], c1["sys"].dV)
+        return lambda v: Ch.mvp_circ_2_level(inv, v, L, M, N, c1["sys"].idx3)
+    blk, _ = Ch.circ_1_level(c1["Tref"], L, M, N, 0, "on")
+    inv = Ch.invert_blocks_dda(blk, c1["alpha"], c1["sys"].dV)
+    return lambda v: Ch.chan_mvp_idx(inv, v, L, M, N, c1["sys"].idx3)
+
+
+def test_rhs_and_alpha_match_oracle(c1):
+    sysd = c1["sys"]
+    assert abs(sysd.alpha - c1["alpha"]) <= 1e-15 * abs(c1["alpha"])
+    x = np.random.default_rng(0).normal(size=sysd.op.n) + 0j
+    assert _rel(sysd.op.mvp(x), c1["afun"](x)) <= 1e-12
+
+
+@pytest.mark.parametrize("levels", [0, 1, 2])
+def test_bicgstab_parity(c1, levels):
+    print(f"[bicgstab] levels={levels}", flush=True)
+    sysd = c1["sys"]
+    pc = sysd.preconditioner(levels) if levels else None
+    mfun = _oracle_pc(c1, levels) if levels else None
+    if pc is not None:
+        v = np.random.default_rng(1).normal(size=sysd.op.n) + 0j
+        assert _rel(pc.apply(v), mfun(v)) <= 1e-11
+    x, flag, relres, it, resvec = api.bicgstab(sysd.op, sysd.b, TOL, 2000, pc)
+    xr, flagr, relresr, itr, resvecr = OK.bicgstab(c1["afun"], sysd.b, TOL, 2000, mfun)
+    print(f"  gpu: flag={flag} iter={it} nres={len(resvec)} relres={relres:.3e}")
+    print(f"  cpu: flag={flagr} iter={itr} nres={len(resvecr)} relres={relresr:.3e}")
+    print(f"  solution rel diff = {_rel(x, xr):.3e}")
+    assert flag == flagr == 0
+    assert len(resvec) == len(resvecr) and it == itr
+    assert _rel(x, xr) <= 1e-10
+    # true FP64 residual
+    assert np.linalg.norm(c1["afun"](x) - sysd.b) / np.linalg.norm(sysd.b) <= TOL
+
+
+@pytest.mark.parametrize("levels", [0, 2])
+def test_gmres_parity(c1, levels):
+    print(f"[gmres] levels={levels}", flush=True)
+    sysd = c1["sys"]
+    pc = sysd.preconditioner(levels) if levels else None
+    mfun = _oracle_pc(c1, levels) if levels else None
+    restart = 50
+    x, flag, relres, it, resvec = api.gmres(sysd.op, sysd.b, restart, TOL, 40, pc)
+    xr, flagr, relresr, itr, resvecr = OK.gmres(c1["afun"], sysd.b, restart, TOL, 40, mfun)
+    print(f"  gpu: flag={flag} iter={it} nres={len(resvec)}  cpu: flag={flagr} iter={itr} nres={len(resvecr)}")
+    assert flag == flagr == 0
+    assert tuple(it) == tuple(itr) and len(resvec) == len(resvecr)
+    assert _rel(x, xr) <= 1e-10
+
+
+@pytest.mark.parametrize("side", ["L1", "R1"])
+def test_pgmres_parity(c1, side):
+    print(f"[pgmres] side={side}", flush=True)
+    sysd = c1["sys"]
+    pc = sysd.preconditioner(2)
+    mfun = _oracle_pc(c1, 2)
+    kw = {side: pc}
+    kwr = {side: mfun}
+    x, flag, relres, it, resvec = api.pgmres(sysd.op, sysd.b, 30, TOL, 20, **kw)
+    xr, flagr, relresr, itr, resvecr = OK.pgmres(c1["afun"], sysd.b, 30, TOL, 20, **kwr)
+    print(f"  gpu: flag={flag} iter={it} nres={len(resvec)}  cpu: flag={flagr} iter={itr} nres={len(resvecr)}")
+    assert flag == flagr == 0
+    assert tuple(it) == tuple(itr) and len(resvec) == len(resvecr)
+    assert np.allclose(resvec, resvecr, rtol=1e-8, atol=1e-14)
+    assert _rel(x, xr) <= 1e-10
+
+
+def test_zero_rhs_and_maxit_flag(c1):
+    sysd = c1["sys"]
+    x, flag, relres, it, resvec = api.bicgstab(sysd.op, np.zeros(sysd.op.n), TOL, 10)
+    assert flag == 0 and np.all(x == 0) and list(resvec) == [0.0]
+    x, flag, relres, it, resvec = api.bicgstab(sysd.op, sysd.b, 1e-14, 2)
+    xr, flagr, relresr, itr, resvecr = OK.bicgstab(c1["afun"], sysd.b, 1e-14, 2)
+    assert flag == flagr == 1 and len(resvec) == len(resvecr) == 5
+    assert it == itr
+    assert _rel(x, xr) <= 1e-10

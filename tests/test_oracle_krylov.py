@@ -1,0 +1,91 @@
+"""Known-answer tests of the oracle's Krylov solvers (MATLAB bicgstab/gmres semantics, pgmres.m):
+exact-arithmetic termination counts, numpy.linalg.solve, documented flags and resvec structure."""
+import numpy as np
+import pytest
+
+from oracle import krylov as K
+
+
+def _rand_system(n, seed, shift=4.0):
+    rng = np.random.default_rng(seed)
+    A = (rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))) / np.sqrt(n) + shift * np.eye(n)
+    b = rng.normal(size=n) + 1j * rng.normal(size=n)
+    return A, b
+
+
+def test_bicgstab_identity_converges_at_first_half_step():
+    b = np.arange(1, 6) + 1j
+    x, flag, relres, it, resvec = K.bicgstab(lambda v: v, b, 1e-10, 10)
+    assert flag == 0 and it == 0.5 and len(resvec) == 2
+    assert np.allclose(x, b) and resvec[0] == pytest.approx(np.linalg.norm(b))
+
+
+def test_bicgstab_exact_preconditioner():
+    A, b = _rand_system(30, 0)
+    Ainv = np.linalg.inv(A)
+    x, flag, relres, it, resvec = K.bicgstab(lambda v: A @ v, b, 1e-10, 10, mfun=lambda v: Ainv @ v)
+    assert flag == 0 and it == 0.5
+    assert np.allclose(x, np.linalg.solve(A, b), rtol=1e-9)
+
+
+def test_bicgstab_matches_direct_solve():
+    A, b = _rand_system(60, 1)
+    x, flag, relres, it, resvec = K.bicgstab(lambda v: A @ v, b, 1e-12, 200)
+    assert flag == 0 and relres <= 1e-12
+    assert np.linalg.norm(x - np.linalg.solve(A, b)) <= 1e-10 * np.linalg.norm(x)
+    # resvec: x0 plus one entry per half step, the last one the true residual
+    assert len(resvec) == int(2 * it) + 1
+    assert resvec[-1] == pytest.approx(np.linalg.norm(b - A @ x), rel=1e-6)
+
+
+def test_bicgstab_two_distinct_eigenvalues_terminates():
+    d = np.array([1.0] * 10 + [3.0] * 10, dtype=complex)
+    b = np.linspace(1, 2, 20) + 0.5j
+    x, flag, relres, it, resvec = K.bicgstab(lambda v: d * v, b, 1e-12, 50)
+    assert flag == 0 and it <= 2
+    assert np.allclose(x, b / d)
+
+
+def test_bicgstab_maxit_flag_and_zero_rhs():
+    A, b = _rand_system(40, 2, shift=0.3)
+    x, flag, relres, it, resvec = K.bicgstab(lambda v: A @ v, b, 1e-14, 2)
+    assert flag == 1 and len(resvec) == 5
+    # the returned iterate is the minimal-residual one
+    assert relres == pytest.approx(np.linalg.norm(b - A @ x) / np.linalg.norm(b), rel=1e-10)
+    x, flag, relres, it, resvec = K.bicgstab(lambda v: A @ v, np.zeros(40), 1e-6, 5)
+    assert flag == 0 and np.all(x == 0) and relres == 0 and list(resvec) == [0.0]
+
+
+def test_gmres_low_rank_update_terminates():
+    n, r = 50, 3
+    rng = np.random.default_rng(7)
+    U = rng.normal(size=(n, r)) + 1j * rng.normal(size=(n, r))
+    Vv = rng.normal(size=(n, r)) + 1j * rng.normal(size=(n, r))
+    A = np.eye(n) + U @ Vv.conj().T / n
+    b = rng.normal(size=n) + 0j
+    x, flag, relres, (outer, inner), resvec = K.gmres(lambda v: A @ v, b, n, 1e-12, 1)
+    assert flag == 0 and inner <= r + 1
+    assert np.allclose(x, np.linalg.solve(A, b), rtol=1e-10)
+
+
+def test_gmres_left_preconditioned_restarted():
+    A, b = _rand_system(60, 3, shift=2.0)
+    D = np.diag(A).copy()
+    x, flag, relres, it, resvec = K.gmres(lambda v: A @ v, b, 10, 1e-11, 20, mfun=lambda v: v / D)
+    assert flag == 0
+    assert np.linalg.norm(x - np.linalg.solve(A, b)) <= 1e-9 * np.linalg.norm(x)
+    # relres is the preconditioned relative residual
+    assert relres == pytest.approx(np.linalg.norm((b - A @ x) / D) / np.linalg.norm(b / D), rel=1e-3, abs=1e-14)
+
+
+@pytest.mark.parametrize("side", ["L1", "R1", None])
+def test_pgmres_matches_direct_solve(side):
+    A, b = _rand_system(60, 4, shift=2.0)
+    D = np.diag(A).copy()
+    pre = (lambda v: v / D)
+    kw = {} if side is None else {side: pre}
+    x, flag, relres, (p, outit), resvec = K.pgmres(lambda v: A @ v, b, 15, 1e-11, 30, **kw)
+    assert flag == 0 and relres <= 1e-11
+    assert np.linalg.norm(x - np.linalg.solve(A, b)) <= 1e-9 * np.linalg.norm(x)
+    assert resvec[0] == pytest.approx(1.0)
+    assert np.all(np.diff(resvec[:p + 1]) <= 1e-12)  # GMRES residuals are monotone within a cycle

@@ -1,0 +1,508 @@
+// Chan circulant preconditioners: build (circ_1_level.m, circ_2_level.m), explicit block inverses
+// (main_DDA.m:105-111, scatteringHumanHead.m:178-184) and apply (mvp_circ_2_level.m, chan_mvp_idx.m).
+//
+// Block orderings (identical to the reference's reshapes):
+//   2-level block (kx, ky): unknown s = z + N*c, size 3N
+//   1-level block kx      : unknown s = y + M*(z + N*c), size 3MN
+// Apply layouts on the device:
+//   2-level: inv[((ky*3N + r)*3N + s)*L + kx]   -- kx fastest, so a tile of consecutive kx streams
+//   1-level: inv[(kx*3MN + r)*3MN + s]          -- row-major per block, one wavefront per row
+#include "fft_device.hpp"
+#include "handles.hpp"
+
+#include <algorithm>
+
+namespace vs {
+
+extern __shared__ __align__(16) unsigned char vs_smem[];
+
+// ------------------------------------------------------------------------------------------------
+// Chan averaging along x and y
+// ------------------------------------------------------------------------------------------------
+// c(1) = A(1); c(i) = s (n+1-i)/n A(i) + (i-1)/n A(n-i+2) along x (circ_1_level.m:13-18,28-31)
+__global__ void k_chan_x(const c128* __restrict__ A, c128* __restrict__ c, int L, int64_t MN) {
+  const int64_t V = (int64_t)L * MN;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 6 * V) return;
+  const int p = (int)(t / V);
+  const int64_t q = t - p * V;
+  const int x = (int)(q % L);
+  const int64_t yz = q / L;
+  const double s = (p == 1 || p == 2) ? -1.0 : 1.0;
+  const c128* a = A + p * V + yz * L;
+  c128 v;
+  if (x == 0)
+    v = a[0];
+  else
+    v = (s * (double)(L - x) / L) * a[x] + ((double)x / L) * a[L - x];
+  c[t] = v;
+}
+
+// same along y on an L x M x N x 6 volume (circ_2_level.m:13-18,26-28,62-64)
+__global__ void k_chan_y(const c128* __restrict__ A, c128* __restrict__ c, int L, int M, int N) {
+  const int64_t V = (int64_t)L * M * N;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 6 * V) return;
+  const int p = (int)(t / V);
+  const int64_t q = t - p * V;
+  const int x = (int)(q % L);
+  const int y = (int)((q / L) % M);
+  const int64_t z = q / ((int64_t)L * M);
+  const double s = (p == 1 || p == 4) ? -1.0 : 1.0;
+  const c128* a = A + p * V + z * L * M + x;
+  c128 v;
+  if (y == 0)
+    v = a[0];
+  else
+    v = (s * (double)(M - y) / M) * a[(int64_t)y * L] + ((double)y / M) * a[(int64_t)(M - y) * L];
+  c[t] = v;
+}
+
+__global__ void k_negate_comp(c128* __restrict__ v, int64_t n) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) v[t] = -v[t];
+}
+
+// ------------------------------------------------------------------------------------------------
+// block assembly:  out = a*I - diag(w)*C   (with a = 0, w = -1 the raw C is produced)
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ int tcomp(int ca, int cb) {
+  // [xx xy xz; xy yy yz; xz yz zz] -> 0..5
+  const int lo = ca < cb ? ca : cb, hi = ca < cb ? cb : ca;
+  return lo == 0 ? hi : (lo == 1 ? 2 + hi : 5);
+}
+
+// 2-level: F = y-FFT'd Chan volume (L x M x N x 6, comp 2 not negated); block (kx,ky) at kx + L*ky
+__global__ void k_assemble2(const c128* __restrict__ F, int L, int M, int N, c128 a, const c128* __restrict__ w,
+                            c128 ws, c128* __restrict__ out) {
+  const int n = 3 * N;
+  const int64_t nn = (int64_t)n * n;
+  const int64_t nblk = (int64_t)L * M;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nblk * nn) return;
+  const int64_t blk = t / nn;
+  const int e = (int)(t - blk * nn);
+  const int ra = e % n, cb_ = e / n;  // row, column (column-major block)
+  const int kx = (int)(blk % L), ky = (int)(blk / L);
+  const int ca = ra / N, za = ra % N, cb = cb_ / N, zb = cb_ % N;
+  const int p = tcomp(ca, cb);
+  const int64_t V = (int64_t)L * M * N;
+  const c128* f = F + p * V + kx + (int64_t)L * ky;
+  const int64_t zs = (int64_t)L * M;
+  const int d = za - zb;
+  c128 val;
+  if (p == 2 || p == 4)  // toeplitz([c0 -c(2:N)], c): T(i,j) = -c(i-j) for i > j, c(j-i) otherwise
+    val = d > 0 ? -f[d * zs] : f[(-d) * zs];
+  else
+    val = f[(d >= 0 ? d : -d) * zs];
+  const c128 wr = w ? w[ra] : ws;
+  c128 o = -(wr * val);
+  if (ra == cb_) o = o + a;
+  out[t] = o;
+}
+
+// 1-level: cL = x-FFT'd Chan volume (L x M x N x 6); block kx, entry (i, j) of 3MN x 3MN, column-major
+__global__ void k_assemble1(const c128* __restrict__ cL, int L, int M, int N, c128 a, const c128* __restrict__ w,
+                            c128 ws, c128* __restrict__ out) {
+  const int64_t MN = (int64_t)M * N;
+  const int64_t n = 3 * MN;
+  const int64_t nn = n * n;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)L * nn) return;
+  const int kx = (int)(t / nn);
+  const int64_t e = t - kx * nn;
+  const int64_t i = e % n, j = e / n;
+  const int ca = (int)(i / MN), cb = (int)(j / MN);
+  int64_t il = i - ca * MN, jl = j - cb * MN;
+  const int p = tcomp(ca, cb);
+  double sgn = 1.0;
+  // xz: antisymmetrised from the upper triangle; yz: symmetrised from the upper triangle
+  if (il > jl && (p == 2 || p == 4)) {
+    const int64_t tmp = il;
+    il = jl;
+    jl = tmp;
+    if (p == 2) sgn = -1.0;
+  }
+  const int za = (int)(il / M), ya = (int)(il % M), zb = (int)(jl / M), yb = (int)(jl % M);
+  const int dz = za > zb ? za - zb : zb - za;
+  const int64_t V = (int64_t)L * MN;
+  const c128* c = cL + p * V + kx + (int64_t)L * M * dz;  // c[y * L] = chan_dz coefficient y
+  const int dy = ya - yb;
+  c128 val;
+  if (p == 1 || p == 4)  // toeplitz([c0 -c(2:M)], c)
+    val = dy > 0 ? -c[(int64_t)dy * L] : c[(int64_t)(-dy) * L];
+  else
+    val = c[(int64_t)(dy >= 0 ? dy : -dy) * L];
+  val = sgn * val;
+  const c128 wr = w ? w[i] : ws;
+  c128 o = -(wr * val);
+  if (i == j) o = o + a;
+  out[t] = o;
+}
+
+// ------------------------------------------------------------------------------------------------
+// batched Gauss-Jordan inversion with partial pivoting (column-major blocks, in place)
+// ------------------------------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void wave_argmax(T& best, int& bi) {
+  for (int off = 32; off >= 1; off >>= 1) {
+    const T ob = __shfl_xor(best, off);
+    const int oi = __shfl_xor(bi, off);
+    if (ob > best || (ob == best && oi < bi)) {
+      best = ob;
+      bi = oi;
+    }
+  }
+}
+
+// whole block in LDS, one workgroup per block
+__global__ __launch_bounds__(256) void k_gj_lds(c128* __restrict__ blocks, int n) {
+  c128* A = reinterpret_cast<c128*>(vs_smem);
+  c128* colk = A + n * n;
+  int* piv = reinterpret_cast<int*>(colk + n);
+  __shared__ int s_p;
+  __shared__ c128 s_pinv;
+  c128* g = blocks + (int64_t)blockIdx.x * n * n;
+  const int tid = threadIdx.x, nth = blockDim.x;
+  for (int e = tid; e < n * n; e += nth) A[e] = g[e];
+  __syncthreads();
+  for (int k = 0; k < n; ++k) {
+    if (tid < 64) {
+      double best = -1.0;
+      int bi = k;
+      for (int i = k + tid; i < n; i += 64) {
+        const double v = abs2(A[i + n * k]);
+        if (v > best) { best = v; bi = i; }
+      }
+      wave_argmax(best, bi);
+      if (tid == 0) {
+        s_p = bi;
+        piv[k] = bi;
+      }
+    }
+    __syncthreads();
+    const int p = s_p;
+    if (p != k)
+      for (int j = tid; j < n; j += nth) {
+        const c128 t = A[k + n * j];
+        A[k + n * j] = A[p + n * j];
+        A[p + n * j] = t;
+      }
+    __syncthreads();
+    for (int i = tid; i < n; i += nth) colk[i] = A[i + n * k];
+    if (tid == 0) s_pinv = cdiv(mk<double>(1, 0), A[k + n * k]);
+    __syncthreads();
+    const c128 pinv = s_pinv;
+    for (int j = tid; j < n; j += nth) A[k + n * j] = (j == k) ? pinv : A[k + n * j] * pinv;
+    __syncthreads();
+    for (int e = tid; e < n * n; e += nth) {
+      const int i = e % n, j = e / n;
+      if (i == k) continue;
+      const c128 f = colk[i];
+      if (j == k)
+        A[e] = -(f * A[k + n * k]);
+      else
+        A[e] = A[e] - f * A[k + n * j];
+    }
+    __syncthreads();
+  }
+  for (int k = n - 1; k >= 0; --k) {
+    const int p = piv[k];
+    if (p != k)
+      for (int i = tid; i < n; i += nth) {
+        const c128 t = A[i + n * k];
+        A[i + n * k] = A[i + n * p];
+        A[i + n * p] = t;
+      }
+    __syncthreads();
+  }
+  for (int e = tid; e < n * n; e += nth) g[e] = A[e];
+}
+
+// global-memory variant for blocks too large for LDS: one pivot kernel + one elimination kernel per step
+__global__ __launch_bounds__(256) void k_gj_pivot(c128* __restrict__ blocks, int64_t n, int64_t k,
+                                                  c128* __restrict__ colk, int* __restrict__ piv) {
+  __shared__ double s_best[4];
+  __shared__ int s_bi[4];
+  __shared__ int s_p;
+  __shared__ c128 s_pinv;
+  const int64_t blk = blockIdx.x;
+  c128* A = blocks + blk * n * n;
+  const int tid = threadIdx.x;
+  double best = -1.0;
+  int bi = (int)k;
+  for (int64_t i = k + tid; i < n; i += 256) {
+    const double v = abs2(A[i + n * k]);
+    if (v > best) { best = v; bi = (int)i; }
+  }
+  wave_argmax(best, bi);
+  if ((tid & 63) == 0) {
+    s_best[tid >> 6] = best;
+    s_bi[tid >> 6] = bi;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double b0 = s_best[0];
+    int i0 = s_bi[0];
+    for (int q = 1; q < 4; ++q)
+      if (s_best[q] > b0 || (s_best[q] == b0 && s_bi[q] < i0)) { b0 = s_best[q]; i0 = s_bi[q]; }
+    s_p = i0;
+    piv[blk * n + k] = i0;
+  }
+  __syncthreads();
+  const int64_t p = s_p;
+  if (p != k)
+    for (int64_t j = tid; j < n; j += 256) {
+      const c128 t = A[k + n * j];
+      A[k + n * j] = A[p + n * j];
+      A[p + n * j] = t;
+    }
+  __syncthreads();
+  for (int64_t i = tid; i < n; i += 256) colk[blk * n + i] = A[i + n * k];
+  if (tid == 0) s_pinv = cdiv(mk<double>(1, 0), A[k + n * k]);
+  __syncthreads();
+  const c128 pinv = s_pinv;
+  for (int64_t j = tid; j < n; j += 256) A[k + n * j] = (j == k) ? pinv : A[k + n * j] * pinv;
+}
+
+__global__ void k_gj_elim(c128* __restrict__ blocks, int64_t n, int64_t k, const c128* __restrict__ colk) {
+  const int64_t blk = blockIdx.y;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * n) return;
+  const int64_t i = e % n, j = e / n;
+  if (i == k) return;
+  c128* A = blocks + blk * n * n;
+  const c128 f = colk[blk * n + i];
+  if (j == k)
+    A[e] = -(f * A[k + n * k]);
+  else
+    A[e] = A[e] - f * A[k + n * j];
+}
+
+__global__ void k_gj_unswap(c128* __restrict__ blocks, int64_t n, const int* __restrict__ piv) {
+  const int64_t blk = blockIdx.y;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  c128* A = blocks + blk * n * n;
+  for (int64_t k = n - 1; k >= 0; --k) {
+    const int64_t p = piv[blk * n + k];
+    if (p != k) {
+      const c128 t = A[i + n * k];
+      A[i + n * k] = A[i + n * p];
+      A[i + n * p] = t;
+    }
+  }
+}
+
+// column-major blocks (kx + L*ky) -> 2-level apply layout, in precision To
+template <typename To>
+__global__ void k_relayout2(const c128* __restrict__ in, cplx<To>* __restrict__ out, int L, int M, int n) {
+  const int64_t nn = (int64_t)n * n;
+  const int64_t tot = (int64_t)L * M * nn;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= tot) return;
+  // t indexes the destination: ((ky*n + r)*n + s)*L + kx
+  const int kx = (int)(t % L);
+  int64_t q = t / L;
+  const int s = (int)(q % n);
+  q /= n;
+  const int r = (int)(q % n);
+  const int ky = (int)(q / n);
+  out[t] = ccast<To>(in[((int64_t)kx + (int64_t)L * ky) * nn + r + (int64_t)n * s]);
+}
+
+// column-major blocks -> 1-level apply layout (row-major per block)
+template <typename To>
+__global__ void k_relayout1(const c128* __restrict__ in, cplx<To>* __restrict__ out, int64_t nblk, int64_t n) {
+  const int64_t nn = n * n;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nblk * nn) return;
+  const int64_t b = t / nn, e = t - b * nn;
+  const int64_t r = e / n, s = e % n;
+  out[t] = ccast<To>(in[b * nn + r + n * s]);
+}
+
+// ------------------------------------------------------------------------------------------------
+// apply kernels
+// ------------------------------------------------------------------------------------------------
+// per (kx tile, ky): u <- Binv(kx,ky) u on the y-FFT'd volume, in place
+template <typename T>
+__global__ __launch_bounds__(256) void k_blockmv2(const cplx<T>* __restrict__ B, cplx<T>* __restrict__ U, int L, int M,
+                                                  int n, int W, int64_t rhs_stride) {
+  cplx<T>* u = reinterpret_cast<cplx<T>*>(vs_smem);
+  const int tiles = (L + W - 1) / W;
+  const int t = blockIdx.x % tiles, ky = blockIdx.x / tiles;
+  const int kx0 = t * W;
+  const int wv = min(W, L - kx0);
+  cplx<T>* Ub = U + (int64_t)blockIdx.y * rhs_stride + kx0 + (int64_t)L * ky;
+  const int64_t sstride = (int64_t)L * M;
+  for (int e = threadIdx.x; e < n * W; e += blockDim.x) {
+    const int w = e % W, s = e / W;
+    u[e] = w < wv ? Ub[s * sstride + w] : mk<T>(0, 0);
+  }
+  __syncthreads();
+  const cplx<T>* Bk = B + (int64_t)ky * n * n * L + kx0;
+  for (int e = threadIdx.x; e < n * W; e += blockDim.x) {
+    const int w = e % W, r = e / W;
+    if (w >= wv) continue;
+    const cplx<T>* br = Bk + (int64_t)r * n * L + w;
+    cplx<T> acc = mk<T>(0, 0);
+    for (int s = 0; s < n; ++s) acc = acc + br[(int64_t)s * L] * u[s * W + w];
+    Ub[r * sstride + w] = acc;
+  }
+}
+
+// per kx: U2(:, kx) = Binv(kx) U(:, kx), one wavefront per row
+template <typename T>
+__global__ __launch_bounds__(256) void k_blockmv1(const cplx<T>* __restrict__ B, const cplx<T>* __restrict__ U,
+                                                  cplx<T>* __restrict__ U2, int L, int n, int rows_per_wg,
+                                                  int64_t rhs_stride) {
+  cplx<T>* u = reinterpret_cast<cplx<T>*>(vs_smem);
+  const int kx = blockIdx.y;
+  const cplx<T>* Ub = U + (int64_t)blockIdx.z * rhs_stride + kx;
+  cplx<T>* U2b = U2 + (int64_t)blockIdx.z * rhs_stride + kx;
+  for (int j = threadIdx.x; j < n; j += blockDim.x) u[j] = Ub[(int64_t)j * L];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r0 = blockIdx.x * rows_per_wg;
+  for (int r = r0 + wave; r < min(n, r0 + rows_per_wg); r += 4) {
+    const cplx<T>* br = B + ((int64_t)kx * n + r) * n;
+    T ar = 0, ai = 0;
+    for (int s = lane; s < n; s += 64) {
+      const cplx<T> v = br[s] * u[s];
+      ar += v.x;
+      ai += v.y;
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+      ar += __shfl_xor(ar, off);
+      ai += __shfl_xor(ai, off);
+    }
+    if (lane == 0) U2b[(int64_t)r * L] = mk<T>(ar, ai);
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// host drivers
+// ------------------------------------------------------------------------------------------------
+static unsigned nblocks(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+// Toep (device, FP64) -> circ_L (device): Chan in x + FFT along x
+void build_circ_L(vs_ctx* ctx, const c128* toep, c128* cL, int L, int M, int N) {
+  const int64_t V = (int64_t)L * M * N;
+  hipLaunchKernelGGL(k_chan_x, dim3(nblocks(6 * V)), dim3(256), 0, ctx->stream, toep, cL, L, (int64_t)M * N);
+  VS_HIP(hipGetLastError());
+  launch_row_fft_dense(ctx, VS_FP64, cL, (int64_t)M * N * 6, L, L, L, L, L, false, 1.0, 1, 0);
+}
+
+// circ_L -> F (device): Chan in y + FFT along y (component 2 not negated)
+void build_circ_M(vs_ctx* ctx, const c128* cL, c128* F, int L, int M, int N) {
+  const int64_t V = (int64_t)L * M * N;
+  hipLaunchKernelGGL(k_chan_y, dim3(nblocks(6 * V)), dim3(256), 0, ctx->stream, cL, F, L, M, N);
+  VS_HIP(hipGetLastError());
+  launch_col_fft(ctx, VS_FP64, F, L, L, (int64_t)N * 6, (int64_t)L * M, M, M, M, false, 1.0, 1, 0);
+}
+
+void assemble2(vs_ctx* ctx, const c128* F, int L, int M, int N, c128 a, const c128* w, c128 ws, c128* out) {
+  const int64_t tot = (int64_t)L * M * 9 * N * N;
+  hipLaunchKernelGGL(k_assemble2, dim3(nblocks(tot)), dim3(256), 0, ctx->stream, F, L, M, N, a, w, ws, out);
+  VS_HIP(hipGetLastError());
+}
+
+void assemble1(vs_ctx* ctx, const c128* cL, int L, int M, int N, c128 a, const c128* w, c128 ws, c128* out) {
+  const int64_t n = 3LL * M * N;
+  const int64_t tot = (int64_t)L * n * n;
+  hipLaunchKernelGGL(k_assemble1, dim3(nblocks(tot)), dim3(256), 0, ctx->stream, cL, L, M, N, a, w, ws, out);
+  VS_HIP(hipGetLastError());
+}
+
+void invert_blocks_dev(vs_ctx* ctx, c128* blocks, int64_t nblk, int64_t n) {
+  const size_t lds = sizeof(c128) * (size_t)(n * n + n) + sizeof(int) * (size_t)n;
+  if (lds <= 65536) {
+    hipLaunchKernelGGL(k_gj_lds, dim3((unsigned)nblk), dim3(256), lds, ctx->stream, blocks, (int)n);
+    VS_HIP(hipGetLastError());
+    return;
+  }
+  DevBuf colk(sizeof(c128) * nblk * n), piv(sizeof(int) * nblk * n);
+  for (int64_t k = 0; k < n; ++k) {
+    hipLaunchKernelGGL(k_gj_pivot, dim3((unsigned)nblk), dim3(256), 0, ctx->stream, blocks, n, k, colk.as<c128>(),
+                       piv.as<int>());
+    hipLaunchKernelGGL(k_gj_elim, dim3(nblocks(n * n), (unsigned)nblk), dim3(256), 0, ctx->stream, blocks, n, k,
+                       colk.as<c128>());
+  }
+  hipLaunchKernelGGL(k_gj_unswap, dim3(nblocks(n), (unsigned)nblk), dim3(256), 0, ctx->stream, blocks, n,
+                     piv.as<int>());
+  VS_HIP(hipGetLastError());
+  VS_HIP(hipStreamSynchronize(ctx->stream));  // colk/piv are released on return
+}
+
+// column-major FP64 inverse blocks (device) -> apply layout(s) of pc
+void pc_store_inverse(vs_pc* pc, const c128* inv) {
+  vs_ctx* ctx = pc->ctx;
+  const int64_t n = pc->nb;
+  const int64_t nblk = pc->levels == 2 ? (int64_t)pc->L * pc->M : pc->L;
+  const int64_t tot = nblk * n * n;
+  auto go = [&](auto tag, DevBuf& dst) {
+    using To = decltype(tag);
+    dst.ensure(sizeof(cplx<To>) * tot);
+    if (pc->levels == 2)
+      hipLaunchKernelGGL((k_relayout2<To>), dim3(nblocks(tot)), dim3(256), 0, ctx->stream, inv, dst.as<cplx<To>>(),
+                         pc->L, pc->M, (int)n);
+    else
+      hipLaunchKernelGGL((k_relayout1<To>), dim3(nblocks(tot)), dim3(256), 0, ctx->stream, inv, dst.as<cplx<To>>(),
+                         nblk, n);
+    VS_HIP(hipGetLastError());
+  };
+  if (pc->precisions & VS_FP64) go(double(0), pc->inv64);
+  if (pc->precisions & VS_FP32) go(float(0), pc->inv32);
+}
+
+template <typename T>
+static void pc_apply_impl(vs_pc* pc, const cplx<T>* x, cplx<T>* y, int nrhs) {
+  vs_ctx* ctx = pc->ctx;
+  const int prec = sizeof(T) == 8 ? VS_FP64 : VS_FP32;
+  const int L = pc->L, M = pc->M, N = pc->N;
+  const int64_t V3 = 3 * pc->V;
+  const cplx<T>* B = sizeof(T) == 8 ? pc->inv64.as<cplx<T>>() : pc->inv32.as<cplx<T>>();
+  VS_CHECK(B != nullptr, VS_ERR_ARG, "preconditioner holds no blocks in the requested precision");
+  pc->U.ensure(sizeof(cplx<T>) * V3 * nrhs);
+  cplx<T>* U = pc->U.as<cplx<T>>();
+  const int* map3 = pc->map3.as<int>();
+  const int64_t lines = (int64_t)M * N * 3;
+  launch_row_fft_gather(ctx, prec, x, map3, pc->nunk, U, L, lines, nrhs, V3);
+  if (pc->levels == 2) {
+    launch_col_fft(ctx, prec, U, L, L, (int64_t)N * 3, (int64_t)L * M, M, M, M, false, 1.0, nrhs, V3);
+    const int n = pc->nb;
+    int W = std::min(64, L);
+    while (W > 1 && (size_t)n * W * sizeof(cplx<T>) > 65536) W /= 2;
+    while (W > 8 && (int64_t)((L + W - 1) / W) * M < 256) W /= 2;
+    const size_t lds = sizeof(cplx<T>) * (size_t)n * W;
+    const int tiles = (L + W - 1) / W;
+    hipLaunchKernelGGL((k_blockmv2<T>), dim3((unsigned)(tiles * M), (unsigned)nrhs), dim3(256), lds, ctx->stream, B,
+                       U, L, M, n, W, V3);
+    VS_HIP(hipGetLastError());
+    launch_col_fft(ctx, prec, U, L, L, (int64_t)N * 3, (int64_t)L * M, M, M, M, true, 1.0, nrhs, V3);
+    launch_row_ifft_scatter(ctx, prec, U, map3, pc->nunk, y, L, lines, 1.0 / ((double)L * M), nrhs, V3);
+  } else {
+    pc->U2.ensure(sizeof(cplx<T>) * V3 * nrhs);
+    cplx<T>* U2 = pc->U2.as<cplx<T>>();
+    const int n = pc->nb;
+    const int rows = 16;
+    const size_t lds = sizeof(cplx<T>) * (size_t)n;
+    hipLaunchKernelGGL((k_blockmv1<T>), dim3((unsigned)((n + rows - 1) / rows), (unsigned)L, (unsigned)nrhs), dim3(256),
+                       lds, ctx->stream, B, U, U2, L, n, rows, V3);
+    VS_HIP(hipGetLastError());
+    launch_row_ifft_scatter(ctx, prec, U2, map3, pc->nunk, y, L, lines, 1.0 / (double)L, nrhs, V3);
+  }
+}
+
+void pc_apply_dev(vs_pc* pc, const void* x, void* y, int precision, int nrhs) {
+  VS_CHECK(x != y, VS_ERR_ARG, "preconditioner input and output must differ");
+  if (precision == VS_FP64)
+    pc_apply_impl<double>(pc, (const c128*)x, (c128*)y, nrhs);
+  else if (precision == VS_FP32)
+    pc_apply_impl<float>(pc, (const c64*)x, (c64*)y, nrhs);
+  else
+    throw Error{VS_ERR_ARG, "precision must be VS_FP64 or VS_FP32"};
+}
+
+}  // namespace vs

@@ -1,0 +1,314 @@
+// Krylov vector kernels and the device-resident solvers.
+//
+// Reductions are deterministic: a fixed-size grid writes per-workgroup partial sums (accumulated in
+// FP64 whatever the vector precision), and one workgroup combines them in a fixed order.  The host
+// reads the reduced scalars once per Krylov half step; all vectors stay in HBM.
+//
+//  bicgstab : MATLAB bicgstab semantics (as called at main_DDA.m:82,120); see solve_bicgstab
+//  gmres    : MATLAB gmres(A,b,restart,tol,maxit,M) -- left preconditioned, restarted; Arnoldi by
+//             classical Gram-Schmidt applied twice, Givens rotations on the host
+//  pgmres   : pgmres.m:70-180 / pgmres_iter.m:22-150 -- one-pass CGS, L1 or R1 preconditioner
+#include "handles.hpp"
+#include "solver.hpp"
+
+#include <algorithm>
+#include <cfloat>
+#include <cstring>
+
+namespace vs {
+
+constexpr int RED_THREADS = 256;
+constexpr int RED_MAX_K = 8;
+
+static int red_grid(int64_t n) {
+  int64_t g = (n + RED_THREADS * 4 - 1) / (RED_THREADS * 4);
+  return (int)std::max<int64_t>(1, std::min<int64_t>(g, 1024));
+}
+
+// block-wide sum of K complex doubles held per thread; result valid in thread 0
+template <int K>
+__device__ __forceinline__ void block_sum(double (&re)[K], double (&im)[K], c128* out) {
+  __shared__ double sre[K][RED_THREADS / 64], sim[K][RED_THREADS / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    for (int off = 32; off >= 1; off >>= 1) {
+      re[k] += __shfl_xor(re[k], off);
+      im[k] += __shfl_xor(im[k], off);
+    }
+    if (lane == 0) {
+      sre[k][wave] = re[k];
+      sim[k][wave] = im[k];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      double a = 0, b = 0;
+      for (int w = 0; w < RED_THREADS / 64; ++w) {
+        a += sre[k][w];
+        b += sim[k][w];
+      }
+      out[k] = mk<double>(a, b);
+    }
+  }
+}
+
+template <typename T>
+struct DotArgs {
+  const cplx<T>* a[RED_MAX_K];
+  const cplx<T>* b[RED_MAX_K];
+};
+
+// partial[blk*K + k] = sum_i conj(a_k[i]) b_k[i] over this workgroup's grid-stride range
+template <typename T, int K>
+__global__ __launch_bounds__(RED_THREADS) void k_dots(DotArgs<T> args, int64_t n, c128* partial) {
+  double re[K], im[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) re[k] = im[k] = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * RED_THREADS) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const cplx<T> a = args.a[k][i], b = args.b[k][i];
+      re[k] += (double)a.x * b.x + (double)a.y * b.y;
+      im[k] += (double)a.x * b.y - (double)a.y * b.x;
+    }
+  }
+  c128 tot[K];
+  block_sum<K>(re, im, tot);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < K; ++k) partial[blockIdx.x * K + k] = tot[k];
+}
+
+// out[k] = sum over nblk partials, fixed order
+__global__ __launch_bounds__(RED_THREADS) void k_reduce_partials(const c128* partial, int nblk, int K, c128* out) {
+  __shared__ double sre[RED_THREADS], sim[RED_THREADS];
+  for (int k = 0; k < K; ++k) {
+    double a = 0, b = 0;
+    for (int i = threadIdx.x; i < nblk; i += RED_THREADS) {
+      a += partial[i * K + k].x;
+      b += partial[i * K + k].y;
+    }
+    sre[threadIdx.x] = a;
+    sim[threadIdx.x] = b;
+    __syncthreads();
+    for (int s = RED_THREADS / 2; s > 0; s >>= 1) {
+      if (threadIdx.x < s) {
+        sre[threadIdx.x] += sre[threadIdx.x + s];
+        sim[threadIdx.x] += sim[threadIdx.x + s];
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[k] = mk<double>(sre[0], sim[0]);
+    __syncthreads();
+  }
+}
+
+// element-wise updates, each optionally accumulating ||result||^2 partials
+enum UpdOp : int {
+  UPD_AXPBY = 0,    // y = al x + be y
+  UPD_BICG_P = 1,   // p = r + be (p - al v)           (x0=r, x1=v, y=p)
+  UPD_HALF = 2,     // y = x + al x1 ; z = x2 - al x3   (xhalf = x + alpha ph ; s = r - alpha v)
+  UPD_SUB = 3,      // y = x - y                        (r = b - Ax)
+  UPD_COPY = 4,     // y = x
+  UPD_SCALE = 5,    // y = al x
+};
+
+template <typename T>
+struct UpdArgs {
+  const cplx<T>* x0;
+  const cplx<T>* x1;
+  const cplx<T>* x2;
+  const cplx<T>* x3;
+  cplx<T>* y;
+  cplx<T>* z;
+  cplx<T> al, be;
+};
+
+template <typename T>
+__global__ __launch_bounds__(RED_THREADS) void k_update(int op, UpdArgs<T> a, int64_t n, int want_norm, c128* partial) {
+  double re[1] = {0.0}, im[1] = {0.0};
+  for (int64_t i = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * RED_THREADS) {
+    cplx<T> out;
+    switch (op) {
+      case UPD_AXPBY: out = a.al * a.x0[i] + a.be * a.y[i]; a.y[i] = out; break;
+      case UPD_BICG_P: out = a.x0[i] + a.be * (a.y[i] - a.al * a.x1[i]); a.y[i] = out; break;
+      case UPD_HALF: {
+        a.y[i] = a.x0[i] + a.al * a.x1[i];
+        out = a.x2[i] - a.al * a.x3[i];
+        a.z[i] = out;
+        break;
+      }
+      case UPD_SUB: out = a.x0[i] - a.y[i]; a.y[i] = out; break;
+      case UPD_COPY: out = a.x0[i]; a.y[i] = out; break;
+      default: out = a.al * a.x0[i]; a.y[i] = out; break;
+    }
+    re[0] += (double)out.x * out.x + (double)out.y * out.y;
+  }
+  if (want_norm) {
+    c128 tot[1];
+    block_sum<1>(re, im, tot);
+    if (threadIdx.x == 0) partial[blockIdx.x] = tot[0];
+  }
+}
+
+// GMRES helpers: partial[blk*(k) + j] = sum_i conj(V_j[i]) w[i]  for j < k
+template <typename T>
+__global__ __launch_bounds__(RED_THREADS) void k_mdot(const cplx<T>* V, int64_t ldv, int k, const cplx<T>* w, int64_t n,
+                                                      c128* partial) {
+  __shared__ double sre[RED_THREADS / 64], sim[RED_THREADS / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int j = 0; j < k; ++j) {
+    double re = 0, im = 0;
+    const cplx<T>* v = V + (int64_t)j * ldv;
+    for (int64_t i = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * RED_THREADS) {
+      const cplx<T> a = v[i], b = w[i];
+      re += (double)a.x * b.x + (double)a.y * b.y;
+      im += (double)a.x * b.y - (double)a.y * b.x;
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+      re += __shfl_xor(re, off);
+      im += __shfl_xor(im, off);
+    }
+    if (lane == 0) {
+      sre[wave] = re;
+      sim[wave] = im;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double a = 0, b = 0;
+      for (int q = 0; q < RED_THREADS / 64; ++q) {
+        a += sre[q];
+        b += sim[q];
+      }
+      partial[(int64_t)blockIdx.x * k + j] = mk<double>(a, b);
+    }
+    __syncthreads();
+  }
+}
+
+// w += sgn * sum_j V_j h_j  (h in FP64 on the device)
+template <typename T>
+__global__ __launch_bounds__(RED_THREADS) void k_gemv_acc(const cplx<T>* V, int64_t ldv, int k, const c128* h, double sgn,
+                                                          cplx<T>* w, int64_t n) {
+  __shared__ cplx<T> hs[512];
+  for (int j = threadIdx.x; j < k; j += RED_THREADS) hs[j] = ccast<T>(sgn * h[j]);
+  __syncthreads();
+  for (int64_t i = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * RED_THREADS) {
+    cplx<T> acc = w[i];
+    for (int j = 0; j < k; ++j) acc = acc + V[(int64_t)j * ldv + i] * hs[j];
+    w[i] = acc;
+  }
+}
+
+template <typename To, typename Ti>
+__global__ void k_cvt(const cplx<Ti>* __restrict__ in, cplx<To>* __restrict__ out, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = ccast<To>(in[i]);
+}
+
+// ------------------------------------------------------------------------------------------------
+// host-side vector engine
+// ------------------------------------------------------------------------------------------------
+template <typename T>
+Engine<T>::Engine(vs_ctx* c, int64_t n_) : ctx(c), n(n_), grid(red_grid(n_)) {
+  ctx->red.ensure(sizeof(c128) * 1024 * 512);
+  ctx->red_out.ensure(sizeof(c128) * 512);
+  if (!ctx->pinned) VS_HIP(hipHostMalloc(&ctx->pinned, sizeof(c128) * 512, hipHostMallocDefault));
+  if (!ctx->pinned_up) VS_HIP(hipHostMalloc(&ctx->pinned_up, sizeof(c128) * 512, hipHostMallocDefault));
+}
+
+template <typename T>
+void Engine<T>::fetch(int K, c128* out) {
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(RED_THREADS), 0, ctx->stream, ctx->red.as<c128>(), grid, K,
+                     ctx->red_out.as<c128>());
+  VS_HIP(hipGetLastError());
+  VS_HIP(hipMemcpyAsync(ctx->pinned, ctx->red_out.p, sizeof(c128) * K, hipMemcpyDeviceToHost, ctx->stream));
+  VS_HIP(hipStreamSynchronize(ctx->stream));
+  std::memcpy(out, ctx->pinned, sizeof(c128) * K);
+}
+
+template <typename T>
+void Engine<T>::dots(int K, const cplx<T>* const* a, const cplx<T>* const* b, c128* out) {
+  DotArgs<T> args{};
+  for (int k = 0; k < K; ++k) {
+    args.a[k] = a[k];
+    args.b[k] = b[k];
+  }
+  c128* part = ctx->red.as<c128>();
+  switch (K) {
+    case 1: hipLaunchKernelGGL((k_dots<T, 1>), dim3(grid), dim3(RED_THREADS), 0, ctx->stream, args, n, part); break;
+    case 2: hipLaunchKernelGGL((k_dots<T, 2>), dim3(grid), dim3(RED_THREADS), 0, ctx->stream, args, n, part); break;
+    case 3: hipLaunchKernelGGL((k_dots<T, 3>), dim3(grid), dim3(RED_THREADS), 0, ctx->stream, args, n, part); break;
+    case 4: hipLaunchKernelGGL((k_dots<T, 4>), dim3(grid), dim3(RED_THREADS), 0, ctx->stream, args, n, part); break;
+    default: throw Error{VS_ERR_INTERNAL, "dots: K out of range"};
+  }
+  VS_HIP(hipGetLastError());
+  fetch(K, out);
+}
+
+template <typename T>
+double Engine<T>::update(int op, const cplx<T>* x0, const cplx<T>* x1, const cplx<T>* x2, const cplx<T>* x3,
+                         cplx<T>* y, cplx<T>* z, c128 al, c128 be, bool want_norm) {
+  UpdArgs<T> a{x0, x1, x2, x3, y, z, ccast<T>(al), ccast<T>(be)};
+  hipLaunchKernelGGL((k_update<T>), dim3(grid), dim3(RED_THREADS), 0, ctx->stream, op, a, n, want_norm ? 1 : 0,
+                     ctx->red.as<c128>());
+  VS_HIP(hipGetLastError());
+  if (!want_norm) return 0.0;
+  c128 r;
+  fetch(1, &r);
+  return std::sqrt(r.x);
+}
+
+template <typename T>
+double Engine<T>::norm(const cplx<T>* x) {
+  c128 r;
+  const cplx<T>* a[1] = {x};
+  dots(1, a, a, &r);
+  return std::sqrt(r.x);
+}
+
+template <typename T>
+void Engine<T>::copy(const cplx<T>* x, cplx<T>* y) {
+  VS_HIP(hipMemcpyAsync(y, x, sizeof(cplx<T>) * n, hipMemcpyDeviceToDevice, ctx->stream));
+}
+
+template <typename T>
+void Engine<T>::mdot(const cplx<T>* V, int64_t ldv, int k, const cplx<T>* w, c128* out) {
+  VS_CHECK(k <= 512, VS_ERR_UNSUPPORTED, "Krylov basis larger than 512 vectors");
+  hipLaunchKernelGGL((k_mdot<T>), dim3(grid), dim3(RED_THREADS), 0, ctx->stream, V, ldv, k, w, n, ctx->red.as<c128>());
+  VS_HIP(hipGetLastError());
+  fetch(k, out);
+}
+
+template <typename T>
+void Engine<T>::gemv_acc(const cplx<T>* V, int64_t ldv, int k, const c128* h_host, double sgn, cplx<T>* w) {
+  VS_CHECK(k <= 512, VS_ERR_UNSUPPORTED, "Krylov basis larger than 512 vectors");
+  // the staging buffer may still feed the previous upload: drain the stream before overwriting it
+  VS_HIP(hipStreamSynchronize(ctx->stream));
+  std::memcpy(ctx->pinned_up, h_host, sizeof(c128) * k);
+  VS_HIP(hipMemcpyAsync(ctx->red_out.p, ctx->pinned_up, sizeof(c128) * k, hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL((k_gemv_acc<T>), dim3(grid), dim3(RED_THREADS), 0, ctx->stream, V, ldv, k, ctx->red_out.as<c128>(),
+                     sgn, w, n);
+  VS_HIP(hipGetLastError());
+}
+
+template struct Engine<float>;
+template struct Engine<double>;
+
+void convert_dev(vs_ctx* ctx, const void* in, int pin, void* out, int pout, int64_t n) {
+  if (pin == pout) {
+    VS_HIP(hipMemcpyAsync(out, in, (pin == VS_FP64 ? sizeof(c128) : sizeof(c64)) * n, hipMemcpyDeviceToDevice,
+                          ctx->stream));
+    return;
+  }
+  if (pin == VS_FP64)
+    hipLaunchKernelGGL((k_cvt<float, double>), dim3(1024), dim3(256), 0, ctx->stream, (const c128*)in, (c64*)out, n);
+  else
+    hipLaunchKernelGGL((k_cvt<double, float>), dim3(1024), dim3(256), 0, ctx->stream, (const c64*)in, (c128*)out, n);
+  VS_HIP(hipGetLastError());
+}
+
+}  // namespace vs

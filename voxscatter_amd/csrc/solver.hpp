@@ -1,0 +1,59 @@
+// Host-side solver engine over device vectors.
+#pragma once
+
+#include <functional>
+
+#include "handles.hpp"
+
+namespace vs {
+
+// Vector engine bound to a context: fused update kernels and deterministic reductions of length-n
+// complex vectors of precision T.
+template <typename T>
+struct Engine {
+  vs_ctx* ctx;
+  int64_t n;
+  int grid;
+  Engine(vs_ctx* c, int64_t n_);
+  void fetch(int K, c128* out);
+  void dots(int K, const cplx<T>* const* a, const cplx<T>* const* b, c128* out);  // sum conj(a) b
+  double update(int op, const cplx<T>* x0, const cplx<T>* x1, const cplx<T>* x2, const cplx<T>* x3, cplx<T>* y,
+                cplx<T>* z, c128 al, c128 be, bool want_norm);
+  double norm(const cplx<T>* x);
+  void copy(const cplx<T>* x, cplx<T>* y);
+  void mdot(const cplx<T>* V, int64_t ldv, int k, const cplx<T>* w, c128* out);
+  void gemv_acc(const cplx<T>* V, int64_t ldv, int k, const c128* h_host, double sgn, cplx<T>* w);
+};
+
+enum { UPD_AXPBY_ = 0, UPD_BICG_P_ = 1, UPD_HALF_ = 2, UPD_SUB_ = 3, UPD_COPY_ = 4, UPD_SCALE_ = 5 };
+
+// device linear operator y = A x and preconditioner y = M^{-1} x (M may be empty)
+template <typename T>
+struct LinOp {
+  std::function<void(const cplx<T>*, cplx<T>*)> A;
+  std::function<void(const cplx<T>*, cplx<T>*)> M;
+};
+
+struct SolveResult {
+  int flag = 1;
+  double relres = 0;
+  double iter = 0;    // bicgstab: possibly x.5
+  int outer = 0, inner = 0;
+  std::vector<double> resvec;
+};
+
+template <typename T>
+SolveResult solve_bicgstab(Engine<T>& E, const LinOp<T>& op, const cplx<T>* b, cplx<T>* x, bool has_x0, double tol,
+                           int maxit);
+template <typename T>
+SolveResult solve_gmres(Engine<T>& E, const LinOp<T>& op, const cplx<T>* b, cplx<T>* x, bool has_x0, int restart,
+                        double tol, int maxit);
+template <typename T>
+SolveResult solve_pgmres(Engine<T>& E, const LinOp<T>& op, int side, const cplx<T>* b, cplx<T>* x, bool has_x0,
+                         int restart, double tol, int maxit);
+
+void convert_dev(vs_ctx* ctx, const void* in, int pin, void* out, int pout, int64_t n);
+void mvp_dev(vs_op* op, const void* x, void* y, int transp, int precision, int nrhs);
+void pc_apply_dev(vs_pc* pc, const void* x, void* y, int precision, int nrhs);
+
+}  // namespace vs

@@ -1,0 +1,178 @@
+// Internal declarations of the voxscatter HIP library (gfx950).
+//
+// Conventions shared by every translation unit:
+//  * complex data are interleaved (re, im) pairs of float or double (cplx<T>);
+//  * volumes are MATLAB-ordered: x fastest, then y, then z, then component;
+//  * the Krylov unknown vector is the compressed [Jx(idx); Jy(idx); Jz(idx)] (main_DDA.m:72), addressed
+//    through map3: map3[g] = position of global unknown g = c*L*M*N + voxel in the compressed vector,
+//    or -1 for air.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/voxscatter.h"
+
+namespace vs {
+
+// ------------------------------------------------------------------------------------------------
+// complex arithmetic
+// ------------------------------------------------------------------------------------------------
+template <typename T>
+struct alignas(2 * sizeof(T)) cplx {
+  T x, y;
+};
+
+template <typename T> __host__ __device__ __forceinline__ cplx<T> mk(T a, T b) { cplx<T> r; r.x = a; r.y = b; return r; }
+template <typename T> __host__ __device__ __forceinline__ cplx<T> operator+(cplx<T> a, cplx<T> b) { return mk<T>(a.x + b.x, a.y + b.y); }
+template <typename T> __host__ __device__ __forceinline__ cplx<T> operator-(cplx<T> a, cplx<T> b) { return mk<T>(a.x - b.x, a.y - b.y); }
+template <typename T> __host__ __device__ __forceinline__ cplx<T> operator-(cplx<T> a) { return mk<T>(-a.x, -a.y); }
+template <typename T> __host__ __device__ __forceinline__ cplx<T> operator*(cplx<T> a, cplx<T> b) {
+  return mk<T>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename T> __host__ __device__ __forceinline__ cplx<T> operator*(T s, cplx<T> a) { return mk<T>(s * a.x, s * a.y); }
+template <typename T> __host__ __device__ __forceinline__ cplx<T> cconj(cplx<T> a) { return mk<T>(a.x, -a.y); }
+// a * conj(b)
+template <typename T> __host__ __device__ __forceinline__ cplx<T> mulconj(cplx<T> a, cplx<T> b) {
+  return mk<T>(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+template <typename T> __host__ __device__ __forceinline__ T abs2(cplx<T> a) { return a.x * a.x + a.y * a.y; }
+template <typename T> __host__ __device__ __forceinline__ cplx<T> cdiv(cplx<T> a, cplx<T> b) {
+  // Smith's algorithm: no overflow for moderate ranges, exact for real divisors
+  if (fabs(b.x) >= fabs(b.y)) {
+    T r = b.y / b.x, den = b.x + b.y * r;
+    return mk<T>((a.x + a.y * r) / den, (a.y - a.x * r) / den);
+  } else {
+    T r = b.x / b.y, den = b.x * r + b.y;
+    return mk<T>((a.x * r + a.y) / den, (a.y * r - a.x) / den);
+  }
+}
+template <typename To, typename Ti> __host__ __device__ __forceinline__ cplx<To> ccast(cplx<Ti> a) {
+  return mk<To>((To)a.x, (To)a.y);
+}
+
+using c64 = cplx<float>;
+using c128 = cplx<double>;
+
+// ------------------------------------------------------------------------------------------------
+// errors
+// ------------------------------------------------------------------------------------------------
+void set_error(const std::string& msg);
+const char* last_error();
+
+struct Error {
+  int code;
+  std::string msg;
+};
+
+#define VS_HIP(call)                                                                            \
+  do {                                                                                          \
+    hipError_t e_ = (call);                                                                     \
+    if (e_ != hipSuccess)                                                                       \
+      throw ::vs::Error{VS_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)};         \
+  } while (0)
+
+#define VS_CHECK(cond, code, msg)                                                               \
+  do {                                                                                          \
+    if (!(cond)) throw ::vs::Error{(code), (msg)};                                              \
+  } while (0)
+
+// run body, convert exceptions into a status code + thread-local message
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    return VS_OK;
+  } catch (const Error& e) {
+    set_error(e.msg);
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_error("host allocation failed");
+    return VS_ERR_OOM;
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return VS_ERR_INTERNAL;
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// device memory
+// ------------------------------------------------------------------------------------------------
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t n) { alloc(n); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void alloc(size_t n);
+  void ensure(size_t n) { if (n > bytes) { release(); alloc(n); } }
+  void release();
+  template <typename U> U* as() const { return reinterpret_cast<U*>(p); }
+};
+
+// ------------------------------------------------------------------------------------------------
+// FFT plans: radix factorisation + twiddle table exp(-2 pi i k / n), k < n
+// ------------------------------------------------------------------------------------------------
+constexpr int FFT_MAX_STAGES = 16;
+constexpr int FFT_THREADS = 256;   // threads per workgroup of every FFT kernel
+constexpr int FFT_EMAX = 16;       // complex elements per thread held in LDS tile
+
+template <typename T>
+struct FftDev {
+  int n;
+  int nst;
+  int rad[FFT_MAX_STAGES];
+  const cplx<T>* tw;
+};
+
+struct FftPlanHost {
+  int n = 0;
+  std::vector<int> radices;
+  DevBuf tw64, tw32;
+  template <typename T> FftDev<T> dev() const;
+};
+
+// ------------------------------------------------------------------------------------------------
+// context
+// ------------------------------------------------------------------------------------------------
+}  // namespace vs
+
+struct vs_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::map<int, std::unique_ptr<vs::FftPlanHost>> plans;
+  vs::DevBuf red;        // reduction partials
+  vs::DevBuf red_out;    // reduced scalars
+  void* pinned = nullptr;     // host staging for reduced scalars (device -> host)
+  void* pinned_up = nullptr;  // host staging for small coefficient uploads (host -> device)
+  const vs::FftPlanHost& plan(int n);
+};
+
+namespace vs {
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    hipGetDevice(&prev);
+    if (prev != dev) hipSetDevice(dev);
+  }
+  ~DeviceGuard() {
+    if (prev >= 0) hipSetDevice(prev);
+  }
+};
+
+std::vector<int> fft_factor(int n);
+}  // namespace vs
