@@ -1,0 +1,194 @@
+"""Benchmark of the solve loop on one MI355X.
+
+Workload: level-3 Koch-snowflake prism on a 128 x 128 x 16 grid (k R = 20, k d = 0.3125,
+m = 1.31 + 0.01i, incidence +x, polarisation +z), DDA with the LDR polarisability.  One step is one
+complete solve, right-hand side resident in HBM to solution resident in HBM with the FP64 relative
+residual norm(b - A x)/norm(b) <= 1e-6 verified: BiCGStab with the 2-level Chan circulant
+preconditioner in complex FP32 inside FP64 iterative refinement.  The plain FP64 solve (BiCGStab +
+FP64 2-level preconditioner) of the same system is timed beside it.
+
+Prints one JSON line (see ``main``).  ``--gpus N`` with N > 1 exits: the build targets one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+TOL = 1e-6
+
+
+def _parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-iters", type=int, default=4, help="BiCGStab iterations timed for the CPU baseline")
+    ap.add_argument("--inner-tol", type=float, default=1e-4)
+    return ap.parse_args()
+
+
+def cpu_baseline(scat, sysd, gpu_iters: float, n_iters: int):
+    """Oracle (NumPy/SciPy FP64 restatement of the MATLAB path) on the host cores: time a bounded
+    number of preconditioned BiCGStab iterations and scale to the GPU FP64 iteration count."""
+    from oracle import circulant as Ch
+    from oracle import krylov as OK
+    from oracle import mvp as OM
+    from oracle import operator as O
+    from oracle import _fft
+
+    L, M, N = scat.shape
+    fref, Tref, alpha = O.getOPERATOR_DDA(scat.r, scat.k0, scat.refInd, sysd.kvec, scat.Eo)
+    ones = np.ones((L, M, N))
+    chi = np.zeros(L * M * N)
+    chi[scat.idx] = 1
+    chi = chi.reshape((L, M, N), order="F")
+    _, cL = Ch.circ_1_level(Tref, L, M, N)
+    _, blk = Ch.circ_2_level(cL, L, M, N)
+    inv = Ch.invert_blocks_dda(blk, alpha, sysd.dV)
+
+    def afun(v):
+        return OM.MVP(v, fref, ones, sysd.dV * chi, 1 / alpha, "notransp", sysd.idx3)
+
+    def mfun(v):
+        return Ch.mvp_circ_2_level(inv, v, L, M, N, sysd.idx3)
+
+    t0 = time.perf_counter()
+    OK.bicgstab(afun, sysd.b, 1e-30, n_iters, mfun)   # tolerance unreachable: exactly n_iters iterations
+    dt = time.perf_counter() - t0
+    per_iter = dt / n_iters
+    return {
+        "value": per_iter * gpu_iters * 1e3,
+        "unit": "ms",
+        "cores": _fft.WORKERS,
+        "kind": "port",
+        "sample": (f"{n_iters} preconditioned BiCGStab iterations of the FP64 restatement on the same Koch "
+                   f"system ({dt:.1f} s, {per_iter * 1e3:.0f} ms/iteration), scaled to the "
+                   f"{gpu_iters:g} iterations of the GPU FP64 solve; scipy.fft workers={_fft.WORKERS}"),
+    }
+
+
+def main():
+    args = _parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(independent particles/right-hand sides would shard across GPUs with no communication)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from voxscatter_amd import api, geometry
+    from voxscatter_amd.capi import VS_FP32, VS_FP64
+    from voxscatter_amd.dda import DDASystem
+
+    ctx = api.get_context()
+    scat = geometry.koch_prism(128, 128, 16, levels=3, kR=20.0, m=1.31 + 0.01j)
+    L, M, N = scat.shape
+    sysd = DDASystem.build(scat, precisions=VS_FP64 | VS_FP32, ctx=ctx)
+    pc = sysd.preconditioner(2, precisions=VS_FP64 | VS_FP32)
+    op = sysd.op
+    n = op.n
+    b = api.DeviceArray.from_numpy(sysd.b, np.complex128, ctx)
+    x = api.DeviceArray(n, np.complex128, ctx)
+
+    def refined():
+        return api.refine_dev(op, b.ptr, x.ptr, pc, tol=TOL, max_outer=10, inner="bicgstab",
+                              inner_tol=args.inner_tol, inner_maxit=2000)
+
+    for _ in range(args.warmup):
+        refined()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        flag, relres, outer, inner_total = refined()
+        print(f"[bench] step {i}: flag={flag} relres={relres:.2e} outer={outer} inner={inner_total}",
+              file=sys.stderr, flush=True)
+    ctx.synchronize()
+    t_ref = (time.perf_counter() - t0) / args.steps
+    x_ref = x.numpy()
+    assert flag == 0 and relres <= TOL, (flag, relres)
+
+    # plain FP64 solve of the same system (timed beside, one warm run + one timed)
+    x64 = api.DeviceArray(n, np.complex128, ctx)
+    api.bicgstab_dev(op, b.ptr, x64.ptr, TOL, 2000, pc)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    f64, rr64, it64, nres64 = api.bicgstab_dev(op, b.ptr, x64.ptr, TOL, 2000, pc)
+    ctx.synchronize()
+    t_64 = time.perf_counter() - t0
+    print(f"[bench] fp64: flag={f64} iter={it64} {t_64 * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    x64n = x64.numpy()
+    # true FP64 residual of both solutions, and their difference
+    r_ref = np.linalg.norm(op.mvp(x_ref) - sysd.b) / np.linalg.norm(sysd.b)
+    r_64 = np.linalg.norm(op.mvp(x64n) - sysd.b) / np.linalg.norm(sysd.b)
+    sol_err = float(np.linalg.norm(x_ref - x64n) / np.linalg.norm(x64n))
+
+    # matvec and preconditioner-apply timing (HIP events on the library stream)
+    P = 8 * L * M * N
+    V = L * M * N
+    mv = {}
+    rng = np.random.default_rng(0)
+    xr = rng.normal(size=n) + 1j * rng.normal(size=n)
+    for prec, dt, sc in ((VS_FP32, np.complex64, 8), (VS_FP64, np.complex128, 16)):
+        xv = api.DeviceArray.from_numpy(xr, dt, ctx)
+        yv = api.DeviceArray(n, dt, ctx)
+        ms = api.bench_mvp(op, xv.ptr, yv.ptr, False, prec, 1, 50)
+        ms_pc = api.bench_pc(pc, xv.ptr, yv.ptr, prec, 1, 50)
+        B_mv = sc * (12 * P + 6 * V)
+        B_pc = sc * (9 * L * M * N * N + 18 * V)
+        mv[prec] = dict(ms=ms, gbs=B_mv / (ms * 1e-3) / 1e9, bytes=B_mv, pc_ms=ms_pc,
+                        pc_gbs=B_pc / (ms_pc * 1e-3) / 1e9)
+
+    m32 = mv[VS_FP32]
+    roof = {"bound": "hbm", "achieved": round(m32["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(m32["gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "FP32 FFT matvec (5 kernels: mv_fx, col FFT y, mv_fzmul, col IFFT y, mv_ix)",
+            "bytes_per_launch": m32["bytes"], "ms_per_launch": round(m32["ms"], 5)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(scat, sysd, gpu_iters=float(it64), n_iters=args.cpu_iters)
+
+    out = {
+        "metric": "time-to-solution @rel-res 1e-6 (FP32+refine vs FP64) + matvec HBM GB/s",
+        "value": round(t_ref * 1e3, 3),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_ref * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64 inner / c128 refinement",
+        "data": "synthetic",
+        "config": {"workload": "koch_prism level 3, 128x128x16, kR=20, m=1.31+0.01i, DDA-LDR, "
+                               "BiCGStab + 2-level Chan, rel-res 1e-6",
+                   "unknowns": int(n), "grid": [L, M, N]},
+        "fp64_ms": round(t_64 * 1e3, 3),
+        "speedup_vs_fp64": round(t_64 / t_ref, 3),
+        "fp64_bicgstab_iters": it64,
+        "refine_outer": outer,
+        "refine_inner_halfsteps": inner_total,
+        "relres_refined_fp64": float(r_ref),
+        "relres_fp64": float(r_64),
+        "solution_rel_diff_vs_fp64": sol_err,
+        "matvec_fp32_ms": round(m32["ms"], 5),
+        "matvec_fp32_gbs": round(m32["gbs"], 1),
+        "matvec_fp64_ms": round(mv[VS_FP64]["ms"], 5),
+        "matvec_fp64_gbs": round(mv[VS_FP64]["gbs"], 1),
+        "pc_apply_fp32_ms": round(m32["pc_ms"], 5),
+        "pc_apply_fp64_ms": round(mv[VS_FP64]["pc_ms"], 5),
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

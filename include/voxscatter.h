@@ -50,6 +50,13 @@ int vs_ctx_create(int device, vs_ctx** out);
 void vs_ctx_destroy(vs_ctx* ctx);
 int vs_ctx_synchronize(vs_ctx* ctx);
 
+/* Device buffers on the context's device (no reference counterpart: plumbing for the *_dev entry
+ * points, so that callers need no second GPU runtime).  kind: 0 host->device, 1 device->host,
+ * 2 device->device; copies are ordered on the context stream and complete on return. */
+int vs_dev_alloc(vs_ctx* ctx, int64_t bytes, void** ptr);
+int vs_dev_free(vs_ctx* ctx, void* ptr);
+int vs_dev_copy(vs_ctx* ctx, void* dst, const void* src, int64_t bytes, int kind);
+
 /* ---------------------------------------------------------------------------------------------- */
 /* operator assembly                                                                               */
 /* ---------------------------------------------------------------------------------------------- */
@@ -131,9 +138,29 @@ int vs_pgmres(vs_op* op, vs_pc* pc, int side, const double* b, double* x, int re
               int maxit, const double* x0, int* flag, double* relres, int iter[2], double* resvec,
               int* nres);
 
-/* Device-pointer forms for benchmarking: b, x are FP64 device vectors. */
+/* Device-pointer forms: b, x are FP64 device vectors of vs_op_nunknowns() values. */
 int vs_bicgstab_dev(vs_op* op, vs_pc* pc, const void* b, void* x, double tol, int maxit, int* flag,
                     double* relres, double* iter, int* nres);
+int vs_gmres_dev(vs_op* op, vs_pc* pc, const void* b, void* x, int restart, double tol, int maxit, int* flag,
+                 double* relres, int iter[2], int* nres);
+
+/* Mixed-precision solve (no reference counterpart; the precision form of the solve loop): the
+ * correction equation A d = r is solved with the complex-FP32 tensor, FP32 preconditioner blocks and
+ * FP32 Krylov vectors (inner_solver 0: BiCGStab, 1: GMRES(restart)) to inner_tol; the residual
+ * b - A x and the update x += d are formed in FP64 until norm(b - A x)/norm(b) <= tol.
+ * op needs VS_FP64|VS_FP32 tensors, pc (nullable) VS_FP32 blocks.  resvec_outer receives the FP64
+ * relative residual before each correction (max_outer+1 doubles). */
+int vs_refine(vs_op* op, vs_pc* pc, const double* b, double* x, double tol, int max_outer, int inner_solver,
+              double inner_tol, int inner_maxit, int restart, const double* x0, int* flag, double* relres,
+              int* outer, int* inner_total, double* resvec_outer, int* nres);
+int vs_refine_dev(vs_op* op, vs_pc* pc, const void* b, void* x, double tol, int max_outer, int inner_solver,
+                  double inner_tol, int inner_maxit, int restart, int* flag, double* relres, int* outer,
+                  int* inner_total);
+
+/* Timing helpers (no reference counterpart): nrep back-to-back matvecs / preconditioner applies on
+ * the context stream, bracketed by HIP events; *ms receives the mean time per call. */
+int vs_bench_mvp(vs_op* op, const void* x, void* y, int transp, int precision, int nrhs, int nrep, double* ms);
+int vs_bench_pc(vs_pc* pc, const void* x, void* y, int precision, int nrhs, int nrep, double* ms);
 
 #ifdef __cplusplus
 }

@@ -7,6 +7,8 @@ from __future__ import annotations
 
 import numpy as np
 
+from . import _fft
+
 # components whose Chan average negates the first term: x-direction (circ_1_level.m:29,42),
 # y-direction (circ_2_level.m:27,63)
 ODD_X = (1, 2)
@@ -41,7 +43,7 @@ def circ_L_opToep(opToeplitz):
     """Chan circulant along x + FFT along x, per component (circ_1_level.m:5-90)."""
     out = np.zeros_like(opToeplitz, dtype=np.complex128)
     for p in range(6):
-        out[..., p] = np.fft.fft(_chan_axis0(opToeplitz[..., p].astype(np.complex128), p in ODD_X), axis=0)
+        out[..., p] = _fft.fft(_chan_axis0(opToeplitz[..., p].astype(np.complex128), p in ODD_X), axis=0)
     return out
 
 
@@ -91,7 +93,7 @@ def circ_2_level(circ_L, L, M, N):
     circ_M = np.zeros_like(circ_L, dtype=np.complex128)
     blocks = np.zeros((L, M, 3 * N, 3 * N), dtype=np.complex128)
     for i in range(L):
-        F = [np.fft.fft(_chan_axis0(circ_L[i, :, :, p], p in ODD_Y), axis=0) for p in range(6)]
+        F = [_fft.fft(_chan_axis0(circ_L[i, :, :, p], p in ODD_Y), axis=0) for p in range(6)]
         for p in range(6):
             circ_M[i, :, :, p] = -F[p] if p == 2 else F[p]   # circ_2_level.m:44 (stored, unused)
         for j in range(M):
@@ -135,13 +137,13 @@ def mvp_circ_2_level(inv_blocks, JIn, L, M, N, idx):
     """Apply the 2-level preconditioner (mvp_circ_2_level.m:2-22).  ``idx`` is 0-based."""
     V = np.zeros(3 * L * M * N, dtype=np.complex128)
     V[idx] = JIn
-    temp = np.fft.fft(V.reshape((L, 3 * M * N), order="F"), axis=0).T        # 3MN x L
+    temp = _fft.fft(V.reshape((L, 3 * M * N), order="F"), axis=0).T        # 3MN x L
     for i in range(L):
-        TEMP = np.fft.fft(temp[:, i].reshape((M, 3 * N), order="F"), axis=0).T   # 3N x M
+        TEMP = _fft.fft(temp[:, i].reshape((M, 3 * N), order="F"), axis=0).T   # 3N x M
         TEMP = np.einsum("jab,bj->aj", inv_blocks[i], TEMP)
-        TEMP = np.fft.ifft(TEMP.T, axis=0)                                      # M x 3N
+        TEMP = _fft.ifft(TEMP.T, axis=0)                                      # M x 3N
         temp[:, i] = TEMP.reshape(-1, order="F")
-    temp = np.fft.ifft(temp.T, axis=0)
+    temp = _fft.ifft(temp.T, axis=0)
     return temp.reshape(-1, order="F")[idx]
 
 
@@ -149,8 +151,8 @@ def chan_mvp_idx(inv_blocks, JIn, L, M, N, idx):
     """Apply the 1-level preconditioner (chan_mvp_idx.m:2-16).  ``idx`` is 0-based."""
     V = np.zeros(3 * L * M * N, dtype=np.complex128)
     V[idx] = JIn
-    temp = np.fft.fft(V.reshape((L, 3 * M * N), order="F"), axis=0).T
+    temp = _fft.fft(V.reshape((L, 3 * M * N), order="F"), axis=0).T
     for i in range(L):
         temp[:, i] = inv_blocks[i] @ temp[:, i]
-    temp = np.fft.ifft(temp.T, axis=0)
+    temp = _fft.ifft(temp.T, axis=0)
     return temp.reshape(-1, order="F")[idx]

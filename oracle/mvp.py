@@ -8,6 +8,8 @@ from __future__ import annotations
 
 import numpy as np
 
+from . import _fft
+
 # symmetric 3x3 tensor from the six stored components [xx, xy, xz, yy, yz, zz] (MVP.m:108-123)
 TENSOR_IDX = ((0, 1, 2), (1, 3, 4), (2, 4, 5))
 
@@ -38,20 +40,20 @@ def MVP(JIn0, fN, Mr, Mc, Gram, transp_flag, idx):
         Mcc = np.conj(Mc)
         acc = [0, 0, 0]
         for c in range(3):
-            fJ = np.fft.fftn(np.conj(Mcc * JIn[..., c]), s=s, axes=(0, 1, 2))
+            fJ = _fft.fftn(np.conj(Mcc * JIn[..., c]), s=s, axes=(0, 1, 2))
             for o in range(3):
                 acc[o] = acc[o] + fN[..., TENSOR_IDX[c][o]] * fJ
         for o in range(3):
-            J = np.fft.ifftn(acc[o])
+            J = _fft.ifftn(acc[o])
             JOut[..., o] = Gram * np.conj(Mr) * JIn[..., o] - np.conj(J[:L, :M, :N])
     elif transp_flag == "notransp":
         acc = [0, 0, 0]
         for c in range(3):
-            fJ = np.fft.fftn(JIn[..., c], s=s, axes=(0, 1, 2))
+            fJ = _fft.fftn(JIn[..., c], s=s, axes=(0, 1, 2))
             for o in range(3):
                 acc[o] = acc[o] + fN[..., TENSOR_IDX[c][o]] * fJ
         for o in range(3):
-            J = np.fft.ifftn(acc[o])
+            J = _fft.ifftn(acc[o])
             JOut[..., o] = Gram * Mr * JIn[..., o] - Mc * J[:L, :M, :N]
     else:
         raise ValueError("transp_flag must be 'transp' or 'notransp'")

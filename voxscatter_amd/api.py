@@ -96,6 +96,56 @@ def _digest(*arrays) -> str:
     return h.hexdigest()
 
 
+class DeviceArray:
+    """A complex vector (or batch of vectors) in HBM, allocated through the library's own runtime."""
+
+    def __init__(self, shape, dtype=np.complex128, ctx: Context | None = None):
+        self.ctx = ctx or get_context()
+        self.shape = tuple(np.atleast_1d(shape))
+        self.dtype = np.dtype(dtype)
+        assert self.dtype in (np.complex128, np.complex64)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        check(self.ctx.lib.vs_dev_alloc(self.ctx.h, self.nbytes, C.byref(p)))
+        self.ptr = p.value
+
+    @property
+    def precision(self) -> int:
+        return VS_FP64 if self.dtype == np.complex128 else VS_FP32
+
+    @classmethod
+    def from_numpy(cls, a, dtype=None, ctx: Context | None = None):
+        a = np.ascontiguousarray(np.asarray(a, dtype=dtype or np.asarray(a).dtype))
+        d = cls(a.shape, a.dtype, ctx)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(np.asarray(a, dtype=self.dtype))
+        assert a.nbytes == self.nbytes
+        check(self.ctx.lib.vs_dev_copy(self.ctx.h, C.c_void_p(self.ptr), a.ctypes.data_as(C.c_void_p), self.nbytes, 0))
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=self.dtype)
+        check(self.ctx.lib.vs_dev_copy(self.ctx.h, out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr),
+                                       self.nbytes, 1))
+        return out
+
+    def zero(self):
+        self.upload(np.zeros(self.shape, dtype=self.dtype))
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            self.ctx.lib.vs_dev_free(self.ctx.h, C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 # --------------------------------------------------------------------------------------------------
 # operator
 # --------------------------------------------------------------------------------------------------
@@ -475,3 +525,73 @@ def pgmres(A, b, restart=50, tol=1e-3, maxit=10, L1=None, L2=None, R1=None, R2=N
                               int(maxit), dptr(x0a), C.byref(flag), C.byref(relres), it, dptr(resvec),
                               C.byref(nres)))
     return x, flag.value, relres.value, (it[0], it[1]), resvec[:nres.value].copy()
+
+
+# --------------------------------------------------------------------------------------------------
+# precision form and device-pointer entry points
+# --------------------------------------------------------------------------------------------------
+
+def refine(A, b, M1=None, tol=1e-6, max_outer=10, inner="bicgstab", inner_tol=1e-4, inner_maxit=2000, restart=50,
+           x0=None):
+    """FP32 inner solve + FP64 iterative refinement; returns ``(x, flag, relres, outer, inner_total, resvec)``.
+
+    ``A`` needs FP64 and FP32 tensors (``precisions=VS_FP64|VS_FP32``), ``M1`` FP32 blocks.
+    ``resvec`` is the FP64 relative residual before each correction.
+    """
+    A = _check_A(A)
+    b = cplx_arr(b, A.n)
+    x = np.zeros_like(b)
+    x0a = None if x0 is None else cplx_arr(x0, A.n)
+    flag = C.c_int()
+    relres = C.c_double()
+    outer = C.c_int()
+    tot = C.c_int()
+    nres = C.c_int()
+    resvec = np.zeros(max_outer + 1)
+    check(A.ctx.lib.vs_refine(A.h, None if M1 is None else M1.h, dptr(b), dptr(x), float(tol), int(max_outer),
+                              0 if inner == "bicgstab" else 1, float(inner_tol), int(inner_maxit), int(restart),
+                              dptr(x0a), C.byref(flag), C.byref(relres), C.byref(outer), C.byref(tot), dptr(resvec),
+                              C.byref(nres)))
+    return x, flag.value, relres.value, outer.value, tot.value, resvec[:nres.value].copy()
+
+
+def bicgstab_dev(A: Operator, b_ptr: int, x_ptr: int, tol=1e-6, maxit=2000, M1=None):
+    flag, relres, it, nres = C.c_int(), C.c_double(), C.c_double(), C.c_int()
+    check(A.ctx.lib.vs_bicgstab_dev(A.h, None if M1 is None else M1.h, C.c_void_p(b_ptr), C.c_void_p(x_ptr),
+                                    float(tol), int(maxit), C.byref(flag), C.byref(relres), C.byref(it),
+                                    C.byref(nres)))
+    return flag.value, relres.value, it.value, nres.value
+
+
+def gmres_dev(A: Operator, b_ptr: int, x_ptr: int, restart=50, tol=1e-6, maxit=40, M1=None):
+    flag, relres, nres = C.c_int(), C.c_double(), C.c_int()
+    it = (C.c_int * 2)()
+    check(A.ctx.lib.vs_gmres_dev(A.h, None if M1 is None else M1.h, C.c_void_p(b_ptr), C.c_void_p(x_ptr),
+                                 int(restart), float(tol), int(maxit), C.byref(flag), C.byref(relres), it,
+                                 C.byref(nres)))
+    return flag.value, relres.value, (it[0], it[1]), nres.value
+
+
+def refine_dev(A: Operator, b_ptr: int, x_ptr: int, M1=None, tol=1e-6, max_outer=10, inner="bicgstab",
+               inner_tol=1e-4, inner_maxit=2000, restart=50):
+    flag, relres, outer, tot = C.c_int(), C.c_double(), C.c_int(), C.c_int()
+    check(A.ctx.lib.vs_refine_dev(A.h, None if M1 is None else M1.h, C.c_void_p(b_ptr), C.c_void_p(x_ptr),
+                                  float(tol), int(max_outer), 0 if inner == "bicgstab" else 1, float(inner_tol),
+                                  int(inner_maxit), int(restart), C.byref(flag), C.byref(relres), C.byref(outer),
+                                  C.byref(tot)))
+    return flag.value, relres.value, outer.value, tot.value
+
+
+def bench_mvp(A: Operator, x_ptr: int, y_ptr: int, transp=False, precision=VS_FP64, nrhs=1, nrep=20) -> float:
+    """Mean milliseconds per matvec, HIP events on the library stream."""
+    ms = C.c_double()
+    check(A.ctx.lib.vs_bench_mvp(A.h, C.c_void_p(x_ptr), C.c_void_p(y_ptr), 1 if transp else 0, int(precision),
+                                 int(nrhs), int(nrep), C.byref(ms)))
+    return ms.value
+
+
+def bench_pc(M: Preconditioner, x_ptr: int, y_ptr: int, precision=VS_FP64, nrhs=1, nrep=20) -> float:
+    ms = C.c_double()
+    check(M.ctx.lib.vs_bench_pc(M.h, C.c_void_p(x_ptr), C.c_void_p(y_ptr), int(precision), int(nrhs), int(nrep),
+                                C.byref(ms)))
+    return ms.value

@@ -47,6 +47,9 @@ SIGNATURES = {
     "vs_ctx_create": (INT, [INT, C.POINTER(P)]),
     "vs_ctx_destroy": (None, [P]),
     "vs_ctx_synchronize": (INT, [P]),
+    "vs_dev_alloc": (INT, [P, C.c_int64, C.POINTER(P)]),
+    "vs_dev_free": (INT, [P, P]),
+    "vs_dev_copy": (INT, [P, P, P, C.c_int64, INT]),
     "vs_alpha_ldr": (INT, [D, C.c_double, C.c_double, D, D, D]),
     "vs_op_create_dda": (INT, [P, INT, INT, INT, D, C.c_double, INT, INT, C.POINTER(P)]),
     "vs_op_create_fN": (INT, [P, INT, INT, INT, D, D, INT, C.POINTER(P)]),
@@ -69,6 +72,14 @@ SIGNATURES = {
     "vs_bicgstab_dev": (INT, [P, P, P, P, C.c_double, INT, C.POINTER(C.c_int), D, D, C.POINTER(C.c_int)]),
     "vs_gmres": (INT, [P, P, D, D, INT, C.c_double, INT, D, C.POINTER(C.c_int), D, C.POINTER(C.c_int), D,
                        C.POINTER(C.c_int)]),
+    "vs_gmres_dev": (INT, [P, P, P, P, INT, C.c_double, INT, C.POINTER(C.c_int), D, C.POINTER(C.c_int),
+                           C.POINTER(C.c_int)]),
+    "vs_refine": (INT, [P, P, D, D, C.c_double, INT, INT, C.c_double, INT, INT, D, C.POINTER(C.c_int), D,
+                        C.POINTER(C.c_int), C.POINTER(C.c_int), D, C.POINTER(C.c_int)]),
+    "vs_refine_dev": (INT, [P, P, P, P, C.c_double, INT, INT, C.c_double, INT, INT, C.POINTER(C.c_int), D,
+                            C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "vs_bench_mvp": (INT, [P, P, P, INT, INT, INT, INT, D]),
+    "vs_bench_pc": (INT, [P, P, P, INT, INT, INT, D]),
     "vs_pgmres": (INT, [P, P, INT, D, D, INT, C.c_double, INT, D, C.POINTER(C.c_int), D, C.POINTER(C.c_int), D,
                         C.POINTER(C.c_int)]),
 }

@@ -141,6 +141,41 @@ int vs_ctx_synchronize(vs_ctx* ctx) {
   });
 }
 
+int vs_dev_alloc(vs_ctx* ctx, int64_t bytes, void** ptr) {
+  return guarded([&] {
+    VS_CHECK(ctx && ptr && bytes >= 0, VS_ERR_ARG, "bad arguments");
+    DeviceGuard g(ctx->device);
+    *ptr = nullptr;
+    hipError_t e = hipMalloc(ptr, (size_t)std::max<int64_t>(bytes, 1));
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      *ptr = nullptr;
+      throw Error{VS_ERR_OOM, "hipMalloc failed: " + std::string(hipGetErrorString(e))};
+    }
+  });
+}
+
+int vs_dev_free(vs_ctx* ctx, void* ptr) {
+  return guarded([&] {
+    VS_CHECK(ctx != nullptr, VS_ERR_ARG, "null context");
+    DeviceGuard g(ctx->device);
+    VS_HIP(hipStreamSynchronize(ctx->stream));
+    if (ptr) VS_HIP(hipFree(ptr));
+  });
+}
+
+int vs_dev_copy(vs_ctx* ctx, void* dst, const void* src, int64_t bytes, int kind) {
+  return guarded([&] {
+    VS_CHECK(ctx && dst && src && bytes >= 0, VS_ERR_ARG, "bad arguments");
+    VS_CHECK(kind >= 0 && kind <= 2, VS_ERR_ARG, "kind must be 0 (h2d), 1 (d2h) or 2 (d2d)");
+    DeviceGuard g(ctx->device);
+    const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : (kind == 1 ? hipMemcpyDeviceToHost
+                                                                           : hipMemcpyDeviceToDevice);
+    VS_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, k, ctx->stream));
+    VS_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
 int vs_alpha_ldr(const double refInd[2], double ko, double d, const double kvec[3], const double Eo[3],
                  double alpha_out[2]) {
   return guarded([&] {
@@ -587,6 +622,123 @@ int vs_pgmres(vs_op* op, vs_pc* pc, int side, const double* b, double* x, int re
     iter[1] = R.outer;
     *nres = (int)R.resvec.size();
     std::copy(R.resvec.begin(), R.resvec.end(), resvec);
+  });
+}
+
+int vs_gmres_dev(vs_op* op, vs_pc* pc, const void* b, void* x, int restart, double tol, int maxit, int* flag,
+                 double* relres, int iter[2], int* nres) {
+  return guarded([&] {
+    VS_CHECK(op && b && x && flag && relres && iter && nres, VS_ERR_ARG, "null pointer");
+    VS_CHECK(maxit >= 1 && restart >= 1, VS_ERR_ARG, "maxit and restart must be >= 1");
+    check_solver_args(op, pc, VS_FP64);
+    DeviceGuard g(op->ctx->device);
+    Engine<double> E(op->ctx, op->nunk);
+    SolveResult R = solve_gmres<double>(E, make_linop<double>(op, pc), (const c128*)b, (c128*)x, false, restart, tol,
+                                        maxit);
+    *flag = R.flag;
+    *relres = R.relres;
+    iter[0] = R.outer;
+    iter[1] = R.inner;
+    *nres = (int)R.resvec.size();
+  });
+}
+
+}  // extern "C"
+
+namespace {
+SolveResult refine_common(vs_op* op, vs_pc* pc, const c128* b, c128* x, bool has_x0, double tol, int max_outer,
+                          int inner_solver, double inner_tol, int inner_maxit, int restart, int* inner_total) {
+  VS_CHECK(op != nullptr, VS_ERR_ARG, "null operator");
+  VS_CHECK((op->precisions & (VS_FP64 | VS_FP32)) == (VS_FP64 | VS_FP32), VS_ERR_ARG,
+           "refinement needs an operator holding both FP64 and FP32 tensors");
+  check_solver_args(op, pc, VS_FP32);
+  VS_CHECK(inner_solver == 0 || inner_solver == 1, VS_ERR_ARG, "inner_solver must be 0 (bicgstab) or 1 (gmres)");
+  VS_CHECK(max_outer >= 0 && inner_maxit >= 1, VS_ERR_ARG, "bad iteration limits");
+  Engine<double> E64(op->ctx, op->nunk);
+  Engine<float> E32(op->ctx, op->nunk);
+  LinOp<double> A64 = make_linop<double>(op, nullptr);
+  LinOp<float> A32 = make_linop<float>(op, pc);
+  return solve_refine(E64, E32, A64, A32, b, x, has_x0, tol, max_outer, inner_solver, inner_tol, inner_maxit,
+                      restart, inner_total);
+}
+}  // namespace
+
+extern "C" {
+
+int vs_refine(vs_op* op, vs_pc* pc, const double* b, double* x, double tol, int max_outer, int inner_solver,
+              double inner_tol, int inner_maxit, int restart, const double* x0, int* flag, double* relres,
+              int* outer, int* inner_total, double* resvec_outer, int* nres) {
+  return guarded([&] {
+    VS_CHECK(op && b && x && flag && relres && outer && inner_total && resvec_outer && nres, VS_ERR_ARG,
+             "null pointer");
+    vs_ctx* ctx = op->ctx;
+    DeviceGuard g(ctx->device);
+    const int64_t n = op->nunk;
+    DevBuf db(sizeof(c128) * n), dx(sizeof(c128) * n);
+    h2d(ctx, db.p, b, sizeof(c128) * n);
+    if (x0) h2d(ctx, dx.p, x0, sizeof(c128) * n);
+    SolveResult R = refine_common(op, pc, db.as<c128>(), dx.as<c128>(), x0 != nullptr, tol, max_outer, inner_solver,
+                                  inner_tol, inner_maxit, restart, inner_total);
+    d2h(ctx, x, dx.p, sizeof(c128) * n);
+    *flag = R.flag;
+    *relres = R.relres;
+    *outer = R.outer;
+    *nres = (int)R.resvec.size();
+    std::copy(R.resvec.begin(), R.resvec.end(), resvec_outer);
+  });
+}
+
+int vs_refine_dev(vs_op* op, vs_pc* pc, const void* b, void* x, double tol, int max_outer, int inner_solver,
+                  double inner_tol, int inner_maxit, int restart, int* flag, double* relres, int* outer,
+                  int* inner_total) {
+  return guarded([&] {
+    VS_CHECK(op && b && x && flag && relres && outer && inner_total, VS_ERR_ARG, "null pointer");
+    DeviceGuard g(op->ctx->device);
+    SolveResult R = refine_common(op, pc, (const c128*)b, (c128*)x, false, tol, max_outer, inner_solver, inner_tol,
+                                  inner_maxit, restart, inner_total);
+    *flag = R.flag;
+    *relres = R.relres;
+    *outer = R.outer;
+  });
+}
+
+}  // extern "C"
+
+namespace {
+template <class F>
+double time_on_stream(vs_ctx* ctx, int nrep, F&& body) {
+  hipEvent_t e0, e1;
+  VS_HIP(hipEventCreate(&e0));
+  VS_HIP(hipEventCreate(&e1));
+  VS_HIP(hipEventRecord(e0, ctx->stream));
+  for (int i = 0; i < nrep; ++i) body();
+  VS_HIP(hipEventRecord(e1, ctx->stream));
+  VS_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  VS_HIP(hipEventElapsedTime(&ms, e0, e1));
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  return (double)ms / nrep;
+}
+}  // namespace
+
+extern "C" {
+
+int vs_bench_mvp(vs_op* op, const void* x, void* y, int transp, int precision, int nrhs, int nrep, double* ms) {
+  return guarded([&] {
+    VS_CHECK(op && x && y && ms && nrep >= 1, VS_ERR_ARG, "bad arguments");
+    DeviceGuard g(op->ctx->device);
+    mvp_dev(op, x, y, transp, precision, nrhs);  // workspace allocation outside the timed region
+    *ms = time_on_stream(op->ctx, nrep, [&] { mvp_dev(op, x, y, transp, precision, nrhs); });
+  });
+}
+
+int vs_bench_pc(vs_pc* pc, const void* x, void* y, int precision, int nrhs, int nrep, double* ms) {
+  return guarded([&] {
+    VS_CHECK(pc && x && y && ms && nrep >= 1, VS_ERR_ARG, "bad arguments");
+    DeviceGuard g(pc->ctx->device);
+    pc_apply_dev(pc, x, y, precision, nrhs);
+    *ms = time_on_stream(pc->ctx, nrep, [&] { pc_apply_dev(pc, x, y, precision, nrhs); });
   });
 }
 

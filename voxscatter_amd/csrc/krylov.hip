@@ -498,3 +498,65 @@ template SolveResult solve_pgmres<float>(Engine<float>&, const LinOp<float>&, in
                                          int);
 
 }  // namespace vs
+
+namespace vs {
+
+// Mixed precision iterative refinement: the correction equation A d = r is solved in complex FP32
+// (FP32 tensor, FP32 preconditioner blocks, FP32 Krylov vectors with FP64 reductions); the residual
+// r = b - A x and the update x += d are formed in FP64 with the FP64 tensor.
+SolveResult solve_refine(Engine<double>& E64, Engine<float>& E32, const LinOp<double>& A64, const LinOp<float>& A32,
+                         const c128* b, c128* x, bool has_x0, double tol, int max_outer, int inner_solver,
+                         double inner_tol, int inner_maxit, int restart, int* inner_total) {
+  const int64_t n = E64.n;
+  SolveResult R;
+  vs_ctx* ctx = E64.ctx;
+  const double n2b = E64.norm(b);
+  if (!has_x0) VS_HIP(hipMemsetAsync(x, 0, sizeof(c128) * n, ctx->stream));
+  if (n2b == 0) {
+    R.flag = 0;
+    R.resvec = {0.0};
+    return R;
+  }
+  DevBuf r64(sizeof(c128) * n), r32(sizeof(c64) * n), d32(sizeof(c64) * n), d64(sizeof(c128) * n);
+  int tot = 0;
+  double relres = 0;
+  R.flag = 1;
+  int outer = 0;
+  for (outer = 0; outer <= max_outer; ++outer) {
+    A64.A(x, r64.as<c128>());
+    const double nr = E64.update(UPD_SUB_, b, nullptr, nullptr, nullptr, r64.as<c128>(), nullptr, mk<double>(0, 0),
+                                 mk<double>(0, 0), true);
+    relres = nr / n2b;
+    R.resvec.push_back(relres);
+    if (!std::isfinite(nr)) {
+      R.flag = 2;
+      break;
+    }
+    if (relres <= tol) {
+      R.flag = 0;
+      break;
+    }
+    if (outer == max_outer) break;
+    // scale the residual to unit norm before rounding to FP32
+    E64.update(UPD_SCALE_, r64.as<c128>(), nullptr, nullptr, nullptr, r64.as<c128>(), nullptr, mk<double>(1.0 / nr, 0),
+               mk<double>(0, 0), false);
+    convert_dev(ctx, r64.p, VS_FP64, r32.p, VS_FP32, n);
+    SolveResult in;
+    if (inner_solver == 0)
+      in = solve_bicgstab<float>(E32, A32, r32.as<c64>(), d32.as<c64>(), false, inner_tol, inner_maxit);
+    else
+      in = solve_gmres<float>(E32, A32, r32.as<c64>(), d32.as<c64>(), false, restart, inner_tol, inner_maxit);
+    tot += inner_solver == 0 ? (int)(in.resvec.size() - 1) : (int)(in.resvec.size() - 1);
+    convert_dev(ctx, d32.p, VS_FP32, d64.p, VS_FP64, n);
+    E64.update(UPD_AXPBY_, d64.as<c128>(), nullptr, nullptr, nullptr, x, nullptr, mk<double>(nr, 0), mk<double>(1, 0),
+               false);
+  }
+  R.relres = relres;
+  R.outer = outer;
+  R.iter = outer;
+  if (inner_total) *inner_total = tot;
+  VS_HIP(hipStreamSynchronize(ctx->stream));
+  return R;
+}
+
+}  // namespace vs

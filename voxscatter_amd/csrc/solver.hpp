@@ -52,6 +52,10 @@ template <typename T>
 SolveResult solve_pgmres(Engine<T>& E, const LinOp<T>& op, int side, const cplx<T>* b, cplx<T>* x, bool has_x0,
                          int restart, double tol, int maxit);
 
+SolveResult solve_refine(Engine<double>& E64, Engine<float>& E32, const LinOp<double>& A64, const LinOp<float>& A32,
+                         const c128* b, c128* x, bool has_x0, double tol, int max_outer, int inner_solver,
+                         double inner_tol, int inner_maxit, int restart, int* inner_total);
+
 void convert_dev(vs_ctx* ctx, const void* in, int pin, void* out, int pout, int64_t n);
 void mvp_dev(vs_op* op, const void* x, void* y, int transp, int precision, int nrhs);
 void pc_apply_dev(vs_pc* pc, const void* x, void* y, int precision, int nrhs);
