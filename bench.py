@@ -30,7 +30,9 @@ def _parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=4, help="BiCGStab iterations timed for the CPU baseline")
-    ap.add_argument("--inner-tol", type=float, default=1e-4)
+    ap.add_argument("--inner-tol", type=float, default=1e-3)
+    ap.add_argument("--inner", default="bicgstab", choices=["bicgstab", "gmres"])
+    ap.add_argument("--sweep", action="store_true", help="also time other inner solvers/tolerances (stderr)")
     return ap.parse_args()
 
 
@@ -96,9 +98,20 @@ def main():
     b = api.DeviceArray.from_numpy(sysd.b, np.complex128, ctx)
     x = api.DeviceArray(n, np.complex128, ctx)
 
-    def refined():
-        return api.refine_dev(op, b.ptr, x.ptr, pc, tol=TOL, max_outer=10, inner="bicgstab",
-                              inner_tol=args.inner_tol, inner_maxit=2000)
+    def refined(inner=args.inner, inner_tol=args.inner_tol):
+        return api.refine_dev(op, b.ptr, x.ptr, pc, tol=TOL, max_outer=10, inner=inner,
+                              inner_tol=inner_tol, inner_maxit=2000, restart=50)
+
+    if args.sweep:
+        for inner in ("bicgstab", "gmres"):
+            for itol in (1e-2, 1e-3, 1e-4, 1e-5):
+                refined(inner, itol)
+                ctx.synchronize()
+                t0 = time.perf_counter()
+                f_, rr_, o_, it_ = refined(inner, itol)
+                ctx.synchronize()
+                print(f"[sweep] inner={inner} inner_tol={itol:g}: {1e3 * (time.perf_counter() - t0):.2f} ms "
+                      f"flag={f_} relres={rr_:.2e} outer={o_} inner_steps={it_}", file=sys.stderr, flush=True)
 
     for _ in range(args.warmup):
         refined()
@@ -122,6 +135,15 @@ def main():
     ctx.synchronize()
     t_64 = time.perf_counter() - t0
     print(f"[bench] fp64: flag={f64} iter={it64} {t_64 * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    # plain FP64 GMRES(50) of the same system
+    xg = api.DeviceArray(n, np.complex128, ctx)
+    api.gmres_dev(op, b.ptr, xg.ptr, 50, TOL, 40, pc)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    fg, rrg, itg, nresg = api.gmres_dev(op, b.ptr, xg.ptr, 50, TOL, 40, pc)
+    ctx.synchronize()
+    t_g64 = time.perf_counter() - t0
+    print(f"[bench] fp64 gmres(50): flag={fg} iter={itg} {t_g64 * 1e3:.1f} ms", file=sys.stderr, flush=True)
     x64n = x64.numpy()
     # true FP64 residual of both solutions, and their difference
     r_ref = np.linalg.norm(op.mvp(x_ref) - sysd.b) / np.linalg.norm(sysd.b)
@@ -173,6 +195,10 @@ def main():
         "fp64_ms": round(t_64 * 1e3, 3),
         "speedup_vs_fp64": round(t_64 / t_ref, 3),
         "fp64_bicgstab_iters": it64,
+        "fp64_gmres50_ms": round(t_g64 * 1e3, 3),
+        "fp64_gmres50_iters": list(itg),
+        "inner_solver": args.inner,
+        "inner_tol": args.inner_tol,
         "refine_outer": outer,
         "refine_inner_halfsteps": inner_total,
         "relres_refined_fp64": float(r_ref),

@@ -119,3 +119,32 @@ def test_zero_rhs_and_maxit_flag(c1):
     assert flag == flagr == 1 and len(resvec) == len(resvecr) == 5
     assert it == itr
     assert _rel(x, xr) <= 1e-10
+
+
+def test_graph_replay_matches_eager(c1, monkeypatch):
+    """BiCGStab iterations replayed from a captured graph give the eager launches' results, replay after replay."""
+    sysd = c1["sys"]
+    pc = sysd.preconditioner(2)
+    monkeypatch.setenv("VS_GRAPH", "0")
+    xe, fe, re_, ie, rve = api.bicgstab(sysd.op, sysd.b, 1e-8, 2000, pc)
+    monkeypatch.setenv("VS_GRAPH", "1")
+    for rep in range(3):
+        xg, fg, rg, ig, rvg = api.bicgstab(sysd.op, sysd.b, 1e-8, 2000, pc)
+        assert fg == fe == 0 and ig == ie and len(rvg) == len(rve)
+        assert np.allclose(rvg, rve, rtol=1e-12, atol=0)
+        assert _rel(xg, xe) <= 1e-13
+
+
+def test_refinement_reaches_fp64_tolerance(c1, ctx):
+    """FP32 inner BiCGStab / GMRES + FP64 refinement: FP64 true residual <= tol, solution close to FP64."""
+    from voxscatter_amd.capi import VS_FP32, VS_FP64
+    s = c1["s"]
+    sysd = DDASystem.build(s, precisions=VS_FP64 | VS_FP32, ctx=ctx)
+    pc = sysd.preconditioner(2, precisions=VS_FP64 | VS_FP32)
+    xr, fr, rr, itr, rvr = OK.bicgstab(c1["afun"], sysd.b, 1e-10, 2000, _oracle_pc(c1, 2))
+    for inner in ("bicgstab", "gmres"):
+        x, flag, relres, outer, tot, resvec = api.refine(sysd.op, sysd.b, pc, tol=TOL, inner=inner, inner_tol=1e-3)
+        print(f"  refine[{inner}]: flag={flag} relres={relres:.2e} outer={outer} inner={tot} resvec={resvec}")
+        assert flag == 0 and relres <= TOL
+        assert np.linalg.norm(c1["afun"](x) - sysd.b) / np.linalg.norm(sysd.b) <= TOL
+        assert _rel(x, xr) <= 50 * TOL

@@ -9,6 +9,7 @@
 namespace vs {
 void op_build_dda(vs_op* op, const double* r_host, double ko, int nearby);
 void op_build_fN(vs_op* op);
+void op_compress_tensor(vs_op* op);
 void build_circ_L(vs_ctx* ctx, const c128* toep, c128* cL, int L, int M, int N);
 void build_circ_M(vs_ctx* ctx, const c128* cL, c128* F, int L, int M, int N);
 void assemble2(vs_ctx* ctx, const c128* F, int L, int M, int N, c128 a, const c128* w, c128 ws, c128* out);
@@ -245,10 +246,7 @@ int vs_op_create_fN(vs_ctx* ctx, int L, int M, int N, const double* fN, const do
       h2d(ctx, op->toep.p, Toep, sizeof(c128) * 6 * op->V);
       op->has_toep = true;
     }
-    if (precisions & VS_FP32) {
-      op->fN32.ensure(sizeof(c64) * 6 * op->P);
-      convert_dev(ctx, op->fN64.p, VS_FP64, op->fN32.p, VS_FP32, 6 * op->P);
-    }
+    op_compress_tensor(op.get());
     VS_HIP(hipStreamSynchronize(ctx->stream));
     if (!(precisions & VS_FP64)) op->fN64.release();
     *out = op.release();
@@ -327,7 +325,7 @@ int64_t vs_op_nunknowns(const vs_op* op) { return op ? op->nunk : -1; }
 int vs_mvp(vs_op* op, const double* x, double* y, int transp) {
   return guarded([&] {
     VS_CHECK(op && x && y, VS_ERR_ARG, "null pointer");
-    VS_CHECK(op->fN64.p != nullptr, VS_ERR_ARG, "vs_mvp needs an FP64 tensor (precisions & VS_FP64)");
+    VS_CHECK(op->fc64.p != nullptr, VS_ERR_ARG, "vs_mvp needs an FP64 tensor (precisions & VS_FP64)");
     VS_CHECK(op->map3.p != nullptr, VS_ERR_ARG, "operator domain not set");
     DeviceGuard g(op->ctx->device);
     const size_t bytes = sizeof(c128) * (size_t)op->nunk;
@@ -467,6 +465,8 @@ int vs_pc_create_circ(vs_op* op, int levels, const double a[2], const double* w,
       assemble1(ctx, cL.as<c128>(), L, M, N, av, w ? wdev.as<c128>() : nullptr, ws, B.as<c128>());
     }
     invert_blocks_dev(ctx, B.as<c128>(), nblk, nb);
+    // a scalar weight keeps inv(a I - w C)^T = D inv(a I - w C) D (D = -1 on the z unknowns): pack it
+    pc->sym = levels == 2 && w == nullptr;
     pc_store_inverse(pc.get(), B.as<c128>());
     VS_HIP(hipStreamSynchronize(ctx->stream));
     *out = pc.release();

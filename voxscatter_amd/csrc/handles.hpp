@@ -48,7 +48,11 @@ struct vs_op {
   int precisions = 0;
   bool has_toep = false;
   vs::DevBuf toep;        // L*M*N*6 complex128
-  vs::DevBuf fN64, fN32;  // 6 * P complex, MATLAB layout (2L x 2M x 2N x 6)
+  vs::DevBuf fN64;        // 6 * P complex, MATLAB layout (2L x 2M x 2N x 6), FP64 (kept for download)
+  // octant of the tensor actually read by the matvec: kx <= L, ky <= M, kz <= N, 6 * Pc values; the
+  // other octants follow from the parity of each component (fN(2L-kx) = s_x fN(kx), etc.)
+  vs::DevBuf fc64, fc32;
+  int64_t Pc = 0;
   // domain
   int64_t nunk = 0;
   vs::DevBuf map3;  // int32[3V]: global unknown -> compressed position or -1
@@ -66,6 +70,7 @@ struct vs_pc {
   int64_t V = 0;
   int precisions = 0;
   int nb = 0;            // block size: 3N (2-level) or 3MN (1-level)
+  bool sym = false;      // 2-level blocks stored as packed upper triangles (B^T = D B D)
   vs::DevBuf inv64, inv32;  // apply layout (2-level: [ky][r][s][kx]; 1-level: [kx][r][s])
   vs::DevBuf map3;       // own copy of the unknown map
   int64_t nunk = 0;

@@ -3,6 +3,7 @@
 
 #include <cfloat>
 #include <cmath>
+#include <cstdlib>
 #include <limits>
 
 namespace vs {
@@ -26,32 +27,40 @@ static double eps_of() {
   return sizeof(T) == 8 ? DBL_EPSILON : (double)FLT_EPSILON;
 }
 
-// MATLAB bicgstab semantics: see the restatement of the built-in documented alongside the oracle.
+// MATLAB bicgstab semantics (the restatement of the built-in documented alongside the oracle), with
+// the scalars of each iteration formed on the device.  Fast path: both half steps are launched back to
+// back (replayed as one captured graph from the second iteration on) and the host reads the scalars
+// once per iteration, then replays MATLAB's control logic on them.  Whenever that logic takes the
+// half-step check branch (a recomputed true residual replaces s), the second half step is recomputed
+// from the true residual, exactly as MATLAB orders it; once a check has failed (moresteps > 0) every
+// half step is checked and the iterations run with one read-back per half step.
 template <typename T>
 SolveResult solve_bicgstab(Engine<T>& E, const LinOp<T>& op, const cplx<T>* b, cplx<T>* x, bool has_x0, double tol,
                            int maxit) {
   using cv = cplx<T>;
   const int64_t n = E.n;
+  vs_ctx* ctx = E.ctx;
   SolveResult R;
   const double eps = eps_of<T>();
   const double n2b = E.norm(b);
   if (n2b == 0) {
-    VS_HIP(hipMemsetAsync(x, 0, sizeof(cv) * n, E.ctx->stream));
+    VS_HIP(hipMemsetAsync(x, 0, sizeof(cv) * n, ctx->stream));
     R.flag = 0;
     R.relres = 0;
     R.iter = 0;
     R.resvec = {0.0};
     return R;
   }
-  if (!has_x0) VS_HIP(hipMemsetAsync(x, 0, sizeof(cv) * n, E.ctx->stream));
+  if (!has_x0) VS_HIP(hipMemsetAsync(x, 0, sizeof(cv) * n, ctx->stream));
   Vecs<T> W(n, 10);
   cv *r = W[0], *rt = W[1], *p = W[2], *v = W[3], *ph = W[4], *s = W[5], *sh = W[6], *t = W[7], *xh = W[8],
      *xmin = W[9];
+  const cv* phv = op.M ? ph : p;
+  const cv* shv = op.M ? sh : s;
   int flag = 1;
   E.copy(x, xmin);
   double imin = 0;
   const double tolb = tol * n2b;
-  // r = b - A x
   op.A(x, r);
   double normr = E.update(UPD_SUB_, b, nullptr, nullptr, nullptr, r, nullptr, C(0), C(0), true);
   double normr_act = normr;
@@ -69,69 +78,107 @@ SolveResult solve_bicgstab(Engine<T>& E, const LinOp<T>& op, const cplx<T>* b, c
   const int64_t maxmsteps = std::min<int64_t>(std::min<int64_t>(n / 50, 10), n - maxit);
   const int maxstagsteps = 3;
   double normrmin = normr;
-  c128 rho = C(1), omega = C(1), alpha = C(0);
   int64_t nres = -1;
   double iter = 0;
   int ii = 0;
-  const cv* a1[1];
-  const cv* b1[1];
-  for (ii = 1; ii <= maxit; ++ii) {
-    const c128 rho1 = rho;
-    a1[0] = rt;
-    b1[0] = r;
-    E.dots(1, a1, b1, &rho);
-    if ((rho.x == 0 && rho.y == 0) || cisinf(rho)) {
-      flag = 4;
-      nres = 2 * ii - 1;
-      break;
-    }
-    if (ii == 1) {
-      E.copy(r, p);
-    } else {
-      const c128 beta = (cdiv(rho, rho1)) * cdiv(alpha, omega);
-      if ((beta.x == 0 && beta.y == 0) || !cfinite(beta)) {
-        flag = 4;
-        nres = 2 * ii - 1;
-        break;
-      }
-      E.update(UPD_BICG_P_, r, v, nullptr, nullptr, p, nullptr, omega, beta, false);
-    }
-    const cv* phv = p;
-    if (op.M) {
-      op.M(p, ph);
-      phv = ph;
-    }
+
+  DevBuf dsc(sizeof(BiScal));
+  BiScal* sc = dsc.as<BiScal>();
+  BiScal* hs = reinterpret_cast<BiScal*>(ctx->pinned);
+  BiScal h{};
+  h.rho = C(1);
+  h.rho1 = C(1);
+  h.omega = C(1);
+  h.alpha = C(0);
+  h.beta = C(0);
+  h.first = 1;
+  h.has_pc = op.M ? 1 : 0;
+  *reinterpret_cast<BiScal*>(ctx->pinned_up) = h;
+  VS_HIP(hipMemcpyAsync(sc, ctx->pinned_up, sizeof(BiScal), hipMemcpyHostToDevice, ctx->stream));
+
+  auto first_half = [&] {
+    bi_stage_rho<T>(E, rt, r, v, p, sc);
+    if (op.M) op.M(p, ph);
     op.A(phv, v);
-    c128 d3[3];
-    {
-      const cv* aa[3] = {rt, phv, x};
-      const cv* bb[3] = {v, phv, x};
-      E.dots(3, aa, bb, d3);
+    bi_stage_alpha_half<T>(E, rt, v, phv, x, r, xh, s, sc);
+  };
+  auto second_half = [&] {
+    if (op.M) op.M(s, sh);
+    op.A(shv, t);
+    bi_stage_omega_full<T>(E, t, s, shv, xh, x, r, sc);
+  };
+  auto readback = [&] {
+    VS_HIP(hipMemcpyAsync(hs, sc, sizeof(BiScal), hipMemcpyDeviceToHost, ctx->stream));
+    VS_HIP(hipStreamSynchronize(ctx->stream));
+    h = *hs;
+  };
+  auto clear_flag = [&] {
+    // a breakdown seen only by a speculative second half step must not block its recomputation
+    VS_HIP(hipStreamSynchronize(ctx->stream));
+    *reinterpret_cast<int*>(ctx->pinned_up) = 0;
+    VS_HIP(hipMemcpyAsync(&sc->flag, ctx->pinned_up, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+  };
+
+  const char* ge = getenv("VS_GRAPH");
+  bool use_graph = !(ge && ge[0] == '0');
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+
+  for (ii = 1; ii <= maxit; ++ii) {
+    const bool fast = moresteps == 0;
+    bool second_done = false;
+    if (fast) {
+      if (use_graph && ii >= 2 && !gexec) {
+        // capture one full iteration (both half steps and the scalar read-back)
+        if (hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+          bool ok = true;
+          try {
+            first_half();
+            second_half();
+            VS_HIP(hipMemcpyAsync(hs, sc, sizeof(BiScal), hipMemcpyDeviceToHost, ctx->stream));
+          } catch (const Error&) {
+            ok = false;
+          }
+          hipGraph_t g = nullptr;
+          const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+          if (ok && ec == hipSuccess && g && hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0) == hipSuccess) {
+            graph = g;
+          } else {
+            if (g) hipGraphDestroy(g);
+            gexec = nullptr;
+            use_graph = false;
+            (void)hipGetLastError();
+          }
+        } else {
+          use_graph = false;
+          (void)hipGetLastError();
+        }
+      }
+      if (gexec) {
+        VS_HIP(hipGraphLaunch(gexec, ctx->stream));
+        VS_HIP(hipStreamSynchronize(ctx->stream));
+        h = *hs;
+      } else {
+        first_half();
+        second_half();
+        readback();
+      }
+      second_done = true;
+    } else {
+      first_half();
+      readback();
     }
-    const c128 rtv = d3[0];
-    const double nph = std::sqrt(d3[1].x), nx = std::sqrt(d3[2].x);
-    if (op.M && !(std::isfinite(nph))) {
-      flag = 2;
+    // ---- first half step: rho, beta, alpha (breakdowns before the half step) ----
+    if (h.flag && h.brk <= 4) {
+      flag = h.flag;
       nres = 2 * ii - 1;
       break;
     }
-    if ((rtv.x == 0 && rtv.y == 0) || cisinf(rtv)) {
-      flag = 4;
-      nres = 2 * ii - 1;
-      break;
-    }
-    alpha = cdiv(rho, rtv);
-    if (cisinf(alpha)) {
-      flag = 4;
-      nres = 2 * ii - 1;
-      break;
-    }
-    if (cabs_(alpha) * nph < eps * nx)
+    if (cabs_(h.alpha) * h.nph < eps * h.nx)
       ++stag;
     else
       stag = 0;
-    // xhalf = x + alpha ph ; s = r - alpha v
-    normr = E.update(UPD_HALF_, x, phv, r, v, xh, s, alpha, C(0), true);
+    normr = h.ns;
     normr_act = normr;
     resvec[2 * ii - 1] = normr;
     if (normr <= tolb || stag >= maxstagsteps || moresteps) {
@@ -154,48 +201,29 @@ SolveResult solve_bicgstab(Engine<T>& E, const LinOp<T>& op, const cplx<T>* b, c
           break;
         }
       }
+      second_done = false;  // s was replaced by the true residual
     }
     if (normr_act < normrmin) {
       normrmin = normr_act;
       E.copy(xh, xmin);
       imin = ii - 0.5;
     }
-    const cv* shv = s;
-    if (op.M) {
-      op.M(s, sh);
-      shv = sh;
+    if (!second_done) {
+      if (h.flag) clear_flag();
+      second_half();
+      readback();
     }
-    op.A(shv, t);
-    c128 d4[4];
-    {
-      const cv* aa[4] = {t, t, shv, xh};
-      const cv* bb[4] = {t, s, shv, xh};
-      E.dots(4, aa, bb, d4);
-    }
-    const c128 tt = d4[0];
-    const double nsh = std::sqrt(d4[2].x), nxh = std::sqrt(d4[3].x);
-    if (op.M && !std::isfinite(nsh)) {
-      flag = 2;
+    // ---- second half step: omega ----
+    if (h.flag) {
+      flag = h.flag;
       nres = 2 * ii;
       break;
     }
-    if ((tt.x == 0 && tt.y == 0) || cisinf(tt)) {
-      flag = 4;
-      nres = 2 * ii;
-      break;
-    }
-    omega = cdiv(d4[1], tt);
-    if (cisinf(omega)) {
-      flag = 4;
-      nres = 2 * ii;
-      break;
-    }
-    if (cabs_(omega) * nsh < eps * nxh)
+    if (cabs_(h.omega) * h.nsh < eps * h.nxh)
       ++stag;
     else
       stag = 0;
-    // x = xhalf + omega sh ; r = s - omega t
-    normr = E.update(UPD_HALF_, xh, shv, s, t, x, r, omega, C(0), true);
+    normr = h.nr;
     normr_act = normr;
     resvec[2 * ii] = normr;
     if (normr <= tolb || stag >= maxstagsteps || moresteps) {
@@ -228,6 +256,8 @@ SolveResult solve_bicgstab(Engine<T>& E, const LinOp<T>& op, const cplx<T>* b, c
       break;
     }
   }
+  if (gexec) hipGraphExecDestroy(gexec);
+  if (graph) hipGraphDestroy(graph);
   if (ii > maxit) ii = maxit;
   if (nres < 0) nres = 2 * (int64_t)maxit + 1;
   double relres;
@@ -250,7 +280,7 @@ SolveResult solve_bicgstab(Engine<T>& E, const LinOp<T>& op, const cplx<T>* b, c
   R.iter = iter;
   resvec.resize((size_t)nres);
   R.resvec = std::move(resvec);
-  VS_HIP(hipStreamSynchronize(E.ctx->stream));
+  VS_HIP(hipStreamSynchronize(ctx->stream));
   return R;
 }
 
@@ -295,6 +325,7 @@ SolveResult solve_gmres(Engine<T>& E, const LinOp<T>& op, const cplx<T>* b, cplx
   if (!has_x0) VS_HIP(hipMemsetAsync(x, 0, sizeof(cv) * n, E.ctx->stream));
   Vecs<T> Vb(n, restart + 1);
   Vecs<T> W(n, 3);
+  DevBuf hdev(sizeof(c128) * 1025);
   cv *r = W[0], *w = W[1], *tmp = W[2];
   auto precond = [&](const cv* in, cv* out) {
     if (op.M)
@@ -328,17 +359,23 @@ SolveResult solve_gmres(Engine<T>& E, const LinOp<T>& op, const cplx<T>* b, cplx
     for (int k = 0; k < restart; ++k) {
       op.A(Vb[k], tmp);
       precond(tmp, w);
-      // CGS2
-      std::vector<c128> h(k + 1), h2(k + 1);
-      E.mdot(Vb[0], n, k + 1, w, h.data());
-      E.gemv_acc(Vb[0], n, k + 1, h.data(), -1.0, w);
-      E.mdot(Vb[0], n, k + 1, w, h2.data());
-      E.gemv_acc(Vb[0], n, k + 1, h2.data(), -1.0, w);
-      for (int i = 0; i <= k; ++i) h[i] = h[i] + h2[i];
-      const double hn = E.norm(w);
+      // CGS2 with the projection coefficients kept on the device; one read-back per Arnoldi step
+      c128* h1d = hdev.as<c128>();
+      c128* h2d = h1d + (k + 1);
+      c128* nrd = h2d + (k + 1);
+      E.mdot_dev(Vb[0], n, k + 1, w, h1d);
+      E.gemv_acc_dev(Vb[0], n, k + 1, h1d, -1.0, w);
+      E.mdot_dev(Vb[0], n, k + 1, w, h2d);
+      E.gemv_acc_dev(Vb[0], n, k + 1, h2d, -1.0, w);
+      E.norm2_dev(w, nrd);
+      E.scale_inv_norm_dev(w, Vb[k + 1], nrd);
+      std::vector<c128> hh(2 * (k + 1) + 1);
+      E.read(h1d, 2 * (k + 1) + 1, hh.data());
+      std::vector<c128> h(k + 1);
+      for (int i = 0; i <= k; ++i) h[i] = hh[i] + hh[k + 1 + i];
+      const double hn = std::sqrt(hh[2 * (k + 1)].x);
       for (int i = 0; i <= k; ++i) H[i + (size_t)ldh * k] = h[i];
       H[k + 1 + (size_t)ldh * k] = C(hn);
-      if (hn != 0) E.update(UPD_SCALE_, w, nullptr, nullptr, nullptr, Vb[k + 1], nullptr, C(1.0 / hn), C(0), false);
       for (int i = 0; i < k; ++i) {
         const c128 hi = H[i + (size_t)ldh * k], hi1 = H[i + 1 + (size_t)ldh * k];
         H[i + (size_t)ldh * k] = cconj(cs[i]) * hi + cconj(sn[i]) * hi1;

@@ -9,6 +9,7 @@
 //   mv_ix   : inverse FFT along x, keep x<L, y = Gram Mr J - Mc conv / P, scatter to the compressed vector
 // The zero padding is never materialised in HBM: pad values are produced in LDS and cropped values are
 // never written.
+#include "fft2.hpp"
 #include "fft_device.hpp"
 #include "handles.hpp"
 
@@ -197,13 +198,35 @@ struct IoMvIx {
   }
 };
 
+// Compressed tensor: the octant kx <= L, ky <= M, kz <= N of fN.  Mirrored points pick up the parity
+// sign of each component: xy is odd in x and y, xz in x and z, yz in y and z (gperiodic_coeff_nop.m).
+template <typename T>
+struct TensorC {
+  const cplx<T>* f;
+  int L, M, N;
+  int64_t Pc;
+  __device__ __forceinline__ void load6(int kx, int ky, int kz, cplx<T>* out) const {
+    const bool fx = kx > L, fy = ky > M, fz = kz > N;
+    const int ix = fx ? 2 * L - kx : kx, iy = fy ? 2 * M - ky : ky, iz = fz ? 2 * N - kz : kz;
+    const int64_t o = ix + (int64_t)(L + 1) * (iy + (int64_t)(M + 1) * iz);
+    out[0] = f[o];
+    out[1] = f[Pc + o];
+    out[2] = f[2 * Pc + o];
+    out[3] = f[3 * Pc + o];
+    out[4] = f[4 * Pc + o];
+    out[5] = f[5 * Pc + o];
+    if (fx != fy) out[1] = -out[1];
+    if (fx != fz) out[2] = -out[2];
+    if (fy != fz) out[4] = -out[4];
+  }
+};
+
 // ------------------------------------------------------------------------------------------------
 // fused z pass: FFT along z, 3x3 tensor product, inverse FFT along z
 // ------------------------------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(FFT_THREADS) void k_mv_fzmul(FftDev<T> plan, cplx<T>* S, const cplx<T>* fN,
-                                                          int L2, int M2, int N, int64_t P, int W,
-                                                          int64_t S_rhs) {
+__global__ __launch_bounds__(FFT_THREADS) void k_mv_fzmul(FftDev<T> plan, cplx<T>* S, TensorC<T> fN, int L2,
+                                                          int M2, int N, int W, int64_t S_rhs) {
   cplx<T>* buf = reinterpret_cast<cplx<T>*>(vs_smem);
   const int n = plan.n;  // 2N
   const int ld = 3 * W;
@@ -227,14 +250,13 @@ __global__ __launch_bounds__(FFT_THREADS) void k_mv_fzmul(FftDev<T> plan, cplx<T
   for (int e = threadIdx.x; e < W * n; e += FFT_THREADS) {
     const int w = e % W, kz = e / W;
     if (w < wv) {
-      const int64_t o = (int64_t)kx0 + w + (int64_t)L2 * (ky + (int64_t)M2 * kz);
-      const cplx<T> f0 = fN[o], f1 = fN[P + o], f2 = fN[2 * P + o];
-      const cplx<T> f3 = fN[3 * P + o], f4 = fN[4 * P + o], f5 = fN[5 * P + o];
+      cplx<T> f[6];
+      fN.load6(kx0 + w, ky, kz, f);
       cplx<T>* rowp = buf + kz * ld + w;
       const cplx<T> jx = rowp[0], jy = rowp[W], jz = rowp[2 * W];
-      rowp[0] = f0 * jx + f1 * jy + f2 * jz;
-      rowp[W] = f1 * jx + f3 * jy + f4 * jz;
-      rowp[2 * W] = f2 * jx + f4 * jy + f5 * jz;
+      rowp[0] = f[0] * jx + f[1] * jy + f[2] * jz;
+      rowp[W] = f[1] * jx + f[3] * jy + f[4] * jz;
+      rowp[2 * W] = f[2] * jx + f[4] * jy + f[5] * jz;
     }
   }
   __syncthreads();
@@ -243,6 +265,208 @@ __global__ __launch_bounds__(FFT_THREADS) void k_mv_fzmul(FftDev<T> plan, cplx<T
     const int w = e % W, q = e / W;
     const int pos = q % N, c = q / N;
     if (w < wv) Sb[c * cs + pos * zs + (int64_t)ky * L2 + kx0 + w] = buf[pos * ld + c * W + w];
+  }
+}
+
+
+
+template <typename To>
+__global__ void k_compress_tensor(const c128* __restrict__ fN, cplx<To>* __restrict__ fc, int L, int M, int N) {
+  const int64_t L1 = L + 1, M1 = M + 1, N1 = N + 1;
+  const int64_t Pc = L1 * M1 * N1;
+  const int64_t P = 8LL * L * M * N;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 6 * Pc) return;
+  const int p = (int)(t / Pc);
+  int64_t q = t - p * Pc;
+  const int64_t ix = q % L1;
+  q /= L1;
+  const int64_t iy = q % M1, iz = q / M1;
+  fc[t] = ccast<To>(fN[p * P + ix + 2LL * L * (iy + 2LL * M * iz)]);
+}
+
+// ------------------------------------------------------------------------------------------------
+// compile-time two-pass FFT kernels (lengths in VS_FFT2_SIZES)
+// ------------------------------------------------------------------------------------------------
+// complex elements of LDS payload per workgroup: 32 KB in both precisions
+template <typename T> constexpr int lds_budget() { return sizeof(T) == 8 ? 2048 : 4096; }
+constexpr int pow2_floor(int x) { return x >= 2 ? 2 * pow2_floor(x / 2) : 1; }
+
+template <typename T, int N>
+struct RowCfg {
+  static constexpr int R1 = Split<N>::R1, R2 = Split<N>::R2, TPL = cmax(R1, R2);
+  static constexpr int LPW0 = 256 / TPL < lds_budget<T>() / N ? 256 / TPL : lds_budget<T>() / N;
+  static constexpr int LPW = LPW0 < 1 ? 1 : LPW0;  // lines per workgroup
+  static constexpr int THREADS = TPL * LPW;
+};
+
+// lines along the contiguous axis; IO supplies load(line, pos, rhs) / store(line, k, rhs, v)
+template <typename T, int N, bool INV, class IO>
+__global__ __launch_bounds__(256) void k_row2(const cplx<T>* __restrict__ tw, IO io, int64_t nlines, int n_in,
+                                              int n_out) {
+  using C = RowCfg<T, N>;
+  constexpr int R1 = C::R1, R2 = C::R2, TPL = C::TPL, LPW = C::LPW, LP = LPW + 1;
+  __shared__ cplx<T> lds[N * LP];
+  const int t = threadIdx.x;
+  const int ll = t / TPL, s = t - ll * TPL;
+  const int64_t g = (int64_t)blockIdx.x * LPW + ll;
+  const int rhs = blockIdx.y;
+  const bool live = g < nlines;
+  if (s < R1) {
+    cplx<T> v[R2];
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) {
+      const int n = s + R1 * n2;
+      v[n2] = (live && n < n_in) ? io.load(g, n, rhs) : mk<T>(0, 0);
+    }
+    fft2_step1<T, N, INV>(v, s, tw);
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) lds[(s * R2 + k2) * LP + ll] = v[k2];
+  }
+  __syncthreads();
+  if (s < R2) {
+    cplx<T> u[R1];
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1) u[n1] = lds[(n1 * R2 + s) * LP + ll];
+    fft2_step2<T, N, INV>(u);
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) {
+      const int k = s + R2 * k1;
+      if (live && k < n_out) io.store(g, k, rhs, u[k1]);
+    }
+  }
+}
+
+template <typename T, int N>
+struct ColCfg {
+  static constexpr int R1 = Split<N>::R1, R2 = Split<N>::R2, TPL = cmax(R1, R2);
+  static constexpr int W0 = pow2_floor(256 / TPL < lds_budget<T>() / N ? 256 / TPL : lds_budget<T>() / N);
+  static constexpr int W = W0 > 64 ? 64 : W0;  // columns per workgroup
+  static constexpr int THREADS = TPL * W;
+};
+
+// in-place FFT along a strided axis: element (col, pos, batch) at data[rhs*rhs_stride + batch*sb + pos*sa + col]
+template <typename T, int N, bool INV>
+__global__ __launch_bounds__(256) void k_col2(const cplx<T>* __restrict__ tw, cplx<T>* data, int64_t fast, int64_t sa,
+                                              int64_t sb, int n_in, int n_out, T scale, int64_t rhs_stride) {
+  using C = ColCfg<T, N>;
+  constexpr int R1 = C::R1, R2 = C::R2, W = C::W;
+  __shared__ cplx<T> lds[N * W];
+  const int t = threadIdx.x;
+  const int col = t % W, s = t / W;
+  const int64_t tiles = (fast + W - 1) / W;
+  const int64_t tile = blockIdx.x % tiles, b = blockIdx.x / tiles;
+  const int64_t c0 = tile * W;
+  const bool live = c0 + col < fast;
+  cplx<T>* base = data + (int64_t)blockIdx.y * rhs_stride + b * sb + c0 + col;
+  if (s < R1) {
+    cplx<T> v[R2];
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) {
+      const int n = s + R1 * n2;
+      v[n2] = (live && n < n_in) ? base[n * sa] : mk<T>(0, 0);
+    }
+    fft2_step1<T, N, INV>(v, s, tw);
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) lds[(s * R2 + k2) * W + col] = v[k2];
+  }
+  __syncthreads();
+  if (s < R2) {
+    cplx<T> u[R1];
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1) u[n1] = lds[(n1 * R2 + s) * W + col];
+    fft2_step2<T, N, INV>(u);
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) {
+      const int k = s + R2 * k1;
+      if (live && k < n_out) base[k * sa] = scale * u[k1];
+    }
+  }
+}
+
+template <typename T, int N2>
+struct FzCfg {
+  static constexpr int R1 = Split<N2>::R1, R2 = Split<N2>::R2, TPL = cmax(R1, R2);
+  static constexpr int WA = 512 / (3 * TPL), WB = lds_budget<T>() / (3 * N2);
+  static constexpr int W0 = pow2_floor(WA < WB ? WA : WB);
+  static constexpr int W = W0 > 32 ? 32 : (W0 < 1 ? 1 : W0);  // columns (kx) per workgroup
+  static constexpr int THREADS = 3 * TPL * W;
+};
+
+// z pass of the matvec: zero-pad z to 2N, FFT, 3x3 symmetric tensor product, inverse FFT, crop to N
+template <typename T, int N2>
+__global__ __launch_bounds__(512) void k_fz2(const cplx<T>* __restrict__ tw, cplx<T>* S, TensorC<T> fN, int L2, int M2,
+                                             int N, int64_t S_rhs) {
+  using C = FzCfg<T, N2>;
+  constexpr int R1 = C::R1, R2 = C::R2, W = C::W;
+  __shared__ cplx<T> xch[3 * N2 * W];
+  __shared__ cplx<T> spec[3 * N2 * W];
+  const int t = threadIdx.x;
+  const int col = t % W, r = t / W;
+  const int c = r % 3, s = r / 3;
+  const int tiles = (L2 + W - 1) / W;
+  const int tile = blockIdx.x % tiles, ky = blockIdx.x / tiles;
+  const int kx0 = tile * W;
+  const bool live = kx0 + col < L2;
+  cplx<T>* Sb = S + (int64_t)blockIdx.y * S_rhs;
+  const int64_t zs = (int64_t)L2 * M2, cs = zs * N;
+  cplx<T>* gcol = Sb + c * cs + (int64_t)ky * L2 + kx0 + col;
+  // forward transform of the three components
+  if (s < R1) {
+    cplx<T> v[R2];
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) {
+      const int n = s + R1 * n2;
+      v[n2] = (live && n < N) ? gcol[n * zs] : mk<T>(0, 0);
+    }
+    fft2_step1<T, N2, false>(v, s, tw);
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) xch[(c * N2 + s * R2 + k2) * W + col] = v[k2];
+  }
+  __syncthreads();
+  if (s < R2) {
+    cplx<T> u[R1];
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1) u[n1] = xch[(c * N2 + n1 * R2 + s) * W + col];
+    fft2_step2<T, N2, false>(u);
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) spec[(c * N2 + s + R2 * k1) * W + col] = u[k1];
+  }
+  __syncthreads();
+  // tensor product, one (kx, kz) per iteration
+  for (int e = t; e < W * N2; e += C::THREADS) {
+    const int cc = e % W, kz = e / W;
+    if (kx0 + cc < L2) {
+      cplx<T> f[6];
+      fN.load6(kx0 + cc, ky, kz, f);
+      cplx<T>* sp = spec + kz * W + cc;
+      const cplx<T> jx = sp[0], jy = sp[N2 * W], jz = sp[2 * N2 * W];
+      sp[0] = f[0] * jx + f[1] * jy + f[2] * jz;
+      sp[N2 * W] = f[1] * jx + f[3] * jy + f[4] * jz;
+      sp[2 * N2 * W] = f[2] * jx + f[4] * jy + f[5] * jz;
+    }
+  }
+  __syncthreads();
+  // inverse transform, keep z < N
+  if (s < R1) {
+    cplx<T> v[R2];
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) v[n2] = spec[(c * N2 + s + R1 * n2) * W + col];
+    fft2_step1<T, N2, true>(v, s, tw);
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) xch[(c * N2 + s * R2 + k2) * W + col] = v[k2];
+  }
+  __syncthreads();
+  if (s < R2) {
+    cplx<T> u[R1];
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1) u[n1] = xch[(c * N2 + n1 * R2 + s) * W + col];
+    fft2_step2<T, N2, true>(u);
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) {
+      const int k = s + R2 * k1;
+      if (live && k < N) gcol[k * zs] = u[k1];
+    }
   }
 }
 
@@ -361,53 +585,93 @@ static int col_tile(int n, int64_t fast, int64_t nbatch, size_t eb) {
   return W;
 }
 
-template <typename T, class IO>
-static void run_rows(vs_ctx* ctx, const IO& io, int n, int64_t nlines, int n_in, int n_out, bool inv, int nrhs) {
+template <typename T, bool INV, class IO>
+static void run_rows(vs_ctx* ctx, const IO& io, int n, int64_t nlines, int n_in, int n_out, int nrhs) {
   const FftPlanHost& ph = ctx->plan(n);
   const FftDev<T> pd = ph.dev<T>();
+  switch (n) {
+#define VS_ROW_CASE(NN)                                                                                  \
+  case NN: {                                                                                             \
+    using Cf = RowCfg<T, NN>;                                                                            \
+    dim3 grid((unsigned)((nlines + Cf::LPW - 1) / Cf::LPW), (unsigned)nrhs);                            \
+    hipLaunchKernelGGL((k_row2<T, NN, INV, IO>), grid, dim3(Cf::THREADS), 0, ctx->stream, pd.tw, io,   \
+                       nlines, n_in, n_out);                                                             \
+    VS_HIP(hipGetLastError());                                                                           \
+    return;                                                                                              \
+  }
+    VS_FFT2_SIZES(VS_ROW_CASE)
+#undef VS_ROW_CASE
+    default: break;
+  }
   VS_CHECK(n <= tile_budget(sizeof(cplx<T>)), VS_ERR_UNSUPPORTED, "FFT length exceeds the LDS tile");
   const int nl = row_tile(n, nlines, sizeof(cplx<T>));
   const size_t lds = sizeof(cplx<T>) * (size_t)(nl + 1) * n;
   dim3 grid((unsigned)((nlines + nl - 1) / nl), (unsigned)nrhs);
   hipLaunchKernelGGL((k_row_fft<T, IO>), grid, dim3(FFT_THREADS), lds, ctx->stream, pd, io, nlines, nl, n_in,
-                     n_out, inv ? 1 : 0);
+                     n_out, INV ? 1 : 0);
   VS_HIP(hipGetLastError());
 }
 
-template <typename T>
+template <typename T, bool INV>
 static void run_cols(vs_ctx* ctx, cplx<T>* data, int64_t fast, int64_t sa, int64_t nbatch, int64_t sb, int n_in,
-                     int n, int n_out, bool inv, double scale, int nrhs, int64_t rhs_stride) {
+                     int n, int n_out, double scale, int nrhs, int64_t rhs_stride) {
   const FftPlanHost& ph = ctx->plan(n);
   const FftDev<T> pd = ph.dev<T>();
+  switch (n) {
+#define VS_COL_CASE(NN)                                                                                  \
+  case NN: {                                                                                             \
+    using Cf = ColCfg<T, NN>;                                                                            \
+    const int64_t tiles = (fast + Cf::W - 1) / Cf::W;                                                    \
+    dim3 grid((unsigned)(tiles * nbatch), (unsigned)nrhs);                                               \
+    hipLaunchKernelGGL((k_col2<T, NN, INV>), grid, dim3(Cf::THREADS), 0, ctx->stream, pd.tw, data, fast, sa, \
+                       sb, n_in, n_out, (T)scale, rhs_stride);                                           \
+    VS_HIP(hipGetLastError());                                                                           \
+    return;                                                                                              \
+  }
+    VS_FFT2_SIZES(VS_COL_CASE)
+#undef VS_COL_CASE
+    default: break;
+  }
   VS_CHECK(n <= tile_budget(sizeof(cplx<T>)), VS_ERR_UNSUPPORTED, "FFT length exceeds the LDS tile");
   const int W = col_tile(n, fast, nbatch, sizeof(cplx<T>));
   const size_t lds = sizeof(cplx<T>) * (size_t)W * n;
   const int64_t tiles = (fast + W - 1) / W;
   dim3 grid((unsigned)(tiles * nbatch), (unsigned)nrhs);
   hipLaunchKernelGGL((k_col_fft<T>), grid, dim3(FFT_THREADS), lds, ctx->stream, pd, data, fast, sa, sb, W, n_in,
-                     n_out, inv ? 1 : 0, (T)scale, rhs_stride);
+                     n_out, INV ? 1 : 0, (T)scale, rhs_stride);
   VS_HIP(hipGetLastError());
+}
+
+template <typename T>
+static void run_cols_dir(vs_ctx* ctx, cplx<T>* data, int64_t fast, int64_t sa, int64_t nbatch, int64_t sb, int n_in,
+                         int n, int n_out, bool inv, double scale, int nrhs, int64_t rhs_stride) {
+  if (inv)
+    run_cols<T, true>(ctx, data, fast, sa, nbatch, sb, n_in, n, n_out, scale, nrhs, rhs_stride);
+  else
+    run_cols<T, false>(ctx, data, fast, sa, nbatch, sb, n_in, n, n_out, scale, nrhs, rhs_stride);
 }
 
 void launch_col_fft(vs_ctx* ctx, int prec, void* data, int64_t fast, int64_t axis_stride, int64_t nbatch,
                     int64_t batch_stride, int n_in, int n, int n_out, bool inv, double scale, int nrhs,
                     int64_t rhs_stride) {
   if (prec == VS_FP64)
-    run_cols<double>(ctx, (c128*)data, fast, axis_stride, nbatch, batch_stride, n_in, n, n_out, inv, scale, nrhs,
-                     rhs_stride);
+    run_cols_dir<double>(ctx, (c128*)data, fast, axis_stride, nbatch, batch_stride, n_in, n, n_out, inv, scale,
+                         nrhs, rhs_stride);
   else
-    run_cols<float>(ctx, (c64*)data, fast, axis_stride, nbatch, batch_stride, n_in, n, n_out, inv, scale, nrhs,
-                    rhs_stride);
+    run_cols_dir<float>(ctx, (c64*)data, fast, axis_stride, nbatch, batch_stride, n_in, n, n_out, inv, scale, nrhs,
+                        rhs_stride);
 }
 
 void launch_row_fft_dense(vs_ctx* ctx, int prec, void* data, int64_t nlines, int n_in, int n, int n_out,
                           int64_t ld_in, int64_t ld_out, bool inv, double scale, int nrhs, int64_t rhs_stride) {
   if (prec == VS_FP64) {
     IoDense<double> io{(c128*)data, (c128*)data, ld_in, ld_out, rhs_stride, scale};
-    run_rows<double>(ctx, io, n, nlines, n_in, n_out, inv, nrhs);
+    if (inv) run_rows<double, true>(ctx, io, n, nlines, n_in, n_out, nrhs);
+    else run_rows<double, false>(ctx, io, n, nlines, n_in, n_out, nrhs);
   } else {
     IoDense<float> io{(c64*)data, (c64*)data, ld_in, ld_out, rhs_stride, (float)scale};
-    run_rows<float>(ctx, io, n, nlines, n_in, n_out, inv, nrhs);
+    if (inv) run_rows<float, true>(ctx, io, n, nlines, n_in, n_out, nrhs);
+    else run_rows<float, false>(ctx, io, n, nlines, n_in, n_out, nrhs);
   }
 }
 
@@ -415,10 +679,10 @@ void launch_row_fft_gather(vs_ctx* ctx, int prec, const void* x, const int* map3
                            int64_t nlines, int nrhs, int64_t out_rhs_stride) {
   if (prec == VS_FP64) {
     IoGather<double> io{(const c128*)x, map3, nunk, L, (c128*)out, out_rhs_stride};
-    run_rows<double>(ctx, io, L, nlines, L, L, false, nrhs);
+    run_rows<double, false>(ctx, io, L, nlines, L, L, nrhs);
   } else {
     IoGather<float> io{(const c64*)x, map3, nunk, L, (c64*)out, out_rhs_stride};
-    run_rows<float>(ctx, io, L, nlines, L, L, false, nrhs);
+    run_rows<float, false>(ctx, io, L, nlines, L, L, nrhs);
   }
 }
 
@@ -426,16 +690,40 @@ void launch_row_ifft_scatter(vs_ctx* ctx, int prec, const void* in, const int* m
                              int64_t nlines, double scale, int nrhs, int64_t in_rhs_stride) {
   if (prec == VS_FP64) {
     IoScatter<double> io{(const c128*)in, map3, nunk, L, (c128*)y, in_rhs_stride, scale};
-    run_rows<double>(ctx, io, L, nlines, L, L, true, nrhs);
+    run_rows<double, true>(ctx, io, L, nlines, L, L, nrhs);
   } else {
     IoScatter<float> io{(const c64*)in, map3, nunk, L, (c64*)y, in_rhs_stride, (float)scale};
-    run_rows<float>(ctx, io, L, nlines, L, L, true, nrhs);
+    run_rows<float, true>(ctx, io, L, nlines, L, L, nrhs);
   }
 }
 
 // ------------------------------------------------------------------------------------------------
 // matvec driver
 // ------------------------------------------------------------------------------------------------
+template <typename T>
+static bool launch_fz2(vs_ctx* ctx, cplx<T>* S, const TensorC<T>& fN, int L2, int M2, int N, int nrhs,
+                       int64_t S_rhs) {
+  const int N2 = 2 * N;
+  const FftPlanHost& ph = ctx->plan(N2);
+  const FftDev<T> pd = ph.dev<T>();
+  switch (N2) {
+#define VS_FZ_CASE(NN)                                                                                   \
+  case NN: {                                                                                             \
+    using Cf = FzCfg<T, NN>;                                                                             \
+    if (3 * NN * Cf::W > lds_budget<T>()) return false;                                                  \
+    const int tiles = (L2 + Cf::W - 1) / Cf::W;                                                          \
+    dim3 grid((unsigned)(tiles * M2), (unsigned)nrhs);                                                   \
+    hipLaunchKernelGGL((k_fz2<T, NN>), grid, dim3(Cf::THREADS), 0, ctx->stream, pd.tw, S, fN, L2, M2, N, \
+                       S_rhs);                                                                           \
+    VS_HIP(hipGetLastError());                                                                           \
+    return true;                                                                                         \
+  }
+    VS_FFT2_SIZES(VS_FZ_CASE)
+#undef VS_FZ_CASE
+    default: return false;
+  }
+}
+
 template <typename T>
 static void mvp_impl(vs_op* op, const cplx<T>* x, cplx<T>* y, int transp, int nrhs) {
   vs_ctx* ctx = op->ctx;
@@ -444,18 +732,23 @@ static void mvp_impl(vs_op* op, const cplx<T>* x, cplx<T>* y, int transp, int nr
   const int64_t S_rhs = (int64_t)L2 * M2 * N * 3;
   op->S.ensure(sizeof(cplx<T>) * S_rhs * nrhs);
   cplx<T>* S = op->S.as<cplx<T>>();
-  const cplx<T>* fN = (sizeof(T) == 8) ? op->fN64.as<cplx<T>>() : op->fN32.as<cplx<T>>();
-  VS_CHECK(fN != nullptr, VS_ERR_ARG, "operator holds no tensor in the requested precision");
+  TensorC<T> fN;
+  fN.f = (sizeof(T) == 8) ? op->fc64.as<cplx<T>>() : op->fc32.as<cplx<T>>();
+  VS_CHECK(fN.f != nullptr, VS_ERR_ARG, "operator holds no tensor in the requested precision");
+  fN.L = L;
+  fN.M = M;
+  fN.N = N;
+  fN.Pc = op->Pc;
   const int* map3 = op->map3.as<int>();
   VS_CHECK(map3 != nullptr, VS_ERR_ARG, "operator domain not set (vs_op_set_domain)");
   const int64_t lines = (int64_t)M * N * 3;
   // 1. gather + x FFT
   IoMvFx<T> fx{x, map3, op->nunk, op->V, L, M, transp, coef_dev<T>(op->b), S, S_rhs};
-  run_rows<T>(ctx, fx, L2, lines, L, L2, false, nrhs);
+  run_rows<T, false>(ctx, fx, L2, lines, L, L2, nrhs);
   // 2. y FFT: columns (kx), axis y (stride 2L), batch (z, c) stride 2L*2M, input y < M
-  run_cols<T>(ctx, S, L2, L2, (int64_t)N * 3, (int64_t)L2 * M2, M, M2, M2, false, 1.0, nrhs, S_rhs);
+  run_cols<T, false>(ctx, S, L2, L2, (int64_t)N * 3, (int64_t)L2 * M2, M, M2, M2, 1.0, nrhs, S_rhs);
   // 3. fused z FFT + tensor product + inverse z FFT
-  {
+  if (!launch_fz2<T>(ctx, S, fN, L2, M2, N, nrhs, S_rhs)) {
     const FftPlanHost& ph = ctx->plan(N2);
     const FftDev<T> pd = ph.dev<T>();
     VS_CHECK(3 * N2 <= tile_budget(sizeof(cplx<T>)), VS_ERR_UNSUPPORTED, "z FFT length exceeds the LDS tile");
@@ -467,16 +760,15 @@ static void mvp_impl(vs_op* op, const cplx<T>* x, cplx<T>* y, int transp, int nr
     const size_t lds = sizeof(cplx<T>) * (size_t)3 * W * N2;
     const int tiles = (L2 + W - 1) / W;
     dim3 grid((unsigned)(tiles * M2), (unsigned)nrhs);
-    hipLaunchKernelGGL((k_mv_fzmul<T>), grid, dim3(FFT_THREADS), lds, ctx->stream, pd, S, fN, L2, M2, N, op->P, W,
-                       S_rhs);
+    hipLaunchKernelGGL((k_mv_fzmul<T>), grid, dim3(FFT_THREADS), lds, ctx->stream, pd, S, fN, L2, M2, N, W, S_rhs);
     VS_HIP(hipGetLastError());
   }
   // 4. inverse y FFT, keep y < M
-  run_cols<T>(ctx, S, L2, L2, (int64_t)N * 3, (int64_t)L2 * M2, M2, M2, M, true, 1.0, nrhs, S_rhs);
+  run_cols<T, true>(ctx, S, L2, L2, (int64_t)N * 3, (int64_t)L2 * M2, M2, M2, M, 1.0, nrhs, S_rhs);
   // 5. inverse x FFT, crop, diagonal, scatter
   IoMvIx<T> ix{x, y, map3, op->nunk, op->V, L, M, transp, coef_dev<T>(op->a), coef_dev<T>(op->at),
                coef_dev<T>(op->b), S, S_rhs, (T)(1.0 / (double)op->P)};
-  run_rows<T>(ctx, ix, L2, lines, L2, L, true, nrhs);
+  run_rows<T, true>(ctx, ix, L2, lines, L2, L, nrhs);
 }
 
 void mvp_dev(vs_op* op, const void* x, void* y, int transp, int precision, int nrhs) {
@@ -514,6 +806,26 @@ static void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w
   }
 }
 
+// octant copies of fN64 in the precisions the operator keeps
+void op_compress_tensor(vs_op* op) {
+  vs_ctx* ctx = op->ctx;
+  op->Pc = (int64_t)(op->L + 1) * (op->M + 1) * (op->N + 1);
+  const int64_t tot = 6 * op->Pc;
+  const unsigned g = (unsigned)((tot + 255) / 256);
+  if (op->precisions & VS_FP64) {
+    op->fc64.ensure(sizeof(c128) * tot);
+    hipLaunchKernelGGL((k_compress_tensor<double>), dim3(g), dim3(256), 0, ctx->stream, op->fN64.as<c128>(),
+                       op->fc64.as<c128>(), op->L, op->M, op->N);
+    VS_HIP(hipGetLastError());
+  }
+  if (op->precisions & VS_FP32) {
+    op->fc32.ensure(sizeof(c64) * tot);
+    hipLaunchKernelGGL((k_compress_tensor<float>), dim3(g), dim3(256), 0, ctx->stream, op->fN64.as<c128>(),
+                       op->fc32.as<c64>(), op->L, op->M, op->N);
+    VS_HIP(hipGetLastError());
+  }
+}
+
 void op_build_fN(vs_op* op) {
   // circulant embedding of the Toeplitz tensor and its 3-D FFT, in FP64 (fft_operator.m:1-8)
   vs_ctx* ctx = op->ctx;
@@ -530,12 +842,7 @@ void op_build_fN(vs_op* op) {
   launch_row_fft_dense(ctx, VS_FP64, d, (int64_t)M2 * N2 * 6, L2, L2, L2, L2, L2, false, 1.0, 1, 0);
   launch_col_fft(ctx, VS_FP64, d, L2, L2, (int64_t)N2 * 6, (int64_t)L2 * M2, M2, M2, M2, false, 1.0, 1, 0);
   launch_col_fft(ctx, VS_FP64, d, (int64_t)L2 * M2, (int64_t)L2 * M2, 6, P, N2, N2, N2, false, 1.0, 1, 0);
-  if (op->precisions & VS_FP32) {
-    op->fN32.ensure(sizeof(c64) * 6 * P);
-    hipLaunchKernelGGL((k_convert<float, double>), dim3(2048), dim3(256), 0, ctx->stream, op->fN64.as<c128>(),
-                       op->fN32.as<c64>(), 6 * P);
-    VS_HIP(hipGetLastError());
-  }
+  op_compress_tensor(op);
 }
 
 void op_build_dda(vs_op* op, const double* r_host, double ko, int nearby) {

@@ -311,6 +311,25 @@ __global__ void k_relayout2(const c128* __restrict__ in, cplx<To>* __restrict__ 
   out[t] = ccast<To>(in[((int64_t)kx + (int64_t)L * ky) * nn + r + (int64_t)n * s]);
 }
 
+// column-major blocks -> packed upper triangles, 2-level apply layout: entry (r <= s) of block (kx, ky) at
+// ((ky*NT + t)*L + kx), t = r*(2n - r + 1)/2 + (s - r), NT = n(n+1)/2.  Valid for blocks with
+// B^T = D B D, D = diag(1 (x, y rows), -1 (z rows)): inv(a I - w C) for a scalar w (C^T = D C D).
+template <typename To>
+__global__ void k_relayout2_sym(const c128* __restrict__ in, cplx<To>* __restrict__ out, int L, int M, int n) {
+  const int64_t nn = (int64_t)n * n;
+  const int64_t NT = (int64_t)n * (n + 1) / 2;
+  const int64_t tot = (int64_t)L * M * nn;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= tot) return;
+  const int64_t blk = e / nn;
+  const int q = (int)(e - blk * nn);
+  const int r = q % n, sidx = q / n;  // column-major source entry (r, s)
+  if (r > sidx) return;
+  const int kx = (int)(blk % L), ky = (int)(blk / L);
+  const int64_t t = (int64_t)r * (2 * n - r + 1) / 2 + (sidx - r);
+  out[((int64_t)ky * NT + t) * L + kx] = ccast<To>(in[blk * nn + q]);
+}
+
 // column-major blocks -> 1-level apply layout (row-major per block)
 template <typename To>
 __global__ void k_relayout1(const c128* __restrict__ in, cplx<To>* __restrict__ out, int64_t nblk, int64_t n) {
@@ -349,6 +368,46 @@ __global__ __launch_bounds__(256) void k_blockmv2(const cplx<T>* __restrict__ B,
     cplx<T> acc = mk<T>(0, 0);
     for (int s = 0; s < n; ++s) acc = acc + br[(int64_t)s * L] * u[s * W + w];
     Ub[r * sstride + w] = acc;
+  }
+}
+
+// same with packed symmetric blocks: B(r,s) = B_up(r,s) for s >= r, d_r d_s B_up(s,r) otherwise, where
+// d = -1 on the z rows (index >= nz0 = 2N)
+template <typename T>
+__global__ __launch_bounds__(256) void k_blockmv2_sym(const cplx<T>* __restrict__ B, cplx<T>* __restrict__ U, int L,
+                                                      int M, int n, int W, int64_t rhs_stride) {
+  cplx<T>* u = reinterpret_cast<cplx<T>*>(vs_smem);
+  const int tiles = (L + W - 1) / W;
+  const int t = blockIdx.x % tiles, ky = blockIdx.x / tiles;
+  const int kx0 = t * W;
+  const int wv = min(W, L - kx0);
+  cplx<T>* Ub = U + (int64_t)blockIdx.y * rhs_stride + kx0 + (int64_t)L * ky;
+  const int64_t sstride = (int64_t)L * M;
+  for (int e = threadIdx.x; e < n * W; e += blockDim.x) {
+    const int w = e % W, s = e / W;
+    u[e] = w < wv ? Ub[s * sstride + w] : mk<T>(0, 0);
+  }
+  __syncthreads();
+  const int64_t NT = (int64_t)n * (n + 1) / 2;
+  const int nz0 = 2 * (n / 3);
+  const cplx<T>* Bk = B + (int64_t)ky * NT * L + kx0;
+  for (int e = threadIdx.x; e < n * W; e += blockDim.x) {
+    const int w = e % W, r = e / W;
+    if (w >= wv) continue;
+    const bool rz = r >= nz0;
+    cplx<T> acc0 = mk<T>(0, 0), acc1 = mk<T>(0, 0);
+    // s < r: transposed entries (s, r), t(s, r) = s(2n - s + 1)/2 + (r - s)
+    int64_t ts = r;  // t(0, r)
+    for (int s = 0; s < r; ++s) {
+      cplx<T> b = Bk[ts * L + w];
+      if (rz != (s >= nz0)) b = -b;
+      acc0 = acc0 + b * u[s * W + w];
+      ts += n - s - 1;
+    }
+    // s >= r: row r of the upper triangle, contiguous in t
+    const int64_t t0 = (int64_t)r * (2 * n - r + 1) / 2;
+    for (int s = r; s < n; ++s) acc1 = acc1 + Bk[(t0 + s - r) * L + w] * u[s * W + w];
+    Ub[r * sstride + w] = acc0 + acc1;
   }
 }
 
@@ -443,8 +502,12 @@ void pc_store_inverse(vs_pc* pc, const c128* inv) {
   const int64_t tot = nblk * n * n;
   auto go = [&](auto tag, DevBuf& dst) {
     using To = decltype(tag);
-    dst.ensure(sizeof(cplx<To>) * tot);
-    if (pc->levels == 2)
+    const int64_t stored = (pc->levels == 2 && pc->sym) ? nblk * n * (n + 1) / 2 : tot;
+    dst.ensure(sizeof(cplx<To>) * stored);
+    if (pc->levels == 2 && pc->sym)
+      hipLaunchKernelGGL((k_relayout2_sym<To>), dim3(nblocks(tot)), dim3(256), 0, ctx->stream, inv,
+                         dst.as<cplx<To>>(), pc->L, pc->M, (int)n);
+    else if (pc->levels == 2)
       hipLaunchKernelGGL((k_relayout2<To>), dim3(nblocks(tot)), dim3(256), 0, ctx->stream, inv, dst.as<cplx<To>>(),
                          pc->L, pc->M, (int)n);
     else
@@ -474,11 +537,15 @@ static void pc_apply_impl(vs_pc* pc, const cplx<T>* x, cplx<T>* y, int nrhs) {
     const int n = pc->nb;
     int W = std::min(64, L);
     while (W > 1 && (size_t)n * W * sizeof(cplx<T>) > 65536) W /= 2;
-    while (W > 8 && (int64_t)((L + W - 1) / W) * M < 256) W /= 2;
+    while (W > 16 && (int64_t)((L + W - 1) / W) * M < 1024) W /= 2;
     const size_t lds = sizeof(cplx<T>) * (size_t)n * W;
     const int tiles = (L + W - 1) / W;
-    hipLaunchKernelGGL((k_blockmv2<T>), dim3((unsigned)(tiles * M), (unsigned)nrhs), dim3(256), lds, ctx->stream, B,
-                       U, L, M, n, W, V3);
+    if (pc->sym)
+      hipLaunchKernelGGL((k_blockmv2_sym<T>), dim3((unsigned)(tiles * M), (unsigned)nrhs), dim3(256), lds,
+                         ctx->stream, B, U, L, M, n, W, V3);
+    else
+      hipLaunchKernelGGL((k_blockmv2<T>), dim3((unsigned)(tiles * M), (unsigned)nrhs), dim3(256), lds, ctx->stream,
+                         B, U, L, M, n, W, V3);
     VS_HIP(hipGetLastError());
     launch_col_fft(ctx, prec, U, L, L, (int64_t)N * 3, (int64_t)L * M, M, M, M, true, 1.0, nrhs, V3);
     launch_row_ifft_scatter(ctx, prec, U, map3, pc->nunk, y, L, lines, 1.0 / ((double)L * M), nrhs, V3);

@@ -203,6 +203,16 @@ __global__ __launch_bounds__(RED_THREADS) void k_gemv_acc(const cplx<T>* V, int6
   }
 }
 
+// w *= 1 / sqrt(nrm2[0].x) into y (Arnoldi normalisation with the norm left on the device)
+template <typename T>
+__global__ __launch_bounds__(RED_THREADS) void k_scale_inv_norm(const cplx<T>* w, cplx<T>* y, int64_t n,
+                                                                const c128* nrm2) {
+  const double nn = sqrt(nrm2[0].x);
+  const T s = (T)(1.0 / nn);
+  for (int64_t i = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * RED_THREADS)
+    y[i] = s * w[i];
+}
+
 template <typename To, typename Ti>
 __global__ void k_cvt(const cplx<Ti>* __restrict__ in, cplx<To>* __restrict__ out, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -216,8 +226,8 @@ template <typename T>
 Engine<T>::Engine(vs_ctx* c, int64_t n_) : ctx(c), n(n_), grid(red_grid(n_)) {
   ctx->red.ensure(sizeof(c128) * 1024 * 512);
   ctx->red_out.ensure(sizeof(c128) * 512);
-  if (!ctx->pinned) VS_HIP(hipHostMalloc(&ctx->pinned, sizeof(c128) * 512, hipHostMallocDefault));
-  if (!ctx->pinned_up) VS_HIP(hipHostMalloc(&ctx->pinned_up, sizeof(c128) * 512, hipHostMallocDefault));
+  if (!ctx->pinned) VS_HIP(hipHostMalloc(&ctx->pinned, sizeof(c128) * VS_PINNED_C128, hipHostMallocDefault));
+  if (!ctx->pinned_up) VS_HIP(hipHostMalloc(&ctx->pinned_up, sizeof(c128) * VS_PINNED_C128, hipHostMallocDefault));
 }
 
 template <typename T>
@@ -284,6 +294,46 @@ void Engine<T>::mdot(const cplx<T>* V, int64_t ldv, int k, const cplx<T>* w, c12
 }
 
 template <typename T>
+void Engine<T>::mdot_dev(const cplx<T>* V, int64_t ldv, int k, const cplx<T>* w, c128* out_dev) {
+  VS_CHECK(k <= 512, VS_ERR_UNSUPPORTED, "Krylov basis larger than 512 vectors");
+  hipLaunchKernelGGL((k_mdot<T>), dim3(grid), dim3(RED_THREADS), 0, ctx->stream, V, ldv, k, w, n, ctx->red.as<c128>());
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(RED_THREADS), 0, ctx->stream, ctx->red.as<c128>(), grid, k,
+                     out_dev);
+  VS_HIP(hipGetLastError());
+}
+
+template <typename T>
+void Engine<T>::gemv_acc_dev(const cplx<T>* V, int64_t ldv, int k, const c128* h_dev, double sgn, cplx<T>* w) {
+  hipLaunchKernelGGL((k_gemv_acc<T>), dim3(grid), dim3(RED_THREADS), 0, ctx->stream, V, ldv, k, h_dev, sgn, w, n);
+  VS_HIP(hipGetLastError());
+}
+
+template <typename T>
+void Engine<T>::norm2_dev(const cplx<T>* x, c128* out_dev) {
+  DotArgs<T> a{};
+  a.a[0] = x;
+  a.b[0] = x;
+  hipLaunchKernelGGL((k_dots<T, 1>), dim3(grid), dim3(RED_THREADS), 0, ctx->stream, a, n, ctx->red.as<c128>());
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(RED_THREADS), 0, ctx->stream, ctx->red.as<c128>(), grid, 1,
+                     out_dev);
+  VS_HIP(hipGetLastError());
+}
+
+template <typename T>
+void Engine<T>::scale_inv_norm_dev(const cplx<T>* w, cplx<T>* y, const c128* nrm2_dev) {
+  hipLaunchKernelGGL((k_scale_inv_norm<T>), dim3(grid), dim3(RED_THREADS), 0, ctx->stream, w, y, n, nrm2_dev);
+  VS_HIP(hipGetLastError());
+}
+
+template <typename T>
+void Engine<T>::read(const c128* dev, int K, c128* host) {
+  VS_CHECK(K <= VS_PINNED_C128, VS_ERR_INTERNAL, "read-back larger than the staging buffer");
+  VS_HIP(hipMemcpyAsync(ctx->pinned, dev, sizeof(c128) * K, hipMemcpyDeviceToHost, ctx->stream));
+  VS_HIP(hipStreamSynchronize(ctx->stream));
+  std::memcpy(host, ctx->pinned, sizeof(c128) * K);
+}
+
+template <typename T>
 void Engine<T>::gemv_acc(const cplx<T>* V, int64_t ldv, int k, const c128* h_host, double sgn, cplx<T>* w) {
   VS_CHECK(k <= 512, VS_ERR_UNSUPPORTED, "Krylov basis larger than 512 vectors");
   // the staging buffer may still feed the previous upload: drain the stream before overwriting it
@@ -294,6 +344,181 @@ void Engine<T>::gemv_acc(const cplx<T>* V, int64_t ldv, int k, const c128* h_hos
                      sgn, w, n);
   VS_HIP(hipGetLastError());
 }
+
+
+// ------------------------------------------------------------------------------------------------
+// device-resident BiCGStab scalars: every scalar of an iteration is formed on the device, so a whole
+// iteration runs without host round trips (and can be replayed as one captured graph)
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ bool cz(c128 a) { return a.x == 0.0 && a.y == 0.0; }
+__device__ __forceinline__ bool cinf(c128 a) { return isinf(a.x) || isinf(a.y); }
+__device__ __forceinline__ bool cfin(c128 a) { return isfinite(a.x) && isfinite(a.y); }
+
+enum BiStage : int { BI_RHO = 0, BI_ALPHA = 1, BI_HALF = 2, BI_OMEGA = 3, BI_FULL = 4 };
+
+// reduce the partials of one stage (fixed order) and apply that stage's scalar logic
+__global__ __launch_bounds__(RED_THREADS) void k_bi_scalar(const c128* partial, int nblk, int K, int stage,
+                                                           BiScal* sc) {
+  __shared__ double sre[RED_THREADS], sim[RED_THREADS];
+  __shared__ c128 tot[4];
+  for (int k = 0; k < K; ++k) {
+    double a = 0, b = 0;
+    for (int i = threadIdx.x; i < nblk; i += RED_THREADS) {
+      a += partial[i * K + k].x;
+      b += partial[i * K + k].y;
+    }
+    sre[threadIdx.x] = a;
+    sim[threadIdx.x] = b;
+    __syncthreads();
+    for (int st = RED_THREADS / 2; st > 0; st >>= 1) {
+      if (threadIdx.x < st) {
+        sre[threadIdx.x] += sre[threadIdx.x + st];
+        sim[threadIdx.x] += sim[threadIdx.x + st];
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) tot[k] = mk<double>(sre[0], sim[0]);
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  BiScal& S = *sc;
+  switch (stage) {
+    case BI_RHO:
+      if (S.flag) return;
+      S.rho = tot[0];
+      if (cz(S.rho) || cinf(S.rho)) { S.flag = 4; S.brk = 1; return; }
+      if (!S.first) {
+        S.beta = cdiv(S.rho, S.rho1) * cdiv(S.alpha, S.omega);
+        if (cz(S.beta) || !cfin(S.beta)) { S.flag = 4; S.brk = 2; }
+      }
+      return;
+    case BI_ALPHA: {
+      if (S.flag) return;
+      const c128 rtv = tot[0];
+      S.nph = sqrt(tot[1].x);
+      S.nx = sqrt(tot[2].x);
+      if (S.has_pc && !isfinite(S.nph)) { S.flag = 2; S.brk = 3; return; }
+      if (cz(rtv) || cinf(rtv)) { S.flag = 4; S.brk = 3; return; }
+      S.alpha = cdiv(S.rho, rtv);
+      if (cinf(S.alpha)) { S.flag = 4; S.brk = 4; }
+      return;
+    }
+    case BI_HALF:
+      S.ns = sqrt(tot[0].x);
+      return;
+    case BI_OMEGA: {
+      if (S.flag) return;
+      const c128 tt = tot[0];
+      S.nsh = sqrt(tot[2].x);
+      S.nxh = sqrt(tot[3].x);
+      if (S.has_pc && !isfinite(S.nsh)) { S.flag = 2; S.brk = 5; return; }
+      if (cz(tt) || cinf(tt)) { S.flag = 4; S.brk = 5; return; }
+      S.omega = cdiv(tot[1], tt);
+      if (cinf(S.omega)) { S.flag = 4; S.brk = 6; }
+      return;
+    }
+    default:  // BI_FULL
+      S.nr = sqrt(tot[0].x);
+      if (!S.flag) {
+        S.rho1 = S.rho;
+        S.first = 0;
+      }
+      return;
+  }
+}
+
+// p = r (first iteration) or p = r + beta (p - omega v)
+template <typename T>
+__global__ __launch_bounds__(RED_THREADS) void k_bi_p(const cplx<T>* r, const cplx<T>* v, cplx<T>* p, int64_t n,
+                                                      const BiScal* sc) {
+  if (sc->flag) return;
+  const bool first = sc->first != 0;
+  const cplx<T> be = ccast<T>(sc->beta), om = ccast<T>(sc->omega);
+  for (int64_t i = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * RED_THREADS)
+    p[i] = first ? r[i] : r[i] + be * (p[i] - om * v[i]);
+}
+
+// y = x0 + c x1 ; z = x2 - c x3 ; partial ||z||^2, with c = alpha (half step) or omega (full step)
+template <typename T>
+__global__ __launch_bounds__(RED_THREADS) void k_bi_axpy2(const cplx<T>* x0, const cplx<T>* x1, const cplx<T>* x2,
+                                                          const cplx<T>* x3, cplx<T>* y, cplx<T>* z, int64_t n,
+                                                          const BiScal* sc, int use_omega, c128* partial) {
+  double re[1] = {0.0}, im[1] = {0.0};
+  if (!sc->flag) {
+    const cplx<T> c = ccast<T>(use_omega ? sc->omega : sc->alpha);
+    for (int64_t i = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * RED_THREADS) {
+      y[i] = x0[i] + c * x1[i];
+      const cplx<T> o = x2[i] - c * x3[i];
+      z[i] = o;
+      re[0] += (double)o.x * o.x + (double)o.y * o.y;
+    }
+  }
+  c128 tot[1];
+  block_sum<1>(re, im, tot);
+  if (threadIdx.x == 0) partial[blockIdx.x] = tot[0];
+}
+
+template <typename T>
+void bi_stage_rho(Engine<T>& E, const cplx<T>* rt, const cplx<T>* r, const cplx<T>* v, cplx<T>* p, BiScal* sc) {
+  DotArgs<T> a{};
+  a.a[0] = rt;
+  a.b[0] = r;
+  c128* part = E.ctx->red.template as<c128>();
+  hipLaunchKernelGGL((k_dots<T, 1>), dim3(E.grid), dim3(RED_THREADS), 0, E.ctx->stream, a, E.n, part);
+  hipLaunchKernelGGL(k_bi_scalar, dim3(1), dim3(RED_THREADS), 0, E.ctx->stream, part, E.grid, 1, (int)BI_RHO, sc);
+  hipLaunchKernelGGL((k_bi_p<T>), dim3(E.grid), dim3(RED_THREADS), 0, E.ctx->stream, r, v, p, E.n, sc);
+  VS_HIP(hipGetLastError());
+}
+
+template <typename T>
+void bi_stage_alpha_half(Engine<T>& E, const cplx<T>* rt, const cplx<T>* v, const cplx<T>* ph, const cplx<T>* x,
+                         const cplx<T>* r, cplx<T>* xh, cplx<T>* s, BiScal* sc) {
+  DotArgs<T> a{};
+  a.a[0] = rt;
+  a.b[0] = v;
+  a.a[1] = ph;
+  a.b[1] = ph;
+  a.a[2] = x;
+  a.b[2] = x;
+  c128* part = E.ctx->red.template as<c128>();
+  hipLaunchKernelGGL((k_dots<T, 3>), dim3(E.grid), dim3(RED_THREADS), 0, E.ctx->stream, a, E.n, part);
+  hipLaunchKernelGGL(k_bi_scalar, dim3(1), dim3(RED_THREADS), 0, E.ctx->stream, part, E.grid, 3, (int)BI_ALPHA, sc);
+  hipLaunchKernelGGL((k_bi_axpy2<T>), dim3(E.grid), dim3(RED_THREADS), 0, E.ctx->stream, x, ph, r, v, xh, s, E.n, sc,
+                     0, part);
+  hipLaunchKernelGGL(k_bi_scalar, dim3(1), dim3(RED_THREADS), 0, E.ctx->stream, part, E.grid, 1, (int)BI_HALF, sc);
+  VS_HIP(hipGetLastError());
+}
+
+template <typename T>
+void bi_stage_omega_full(Engine<T>& E, const cplx<T>* t, const cplx<T>* s, const cplx<T>* sh, const cplx<T>* xh,
+                         cplx<T>* x, cplx<T>* r, BiScal* sc) {
+  DotArgs<T> a{};
+  a.a[0] = t;
+  a.b[0] = t;
+  a.a[1] = t;
+  a.b[1] = s;
+  a.a[2] = sh;
+  a.b[2] = sh;
+  a.a[3] = xh;
+  a.b[3] = xh;
+  c128* part = E.ctx->red.template as<c128>();
+  hipLaunchKernelGGL((k_dots<T, 4>), dim3(E.grid), dim3(RED_THREADS), 0, E.ctx->stream, a, E.n, part);
+  hipLaunchKernelGGL(k_bi_scalar, dim3(1), dim3(RED_THREADS), 0, E.ctx->stream, part, E.grid, 4, (int)BI_OMEGA, sc);
+  hipLaunchKernelGGL((k_bi_axpy2<T>), dim3(E.grid), dim3(RED_THREADS), 0, E.ctx->stream, xh, sh, s, t, x, r, E.n, sc,
+                     1, part);
+  hipLaunchKernelGGL(k_bi_scalar, dim3(1), dim3(RED_THREADS), 0, E.ctx->stream, part, E.grid, 1, (int)BI_FULL, sc);
+  VS_HIP(hipGetLastError());
+}
+
+#define VS_BI_INST(T)                                                                                             \
+  template void bi_stage_rho<T>(Engine<T>&, const cplx<T>*, const cplx<T>*, const cplx<T>*, cplx<T>*, BiScal*);  \
+  template void bi_stage_alpha_half<T>(Engine<T>&, const cplx<T>*, const cplx<T>*, const cplx<T>*, const cplx<T>*, \
+                                       const cplx<T>*, cplx<T>*, cplx<T>*, BiScal*);                             \
+  template void bi_stage_omega_full<T>(Engine<T>&, const cplx<T>*, const cplx<T>*, const cplx<T>*, const cplx<T>*, \
+                                       cplx<T>*, cplx<T>*, BiScal*);
+VS_BI_INST(float)
+VS_BI_INST(double)
+#undef VS_BI_INST
 
 template struct Engine<float>;
 template struct Engine<double>;

@@ -23,7 +23,33 @@ struct Engine {
   void copy(const cplx<T>* x, cplx<T>* y);
   void mdot(const cplx<T>* V, int64_t ldv, int k, const cplx<T>* w, c128* out);
   void gemv_acc(const cplx<T>* V, int64_t ldv, int k, const c128* h_host, double sgn, cplx<T>* w);
+  // variants that leave their results on the device (no host round trip)
+  void mdot_dev(const cplx<T>* V, int64_t ldv, int k, const cplx<T>* w, c128* out_dev);
+  void gemv_acc_dev(const cplx<T>* V, int64_t ldv, int k, const c128* h_dev, double sgn, cplx<T>* w);
+  void norm2_dev(const cplx<T>* x, c128* out_dev);
+  void scale_inv_norm_dev(const cplx<T>* w, cplx<T>* y, const c128* nrm2_dev);
+  void read(const c128* dev, int K, c128* host);
 };
+
+// device-resident scalars of one BiCGStab iteration (read back once per iteration)
+struct BiScal {
+  c128 rho, rho1, alpha, omega, beta;
+  double ns, nr;              // ||s|| after the half step, ||r|| after the full step
+  double nph, nx, nsh, nxh;   // norms of the stagnation tests
+  int first;                  // 1 until the first full step (p = r)
+  int flag;                   // 0 running, 2 non-finite preconditioned vector, 4 breakdown
+  int brk;                    // where the breakdown was detected
+  int has_pc;
+};
+
+template <typename T>
+void bi_stage_rho(Engine<T>& E, const cplx<T>* rt, const cplx<T>* r, const cplx<T>* v, cplx<T>* p, BiScal* sc);
+template <typename T>
+void bi_stage_alpha_half(Engine<T>& E, const cplx<T>* rt, const cplx<T>* v, const cplx<T>* ph, const cplx<T>* x,
+                         const cplx<T>* r, cplx<T>* xh, cplx<T>* s, BiScal* sc);
+template <typename T>
+void bi_stage_omega_full(Engine<T>& E, const cplx<T>* t, const cplx<T>* s, const cplx<T>* sh, const cplx<T>* xh,
+                         cplx<T>* x, cplx<T>* r, BiScal* sc);
 
 enum { UPD_AXPBY_ = 0, UPD_BICG_P_ = 1, UPD_HALF_ = 2, UPD_SUB_ = 3, UPD_COPY_ = 4, UPD_SCALE_ = 5 };
 

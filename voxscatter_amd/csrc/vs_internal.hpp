@@ -151,6 +151,8 @@ struct FftPlanHost {
 // ------------------------------------------------------------------------------------------------
 }  // namespace vs
 
+constexpr int VS_PINNED_C128 = 2048;  // complex doubles in each pinned staging buffer
+
 struct vs_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
