@@ -157,6 +157,25 @@ int vs_refine_dev(vs_op* op, vs_pc* pc, const void* b, void* x, double tol, int 
                   double inner_tol, int inner_maxit, int restart, int* flag, double* relres, int* outer,
                   int* inner_total);
 
+/* ---------------------------------------------------------------------------------------------- */
+/* fields and cross sections (no reference counterpart: README.md:35 lists them as future work)     */
+/* ---------------------------------------------------------------------------------------------- */
+/* Far-field amplitude F(n) = k^2 sum_j (I - n n) P_j dV exp(-i k n.r_j) for ndir unit vectors.
+ * r: nvox*3 voxel centres (row per voxel); P: [Px; Py; Pz] (3*nvox complex128); F: ndir*3 complex128. */
+int vs_dda_farfield(vs_ctx* ctx, int64_t nvox, const double* r, const double* P, double dV, double k, int64_t ndir,
+                    const double* dirs, double* F);
+/* Extinction, absorption and scattering cross sections (Draine 1988) of a DDA polarisation P with the
+ * incident field Einc at the same voxels; alpha_inv: 1/alpha (1 or nvox complex values, the DDA
+ * diagonal).  out = {C_ext, C_abs, C_sca = C_ext - C_abs}. */
+int vs_dda_cross_sections(vs_ctx* ctx, int64_t nvox, const double* P, const double* Einc, const double* alpha_inv,
+                          int64_t ainv_len, double dV, double k, double E0sq, double out[3]);
+
+/* Adjoint gradient assembly (no reference counterpart): g_j = -dainv_j sum_c lam_{c,j} P_{c,j}, the
+ * derivative d(w^T P)/d eps_j of a holomorphic objective, given the forward solution P and the adjoint
+ * solution lam of A^T lam = w (A^T = A for the DDA system: solve it with vs_bicgstab/vs_gmres on the same
+ * operator); dainv_j = d(1/alpha)/d eps at voxel j.  P, lam: [x; y; z] (3*nvox); dainv, g: nvox. */
+int vs_dda_grad_eps(vs_ctx* ctx, int64_t nvox, const double* P, const double* lam, const double* dainv, double* g);
+
 /* Timing helpers (no reference counterpart): nrep back-to-back matvecs / preconditioner applies on
  * the context stream, bracketed by HIP events; *ms receives the mean time per call. */
 int vs_bench_mvp(vs_op* op, const void* x, void* y, int transp, int precision, int nrhs, int nrep, double* ms);

@@ -229,6 +229,7 @@ class Operator:
             return self
         check(self.ctx.lib.vs_op_set_domain(self.h, i64ptr(idx), idx.size, dptr(g), gl, dptr(mr), ml, dptr(mc), cl))
         self._domain_key = key
+        self._domain_args = (idx, Gram, Mr, Mc)
         self.n = int(idx.size)
         self.idx = idx
         return self

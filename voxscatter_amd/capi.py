@@ -78,6 +78,9 @@ SIGNATURES = {
                         C.POINTER(C.c_int), C.POINTER(C.c_int), D, C.POINTER(C.c_int)]),
     "vs_refine_dev": (INT, [P, P, P, P, C.c_double, INT, INT, C.c_double, INT, INT, C.POINTER(C.c_int), D,
                             C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "vs_dda_farfield": (INT, [P, C.c_int64, D, D, C.c_double, C.c_double, C.c_int64, D, D]),
+    "vs_dda_cross_sections": (INT, [P, C.c_int64, D, D, D, C.c_int64, C.c_double, C.c_double, C.c_double, D]),
+    "vs_dda_grad_eps": (INT, [P, C.c_int64, D, D, D, D]),
     "vs_bench_mvp": (INT, [P, P, P, INT, INT, INT, INT, D]),
     "vs_bench_pc": (INT, [P, P, P, INT, INT, INT, D]),
     "vs_pgmres": (INT, [P, P, INT, D, D, INT, C.c_double, INT, D, C.POINTER(C.c_int), D, C.POINTER(C.c_int), D,
