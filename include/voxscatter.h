@@ -176,6 +176,16 @@ int vs_dda_cross_sections(vs_ctx* ctx, int64_t nvox, const double* P, const doub
  * operator); dainv_j = d(1/alpha)/d eps at voxel j.  P, lam: [x; y; z] (3*nvox); dainv, g: nvox. */
 int vs_dda_grad_eps(vs_ctx* ctx, int64_t nvox, const double* P, const double* lam, const double* dainv, double* g);
 
+/* Batched solves (no reference counterpart; the batching form of the solve loop): nrhs right-hand
+ * sides B (nrhs consecutive device vectors) solved together, every matvec and preconditioner launch
+ * covering the whole batch, one captured graph per iteration.  flags/relres/iters receive nrhs values;
+ * flag 0 = true relative residual <= tol, 1 = not reached, 4 = breakdown. */
+int vs_bicgstab_batch_dev(vs_op* op, vs_pc* pc, int nrhs, const void* B, void* X, int precision, double tol,
+                          int maxit, int* flags, double* relres, double* iters);
+/* Batched FP32-inner / FP64-refinement solve; B, X are FP64 device vectors. */
+int vs_refine_batch_dev(vs_op* op, vs_pc* pc, int nrhs, const void* B, void* X, double tol, int max_outer,
+                        double inner_tol, int inner_maxit, int* flags, double* relres, int* outer, int* inner_total);
+
 /* Timing helpers (no reference counterpart): nrep back-to-back matvecs / preconditioner applies on
  * the context stream, bracketed by HIP events; *ms receives the mean time per call. */
 int vs_bench_mvp(vs_op* op, const void* x, void* y, int transp, int precision, int nrhs, int nrep, double* ms);

@@ -596,3 +596,28 @@ def bench_pc(M: Preconditioner, x_ptr: int, y_ptr: int, precision=VS_FP64, nrhs=
     check(M.ctx.lib.vs_bench_pc(M.h, C.c_void_p(x_ptr), C.c_void_p(y_ptr), int(precision), int(nrhs), int(nrep),
                                 C.byref(ms)))
     return ms.value
+
+
+def bicgstab_batch_dev(A: Operator, B: "DeviceArray", X: "DeviceArray", tol=1e-6, maxit=2000, M1=None):
+    """Batched BiCGStab over the rows of B (nrhs x n device array, FP64 or FP32); returns (flags, relres, iters)."""
+    k = B.shape[0]
+    flags = (C.c_int * k)()
+    relres = np.zeros(k)
+    iters = np.zeros(k)
+    check(A.ctx.lib.vs_bicgstab_batch_dev(A.h, None if M1 is None else M1.h, k, C.c_void_p(B.ptr), C.c_void_p(X.ptr),
+                                          B.precision, float(tol), int(maxit), flags, dptr(relres), dptr(iters)))
+    return np.array(list(flags)), relres, iters
+
+
+def refine_batch_dev(A: Operator, B: "DeviceArray", X: "DeviceArray", M1=None, tol=1e-6, max_outer=10, inner_tol=1e-3,
+                     inner_maxit=2000):
+    """Batched FP32 + FP64-refinement solve; returns (flags, relres, outer, inner_iterations)."""
+    k = B.shape[0]
+    flags = (C.c_int * k)()
+    relres = np.zeros(k)
+    outer = C.c_int()
+    tot = C.c_int()
+    check(A.ctx.lib.vs_refine_batch_dev(A.h, None if M1 is None else M1.h, k, C.c_void_p(B.ptr), C.c_void_p(X.ptr),
+                                        float(tol), int(max_outer), float(inner_tol), int(inner_maxit), flags,
+                                        dptr(relres), C.byref(outer), C.byref(tot)))
+    return np.array(list(flags)), relres, outer.value, tot.value

@@ -81,6 +81,9 @@ SIGNATURES = {
     "vs_dda_farfield": (INT, [P, C.c_int64, D, D, C.c_double, C.c_double, C.c_int64, D, D]),
     "vs_dda_cross_sections": (INT, [P, C.c_int64, D, D, D, C.c_int64, C.c_double, C.c_double, C.c_double, D]),
     "vs_dda_grad_eps": (INT, [P, C.c_int64, D, D, D, D]),
+    "vs_bicgstab_batch_dev": (INT, [P, P, INT, P, P, INT, C.c_double, INT, C.POINTER(C.c_int), D, D]),
+    "vs_refine_batch_dev": (INT, [P, P, INT, P, P, C.c_double, INT, C.c_double, INT, C.POINTER(C.c_int), D,
+                                  C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "vs_bench_mvp": (INT, [P, P, P, INT, INT, INT, INT, D]),
     "vs_bench_pc": (INT, [P, P, P, INT, INT, INT, D]),
     "vs_pgmres": (INT, [P, P, INT, D, D, INT, C.c_double, INT, D, C.POINTER(C.c_int), D, C.POINTER(C.c_int), D,

@@ -70,6 +70,8 @@ LinOp<T> make_linop(vs_op* op, vs_pc* pc) {
   LinOp<T> L;
   L.A = [op, prec](const cplx<T>* x, cplx<T>* y) { mvp_dev(op, x, y, 0, prec, 1); };
   if (pc) L.M = [pc, prec](const cplx<T>* x, cplx<T>* y) { pc_apply_dev(pc, x, y, prec, 1); };
+  L.Ab = [op, prec](const cplx<T>* x, cplx<T>* y, int k) { mvp_dev(op, x, y, 0, prec, k); };
+  if (pc) L.Mb = [pc, prec](const cplx<T>* x, cplx<T>* y, int k) { pc_apply_dev(pc, x, y, prec, k); };
   return L;
 }
 
@@ -699,6 +701,49 @@ int vs_refine_dev(vs_op* op, vs_pc* pc, const void* b, void* x, double tol, int 
     *flag = R.flag;
     *relres = R.relres;
     *outer = R.outer;
+  });
+}
+
+int vs_bicgstab_batch_dev(vs_op* op, vs_pc* pc, int nrhs, const void* B, void* X, int precision, double tol,
+                          int maxit, int* flags, double* relres, double* iters) {
+  return guarded([&] {
+    VS_CHECK(op && B && X && flags && relres && iters && nrhs >= 1 && maxit >= 1, VS_ERR_ARG, "bad arguments");
+    VS_CHECK(precision == VS_FP64 || precision == VS_FP32, VS_ERR_ARG, "precision must be VS_FP64 or VS_FP32");
+    check_solver_args(op, pc, precision);
+    DeviceGuard g(op->ctx->device);
+    BatchResult R;
+    if (precision == VS_FP64) {
+      Engine<double> E(op->ctx, op->nunk);
+      R = solve_bicgstab_batch<double>(E, make_linop<double>(op, pc), nrhs, (const c128*)B, (c128*)X, tol, maxit);
+    } else {
+      Engine<float> E(op->ctx, op->nunk);
+      R = solve_bicgstab_batch<float>(E, make_linop<float>(op, pc), nrhs, (const c64*)B, (c64*)X, tol, maxit);
+    }
+    for (int i = 0; i < nrhs; ++i) {
+      flags[i] = R.flag[i];
+      relres[i] = R.relres[i];
+      iters[i] = R.iter[i];
+    }
+  });
+}
+
+int vs_refine_batch_dev(vs_op* op, vs_pc* pc, int nrhs, const void* B, void* X, double tol, int max_outer,
+                        double inner_tol, int inner_maxit, int* flags, double* relres, int* outer, int* inner_total) {
+  return guarded([&] {
+    VS_CHECK(op && B && X && flags && relres && outer && inner_total && nrhs >= 1, VS_ERR_ARG, "bad arguments");
+    VS_CHECK((op->precisions & (VS_FP64 | VS_FP32)) == (VS_FP64 | VS_FP32), VS_ERR_ARG,
+             "refinement needs an operator holding both FP64 and FP32 tensors");
+    check_solver_args(op, pc, VS_FP32);
+    DeviceGuard g(op->ctx->device);
+    Engine<double> E64(op->ctx, op->nunk);
+    Engine<float> E32(op->ctx, op->nunk);
+    BatchResult R = solve_refine_batch(E64, E32, make_linop<double>(op, nullptr), make_linop<float>(op, pc), nrhs,
+                                       (const c128*)B, (c128*)X, tol, max_outer, inner_tol, inner_maxit, outer,
+                                       inner_total);
+    for (int i = 0; i < nrhs; ++i) {
+      flags[i] = R.flag[i];
+      relres[i] = R.relres[i];
+    }
   });
 }
 

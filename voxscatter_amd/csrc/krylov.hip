@@ -2,6 +2,7 @@
 #include "solver.hpp"
 
 #include <cfloat>
+#include <cstring>
 #include <cmath>
 #include <cstdlib>
 #include <limits>
@@ -591,6 +592,226 @@ SolveResult solve_refine(Engine<double>& E64, Engine<float>& E32, const LinOp<do
   R.relres = relres;
   R.outer = outer;
   R.iter = outer;
+  if (inner_total) *inner_total = tot;
+  VS_HIP(hipStreamSynchronize(ctx->stream));
+  return R;
+}
+
+}  // namespace vs
+
+namespace vs {
+
+// Batched BiCGStab: nrhs independent systems advance together, every matvec / preconditioner launch
+// covers the whole batch, the per-system scalars live on the device and one captured graph replays a
+// full iteration.  A system stops when its recursive residual meets tol at a half or full step (x =
+// xhalf at a half step); breakdowns stop that system alone.  At the end the true residuals are formed;
+// flag 0 requires the true relative residual <= tol, flag 1 otherwise (or when maxit is reached), 4 on
+// breakdown.
+template <typename T>
+BatchResult solve_bicgstab_batch(Engine<T>& E, const LinOp<T>& op, int k, const cplx<T>* B, cplx<T>* X, double tol,
+                                 int maxit) {
+  using cv = cplx<T>;
+  const int64_t n = E.n;
+  vs_ctx* ctx = E.ctx;
+  VS_CHECK(k >= 1 && (size_t)k * sizeof(BiScal) <= sizeof(c128) * VS_PINNED_C128, VS_ERR_UNSUPPORTED,
+           "too many right-hand sides for one batch");
+  BatchResult R;
+  R.flag.assign(k, 1);
+  R.relres.assign(k, 0.0);
+  R.iter.assign(k, 0.0);
+  Vecs<T> W(n * k, 9);
+  cv *r = W[0], *rt = W[1], *p = W[2], *v = W[3], *ph = W[4], *s = W[5], *sh = W[6], *t = W[7], *xh = W[8];
+  const cv* phv = op.Mb ? ph : p;
+  const cv* shv = op.Mb ? sh : s;
+  DevBuf part(sizeof(c128) * (size_t)E.grid * k * 4), nrm(sizeof(c128) * k), dsc(sizeof(BiScal) * k);
+  BiScal* sc = dsc.as<BiScal>();
+  VS_HIP(hipMemsetAsync(X, 0, sizeof(cv) * n * k, ctx->stream));
+  batch_norm2<T>(E, k, B, part.as<c128>(), nrm.as<c128>());
+  std::vector<c128> nb(k);
+  E.read(nrm.as<c128>(), k, nb.data());
+  std::vector<double> tolb(k);
+  std::vector<int> active(k);
+  BiScal* up = reinterpret_cast<BiScal*>(ctx->pinned_up);
+  for (int i = 0; i < k; ++i) {
+    const double nbi = std::sqrt(nb[i].x);
+    tolb[i] = tol * nbi;
+    active[i] = nbi > 0 ? 1 : 0;
+    if (!active[i]) R.flag[i] = 0;
+    BiScal h{};
+    h.rho = C(1);
+    h.rho1 = C(1);
+    h.omega = C(1);
+    h.first = 1;
+    h.has_pc = op.Mb ? 1 : 0;
+    h.active = active[i];
+    up[i] = h;
+  }
+  VS_HIP(hipMemcpyAsync(sc, up, sizeof(BiScal) * k, hipMemcpyHostToDevice, ctx->stream));
+  VS_HIP(hipMemcpyAsync(r, B, sizeof(cv) * n * k, hipMemcpyDeviceToDevice, ctx->stream));
+  VS_HIP(hipMemcpyAsync(rt, B, sizeof(cv) * n * k, hipMemcpyDeviceToDevice, ctx->stream));
+  BiScal* hs = reinterpret_cast<BiScal*>(ctx->pinned);
+  c128* pt = part.as<c128>();
+  auto iteration = [&] {
+    bib_stage_rho<T>(E, k, rt, r, v, p, sc, pt);
+    if (op.Mb) op.Mb(p, ph, k);
+    op.Ab(phv, v, k);
+    bib_stage_alpha_half<T>(E, k, rt, v, phv, X, r, xh, s, sc, pt);
+    if (op.Mb) op.Mb(s, sh, k);
+    op.Ab(shv, t, k);
+    bib_stage_omega_full<T>(E, k, t, s, shv, xh, X, r, sc, pt);
+    VS_HIP(hipMemcpyAsync(hs, sc, sizeof(BiScal) * k, hipMemcpyDeviceToHost, ctx->stream));
+  };
+  const char* ge = getenv("VS_GRAPH");
+  bool use_graph = !(ge && ge[0] == '0');
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  int nact = 0;
+  for (int i = 0; i < k; ++i) nact += active[i];
+  int ii = 0;
+  for (ii = 1; ii <= maxit && nact > 0; ++ii) {
+    if (use_graph && ii >= 2 && !gexec) {
+      if (hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+        bool ok = true;
+        try {
+          iteration();
+        } catch (const Error&) {
+          ok = false;
+        }
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+        if (ok && ec == hipSuccess && g && hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0) == hipSuccess) {
+          graph = g;
+        } else {
+          if (g) hipGraphDestroy(g);
+          gexec = nullptr;
+          use_graph = false;
+          (void)hipGetLastError();
+        }
+      } else {
+        use_graph = false;
+        (void)hipGetLastError();
+      }
+    }
+    if (gexec)
+      VS_HIP(hipGraphLaunch(gexec, ctx->stream));
+    else
+      iteration();
+    VS_HIP(hipStreamSynchronize(ctx->stream));
+    bool changed = false;
+    for (int i = 0; i < k; ++i) {
+      if (!active[i]) continue;
+      const BiScal& h = hs[i];
+      int stop = 0;
+      if (h.flag && h.brk <= 4) {
+        R.flag[i] = h.flag;
+        R.iter[i] = ii - 1;
+        stop = 1;
+      } else if (h.ns <= tolb[i]) {
+        VS_HIP(hipMemcpyAsync(X + (int64_t)i * n, xh + (int64_t)i * n, sizeof(cv) * n, hipMemcpyDeviceToDevice,
+                              ctx->stream));
+        R.flag[i] = 0;
+        R.iter[i] = ii - 0.5;
+        stop = 1;
+      } else if (h.flag) {
+        R.flag[i] = h.flag;
+        R.iter[i] = ii - 1;
+        stop = 1;
+      } else if (h.nr <= tolb[i]) {
+        R.flag[i] = 0;
+        R.iter[i] = ii;
+        stop = 1;
+      }
+      if (stop) {
+        active[i] = 0;
+        --nact;
+        changed = true;
+        *reinterpret_cast<int*>(reinterpret_cast<char*>(ctx->pinned_up) + sizeof(int) * i) = 0;
+      } else {
+        R.iter[i] = ii;
+      }
+    }
+    if (changed) {
+      // deactivate the finished systems on the device (the staging ints are consumed before returning)
+      for (int i = 0; i < k; ++i)
+        if (!active[i])
+          VS_HIP(hipMemcpyAsync(&sc[i].active, reinterpret_cast<char*>(ctx->pinned_up) + sizeof(int) * i, sizeof(int),
+                                hipMemcpyHostToDevice, ctx->stream));
+      VS_HIP(hipStreamSynchronize(ctx->stream));
+    }
+  }
+  R.iterations = ii - 1;
+  if (gexec) hipGraphExecDestroy(gexec);
+  if (graph) hipGraphDestroy(graph);
+  // true residuals
+  op.Ab(X, r, k);
+  batch_sub<T>(E, k, B, r);
+  batch_norm2<T>(E, k, r, part.as<c128>(), nrm.as<c128>());
+  std::vector<c128> nr(k);
+  E.read(nrm.as<c128>(), k, nr.data());
+  for (int i = 0; i < k; ++i) {
+    const double nbi = tolb[i] / (tol > 0 ? tol : 1.0);
+    R.relres[i] = nbi > 0 ? std::sqrt(nr[i].x) / nbi : 0.0;
+    if (R.flag[i] == 0 && nbi > 0 && R.relres[i] > tol) R.flag[i] = 1;
+  }
+  VS_HIP(hipStreamSynchronize(ctx->stream));
+  return R;
+}
+
+template BatchResult solve_bicgstab_batch<double>(Engine<double>&, const LinOp<double>&, int, const c128*, c128*,
+                                                  double, int);
+template BatchResult solve_bicgstab_batch<float>(Engine<float>&, const LinOp<float>&, int, const c64*, c64*, double,
+                                                 int);
+
+// batched refinement: FP64 residuals of the whole batch, FP32 batched BiCGStab for the corrections
+BatchResult solve_refine_batch(Engine<double>& E64, Engine<float>& E32, const LinOp<double>& A64,
+                               const LinOp<float>& A32, int k, const c128* B, c128* X, double tol, int max_outer,
+                               double inner_tol, int inner_maxit, int* outer_out, int* inner_total) {
+  const int64_t n = E64.n;
+  vs_ctx* ctx = E64.ctx;
+  BatchResult R;
+  R.flag.assign(k, 1);
+  R.relres.assign(k, 0.0);
+  R.iter.assign(k, 0.0);
+  DevBuf R64(sizeof(c128) * n * k), R32(sizeof(c64) * n * k), D32(sizeof(c64) * n * k), D64(sizeof(c128) * n * k);
+  DevBuf part(sizeof(c128) * (size_t)E64.grid * k), nrm(sizeof(c128) * k), coef(sizeof(double) * k);
+  VS_HIP(hipMemsetAsync(X, 0, sizeof(c128) * n * k, ctx->stream));
+  batch_norm2<double>(E64, k, B, part.as<c128>(), nrm.as<c128>());
+  std::vector<c128> h(k);
+  E64.read(nrm.as<c128>(), k, h.data());
+  std::vector<double> nb(k);
+  for (int i = 0; i < k; ++i) nb[i] = std::sqrt(h[i].x);
+  int tot = 0, outer = 0;
+  for (outer = 0; outer <= max_outer; ++outer) {
+    A64.Ab(X, R64.as<c128>(), k);
+    batch_sub<double>(E64, k, B, R64.as<c128>());
+    batch_norm2<double>(E64, k, R64.as<c128>(), part.as<c128>(), nrm.as<c128>());
+    E64.read(nrm.as<c128>(), k, h.data());
+    bool all = true;
+    std::vector<double> sc(k), back(k);
+    for (int i = 0; i < k; ++i) {
+      const double nr = std::sqrt(h[i].x);
+      R.relres[i] = nb[i] > 0 ? nr / nb[i] : 0.0;
+      const bool done = R.relres[i] <= tol;
+      R.flag[i] = done ? 0 : 1;
+      all = all && done;
+      sc[i] = done || nr == 0 ? 0.0 : 1.0 / nr;  // converged systems get a zero right-hand side
+      back[i] = done ? 0.0 : nr;
+    }
+    if (all || outer == max_outer) break;
+    VS_HIP(hipStreamSynchronize(ctx->stream));
+    std::memcpy(ctx->pinned_up, sc.data(), sizeof(double) * k);
+    VS_HIP(hipMemcpyAsync(coef.p, ctx->pinned_up, sizeof(double) * k, hipMemcpyHostToDevice, ctx->stream));
+    batch_scale<double>(E64, k, R64.as<c128>(), R64.as<c128>(), coef.as<double>(), false);
+    convert_dev(ctx, R64.p, VS_FP64, R32.p, VS_FP32, n * k);
+    BatchResult in = solve_bicgstab_batch<float>(E32, A32, k, R32.as<c64>(), D32.as<c64>(), inner_tol, inner_maxit);
+    tot += in.iterations;
+    convert_dev(ctx, D32.p, VS_FP32, D64.p, VS_FP64, n * k);
+    VS_HIP(hipStreamSynchronize(ctx->stream));
+    std::memcpy(ctx->pinned_up, back.data(), sizeof(double) * k);
+    VS_HIP(hipMemcpyAsync(coef.p, ctx->pinned_up, sizeof(double) * k, hipMemcpyHostToDevice, ctx->stream));
+    batch_scale<double>(E64, k, D64.as<c128>(), X, coef.as<double>(), true);
+  }
+  if (outer_out) *outer_out = outer;
   if (inner_total) *inner_total = tot;
   VS_HIP(hipStreamSynchronize(ctx->stream));
   return R;

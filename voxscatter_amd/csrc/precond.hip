@@ -344,20 +344,24 @@ __global__ void k_relayout1(const c128* __restrict__ in, cplx<To>* __restrict__ 
 // ------------------------------------------------------------------------------------------------
 // apply kernels
 // ------------------------------------------------------------------------------------------------
-// per (kx tile, ky): u <- Binv(kx,ky) u on the y-FFT'd volume, in place
-template <typename T>
+// per (kx tile, ky): u <- Binv(kx,ky) u on the y-FFT'd volume, in place, for RC right-hand sides at once
+// (each block entry is read once per chunk of RC right-hand sides; blockIdx.y = chunk)
+template <typename T, int RC>
 __global__ __launch_bounds__(256) void k_blockmv2(const cplx<T>* __restrict__ B, cplx<T>* __restrict__ U, int L, int M,
-                                                  int n, int W, int64_t rhs_stride) {
-  cplx<T>* u = reinterpret_cast<cplx<T>*>(vs_smem);
+                                                  int n, int W, int64_t rhs_stride, int nrhs) {
+  cplx<T>* u = reinterpret_cast<cplx<T>*>(vs_smem);  // [rc][s][w]
   const int tiles = (L + W - 1) / W;
   const int t = blockIdx.x % tiles, ky = blockIdx.x / tiles;
   const int kx0 = t * W;
   const int wv = min(W, L - kx0);
-  cplx<T>* Ub = U + (int64_t)blockIdx.y * rhs_stride + kx0 + (int64_t)L * ky;
+  const int r0 = blockIdx.y * RC;
+  const int nr = min(RC, nrhs - r0);
   const int64_t sstride = (int64_t)L * M;
-  for (int e = threadIdx.x; e < n * W; e += blockDim.x) {
-    const int w = e % W, s = e / W;
-    u[e] = w < wv ? Ub[s * sstride + w] : mk<T>(0, 0);
+  cplx<T>* U0 = U + (int64_t)r0 * rhs_stride + kx0 + (int64_t)L * ky;
+  for (int e = threadIdx.x; e < RC * n * W; e += blockDim.x) {
+    const int w = e % W, q = e / W;
+    const int s = q % n, rc = q / n;
+    u[e] = (w < wv && rc < nr) ? U0[rc * rhs_stride + s * sstride + w] : mk<T>(0, 0);
   }
   __syncthreads();
   const cplx<T>* Bk = B + (int64_t)ky * n * n * L + kx0;
@@ -365,27 +369,38 @@ __global__ __launch_bounds__(256) void k_blockmv2(const cplx<T>* __restrict__ B,
     const int w = e % W, r = e / W;
     if (w >= wv) continue;
     const cplx<T>* br = Bk + (int64_t)r * n * L + w;
-    cplx<T> acc = mk<T>(0, 0);
-    for (int s = 0; s < n; ++s) acc = acc + br[(int64_t)s * L] * u[s * W + w];
-    Ub[r * sstride + w] = acc;
+    cplx<T> acc[RC];
+#pragma unroll
+    for (int c = 0; c < RC; ++c) acc[c] = mk<T>(0, 0);
+    for (int s = 0; s < n; ++s) {
+      const cplx<T> b = br[(int64_t)s * L];
+#pragma unroll
+      for (int c = 0; c < RC; ++c) acc[c] = acc[c] + b * u[(c * n + s) * W + w];
+    }
+#pragma unroll
+    for (int c = 0; c < RC; ++c)
+      if (c < nr) U0[c * rhs_stride + r * sstride + w] = acc[c];
   }
 }
 
 // same with packed symmetric blocks: B(r,s) = B_up(r,s) for s >= r, d_r d_s B_up(s,r) otherwise, where
 // d = -1 on the z rows (index >= nz0 = 2N)
-template <typename T>
+template <typename T, int RC>
 __global__ __launch_bounds__(256) void k_blockmv2_sym(const cplx<T>* __restrict__ B, cplx<T>* __restrict__ U, int L,
-                                                      int M, int n, int W, int64_t rhs_stride) {
-  cplx<T>* u = reinterpret_cast<cplx<T>*>(vs_smem);
+                                                      int M, int n, int W, int64_t rhs_stride, int nrhs) {
+  cplx<T>* u = reinterpret_cast<cplx<T>*>(vs_smem);  // [rc][s][w]
   const int tiles = (L + W - 1) / W;
   const int t = blockIdx.x % tiles, ky = blockIdx.x / tiles;
   const int kx0 = t * W;
   const int wv = min(W, L - kx0);
-  cplx<T>* Ub = U + (int64_t)blockIdx.y * rhs_stride + kx0 + (int64_t)L * ky;
+  const int r0 = blockIdx.y * RC;
+  const int nr = min(RC, nrhs - r0);
   const int64_t sstride = (int64_t)L * M;
-  for (int e = threadIdx.x; e < n * W; e += blockDim.x) {
-    const int w = e % W, s = e / W;
-    u[e] = w < wv ? Ub[s * sstride + w] : mk<T>(0, 0);
+  cplx<T>* U0 = U + (int64_t)r0 * rhs_stride + kx0 + (int64_t)L * ky;
+  for (int e = threadIdx.x; e < RC * n * W; e += blockDim.x) {
+    const int w = e % W, q = e / W;
+    const int s = q % n, rc = q / n;
+    u[e] = (w < wv && rc < nr) ? U0[rc * rhs_stride + s * sstride + w] : mk<T>(0, 0);
   }
   __syncthreads();
   const int64_t NT = (int64_t)n * (n + 1) / 2;
@@ -395,19 +410,28 @@ __global__ __launch_bounds__(256) void k_blockmv2_sym(const cplx<T>* __restrict_
     const int w = e % W, r = e / W;
     if (w >= wv) continue;
     const bool rz = r >= nz0;
-    cplx<T> acc0 = mk<T>(0, 0), acc1 = mk<T>(0, 0);
+    cplx<T> acc[RC];
+#pragma unroll
+    for (int c = 0; c < RC; ++c) acc[c] = mk<T>(0, 0);
     // s < r: transposed entries (s, r), t(s, r) = s(2n - s + 1)/2 + (r - s)
-    int64_t ts = r;  // t(0, r)
+    int64_t ts = r;
     for (int s = 0; s < r; ++s) {
       cplx<T> b = Bk[ts * L + w];
       if (rz != (s >= nz0)) b = -b;
-      acc0 = acc0 + b * u[s * W + w];
+#pragma unroll
+      for (int c = 0; c < RC; ++c) acc[c] = acc[c] + b * u[(c * n + s) * W + w];
       ts += n - s - 1;
     }
     // s >= r: row r of the upper triangle, contiguous in t
     const int64_t t0 = (int64_t)r * (2 * n - r + 1) / 2;
-    for (int s = r; s < n; ++s) acc1 = acc1 + Bk[(t0 + s - r) * L + w] * u[s * W + w];
-    Ub[r * sstride + w] = acc0 + acc1;
+    for (int s = r; s < n; ++s) {
+      const cplx<T> b = Bk[(t0 + s - r) * L + w];
+#pragma unroll
+      for (int c = 0; c < RC; ++c) acc[c] = acc[c] + b * u[(c * n + s) * W + w];
+    }
+#pragma unroll
+    for (int c = 0; c < RC; ++c)
+      if (c < nr) U0[c * rhs_stride + r * sstride + w] = acc[c];
   }
 }
 
@@ -535,17 +559,29 @@ static void pc_apply_impl(vs_pc* pc, const cplx<T>* x, cplx<T>* y, int nrhs) {
   if (pc->levels == 2) {
     launch_col_fft(ctx, prec, U, L, L, (int64_t)N * 3, (int64_t)L * M, M, M, M, false, 1.0, nrhs, V3);
     const int n = pc->nb;
+    // right-hand sides per block-matrix pass: 1, 4 or 8
+    int RC = nrhs >= 8 ? 8 : (nrhs >= 4 ? 4 : 1);
+    while (RC > 1 && (size_t)RC * n * sizeof(cplx<T>) > 65536 / 4) RC = RC == 8 ? 4 : 1;
     int W = std::min(64, L);
-    while (W > 1 && (size_t)n * W * sizeof(cplx<T>) > 65536) W /= 2;
-    while (W > 16 && (int64_t)((L + W - 1) / W) * M < 1024) W /= 2;
-    const size_t lds = sizeof(cplx<T>) * (size_t)n * W;
+    while (W > 1 && (size_t)RC * n * W * sizeof(cplx<T>) > 65536) W /= 2;
+    while (W > 16 && (int64_t)((L + W - 1) / W) * M * ((nrhs + RC - 1) / RC) < 1024) W /= 2;
+    VS_CHECK((size_t)RC * n * W * sizeof(cplx<T>) <= 65536, VS_ERR_UNSUPPORTED, "2-level block too large for LDS");
+    const size_t lds = sizeof(cplx<T>) * (size_t)RC * n * W;
     const int tiles = (L + W - 1) / W;
-    if (pc->sym)
-      hipLaunchKernelGGL((k_blockmv2_sym<T>), dim3((unsigned)(tiles * M), (unsigned)nrhs), dim3(256), lds,
-                         ctx->stream, B, U, L, M, n, W, V3);
-    else
-      hipLaunchKernelGGL((k_blockmv2<T>), dim3((unsigned)(tiles * M), (unsigned)nrhs), dim3(256), lds, ctx->stream,
-                         B, U, L, M, n, W, V3);
+    dim3 grid((unsigned)(tiles * M), (unsigned)((nrhs + RC - 1) / RC));
+#define VS_BMV(RCV)                                                                                        \
+  if (pc->sym)                                                                                             \
+    hipLaunchKernelGGL((k_blockmv2_sym<T, RCV>), grid, dim3(256), lds, ctx->stream, B, U, L, M, n, W, V3, nrhs); \
+  else                                                                                                     \
+    hipLaunchKernelGGL((k_blockmv2<T, RCV>), grid, dim3(256), lds, ctx->stream, B, U, L, M, n, W, V3, nrhs);
+    if (RC == 8) {
+      VS_BMV(8)
+    } else if (RC == 4) {
+      VS_BMV(4)
+    } else {
+      VS_BMV(1)
+    }
+#undef VS_BMV
     VS_HIP(hipGetLastError());
     launch_col_fft(ctx, prec, U, L, L, (int64_t)N * 3, (int64_t)L * M, M, M, M, true, 1.0, nrhs, V3);
     launch_row_ifft_scatter(ctx, prec, U, map3, pc->nunk, y, L, lines, 1.0 / ((double)L * M), nrhs, V3);

@@ -520,6 +520,235 @@ VS_BI_INST(float)
 VS_BI_INST(double)
 #undef VS_BI_INST
 
+
+// ------------------------------------------------------------------------------------------------
+// batched BiCGStab kernels: nrhs independent systems sharing every launch (blockIdx.y = right-hand side)
+// ------------------------------------------------------------------------------------------------
+template <typename T, int K>
+__global__ __launch_bounds__(RED_THREADS) void k_dots_b(DotArgs<T> args, int64_t n, c128* partial) {
+  const int64_t off = (int64_t)blockIdx.y * n;
+  double re[K], im[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) re[k] = im[k] = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * RED_THREADS) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const cplx<T> a = args.a[k][off + i], b = args.b[k][off + i];
+      re[k] += (double)a.x * b.x + (double)a.y * b.y;
+      im[k] += (double)a.x * b.y - (double)a.y * b.x;
+    }
+  }
+  c128 tot[K];
+  block_sum<K>(re, im, tot);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < K; ++k) partial[((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * K + k] = tot[k];
+}
+
+// one workgroup per right-hand side
+__global__ __launch_bounds__(RED_THREADS) void k_bi_scalar_b(const c128* partial, int nblk, int K, int stage,
+                                                             BiScal* scs) {
+  __shared__ double sre[RED_THREADS], sim[RED_THREADS];
+  __shared__ c128 tot[4];
+  const int rhs = blockIdx.x;
+  const c128* pp = partial + (int64_t)rhs * nblk * K;
+  for (int k = 0; k < K; ++k) {
+    double a = 0, b = 0;
+    for (int i = threadIdx.x; i < nblk; i += RED_THREADS) {
+      a += pp[i * K + k].x;
+      b += pp[i * K + k].y;
+    }
+    sre[threadIdx.x] = a;
+    sim[threadIdx.x] = b;
+    __syncthreads();
+    for (int st = RED_THREADS / 2; st > 0; st >>= 1) {
+      if (threadIdx.x < st) {
+        sre[threadIdx.x] += sre[threadIdx.x + st];
+        sim[threadIdx.x] += sim[threadIdx.x + st];
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) tot[k] = mk<double>(sre[0], sim[0]);
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  BiScal& S = scs[rhs];
+  if (!S.active) return;
+  switch (stage) {
+    case BI_RHO:
+      if (S.flag) return;
+      S.rho = tot[0];
+      if (cz(S.rho) || cinf(S.rho)) { S.flag = 4; S.brk = 1; return; }
+      if (!S.first) {
+        S.beta = cdiv(S.rho, S.rho1) * cdiv(S.alpha, S.omega);
+        if (cz(S.beta) || !cfin(S.beta)) { S.flag = 4; S.brk = 2; }
+      }
+      return;
+    case BI_ALPHA: {
+      if (S.flag) return;
+      const c128 rtv = tot[0];
+      if (cz(rtv) || cinf(rtv)) { S.flag = 4; S.brk = 3; return; }
+      S.alpha = cdiv(S.rho, rtv);
+      if (cinf(S.alpha)) { S.flag = 4; S.brk = 4; }
+      return;
+    }
+    case BI_HALF:
+      S.ns = sqrt(tot[0].x);
+      return;
+    case BI_OMEGA: {
+      if (S.flag) return;
+      const c128 tt = tot[0];
+      if (cz(tt) || cinf(tt)) { S.flag = 4; S.brk = 5; return; }
+      S.omega = cdiv(tot[1], tt);
+      if (cinf(S.omega)) { S.flag = 4; S.brk = 6; }
+      return;
+    }
+    default:
+      S.nr = sqrt(tot[0].x);
+      if (!S.flag) {
+        S.rho1 = S.rho;
+        S.first = 0;
+      }
+      return;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(RED_THREADS) void k_bi_p_b(const cplx<T>* r, const cplx<T>* v, cplx<T>* p, int64_t n,
+                                                        const BiScal* scs) {
+  const BiScal& S = scs[blockIdx.y];
+  if (!S.active || S.flag) return;
+  const int64_t off = (int64_t)blockIdx.y * n;
+  const bool first = S.first != 0;
+  const cplx<T> be = ccast<T>(S.beta), om = ccast<T>(S.omega);
+  for (int64_t i = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * RED_THREADS)
+    p[off + i] = first ? r[off + i] : r[off + i] + be * (p[off + i] - om * v[off + i]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(RED_THREADS) void k_bi_axpy2_b(const cplx<T>* x0, const cplx<T>* x1, const cplx<T>* x2,
+                                                            const cplx<T>* x3, cplx<T>* y, cplx<T>* z, int64_t n,
+                                                            const BiScal* scs, int use_omega, c128* partial) {
+  const BiScal& S = scs[blockIdx.y];
+  const int64_t off = (int64_t)blockIdx.y * n;
+  double re[1] = {0.0}, im[1] = {0.0};
+  if (S.active && !S.flag) {
+    const cplx<T> c = ccast<T>(use_omega ? S.omega : S.alpha);
+    for (int64_t i = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * RED_THREADS) {
+      y[off + i] = x0[off + i] + c * x1[off + i];
+      const cplx<T> o = x2[off + i] - c * x3[off + i];
+      z[off + i] = o;
+      re[0] += (double)o.x * o.x + (double)o.y * o.y;
+    }
+  }
+  c128 tot[1];
+  block_sum<1>(re, im, tot);
+  if (threadIdx.x == 0) partial[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = tot[0];
+}
+
+template <typename T>
+void bib_stage_rho(Engine<T>& E, int nrhs, const cplx<T>* rt, const cplx<T>* r, const cplx<T>* v, cplx<T>* p,
+                   BiScal* sc, c128* part) {
+  DotArgs<T> a{};
+  a.a[0] = rt;
+  a.b[0] = r;
+  hipLaunchKernelGGL((k_dots_b<T, 1>), dim3(E.grid, nrhs), dim3(RED_THREADS), 0, E.ctx->stream, a, E.n, part);
+  hipLaunchKernelGGL(k_bi_scalar_b, dim3(nrhs), dim3(RED_THREADS), 0, E.ctx->stream, part, E.grid, 1, (int)BI_RHO, sc);
+  hipLaunchKernelGGL((k_bi_p_b<T>), dim3(E.grid, nrhs), dim3(RED_THREADS), 0, E.ctx->stream, r, v, p, E.n, sc);
+  VS_HIP(hipGetLastError());
+}
+
+template <typename T>
+void bib_stage_alpha_half(Engine<T>& E, int nrhs, const cplx<T>* rt, const cplx<T>* v, const cplx<T>* ph,
+                          const cplx<T>* x, const cplx<T>* r, cplx<T>* xh, cplx<T>* s, BiScal* sc, c128* part) {
+  DotArgs<T> a{};
+  a.a[0] = rt;
+  a.b[0] = v;
+  hipLaunchKernelGGL((k_dots_b<T, 1>), dim3(E.grid, nrhs), dim3(RED_THREADS), 0, E.ctx->stream, a, E.n, part);
+  hipLaunchKernelGGL(k_bi_scalar_b, dim3(nrhs), dim3(RED_THREADS), 0, E.ctx->stream, part, E.grid, 1, (int)BI_ALPHA,
+                     sc);
+  hipLaunchKernelGGL((k_bi_axpy2_b<T>), dim3(E.grid, nrhs), dim3(RED_THREADS), 0, E.ctx->stream, x, ph, r, v, xh, s,
+                     E.n, sc, 0, part);
+  hipLaunchKernelGGL(k_bi_scalar_b, dim3(nrhs), dim3(RED_THREADS), 0, E.ctx->stream, part, E.grid, 1, (int)BI_HALF, sc);
+  VS_HIP(hipGetLastError());
+}
+
+template <typename T>
+void bib_stage_omega_full(Engine<T>& E, int nrhs, const cplx<T>* t, const cplx<T>* s, const cplx<T>* sh,
+                          const cplx<T>* xh, cplx<T>* x, cplx<T>* r, BiScal* sc, c128* part) {
+  DotArgs<T> a{};
+  a.a[0] = t;
+  a.b[0] = t;
+  a.a[1] = t;
+  a.b[1] = s;
+  hipLaunchKernelGGL((k_dots_b<T, 2>), dim3(E.grid, nrhs), dim3(RED_THREADS), 0, E.ctx->stream, a, E.n, part);
+  hipLaunchKernelGGL(k_bi_scalar_b, dim3(nrhs), dim3(RED_THREADS), 0, E.ctx->stream, part, E.grid, 2, (int)BI_OMEGA,
+                     sc);
+  hipLaunchKernelGGL((k_bi_axpy2_b<T>), dim3(E.grid, nrhs), dim3(RED_THREADS), 0, E.ctx->stream, xh, sh, s, t, x, r,
+                     E.n, sc, 1, part);
+  hipLaunchKernelGGL(k_bi_scalar_b, dim3(nrhs), dim3(RED_THREADS), 0, E.ctx->stream, part, E.grid, 1, (int)BI_FULL, sc);
+  VS_HIP(hipGetLastError());
+}
+
+// ||x_i||^2 of nrhs vectors into out_dev[i] (deterministic)
+template <typename T>
+void batch_norm2(Engine<T>& E, int nrhs, const cplx<T>* x, c128* part, c128* out_dev) {
+  DotArgs<T> a{};
+  a.a[0] = x;
+  a.b[0] = x;
+  hipLaunchKernelGGL((k_dots_b<T, 1>), dim3(E.grid, nrhs), dim3(RED_THREADS), 0, E.ctx->stream, a, E.n, part);
+  for (int i = 0; i < nrhs; ++i)
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(RED_THREADS), 0, E.ctx->stream, part + (int64_t)i * E.grid,
+                       E.grid, 1, out_dev + i);
+  VS_HIP(hipGetLastError());
+}
+
+#define VS_BIB_INST(T)                                                                                             \
+  template void bib_stage_rho<T>(Engine<T>&, int, const cplx<T>*, const cplx<T>*, const cplx<T>*, cplx<T>*, BiScal*, \
+                                 c128*);                                                                           \
+  template void bib_stage_alpha_half<T>(Engine<T>&, int, const cplx<T>*, const cplx<T>*, const cplx<T>*,           \
+                                        const cplx<T>*, const cplx<T>*, cplx<T>*, cplx<T>*, BiScal*, c128*);       \
+  template void bib_stage_omega_full<T>(Engine<T>&, int, const cplx<T>*, const cplx<T>*, const cplx<T>*,           \
+                                        const cplx<T>*, cplx<T>*, cplx<T>*, BiScal*, c128*);                       \
+  template void batch_norm2<T>(Engine<T>&, int, const cplx<T>*, c128*, c128*);
+VS_BIB_INST(float)
+VS_BIB_INST(double)
+#undef VS_BIB_INST
+
+
+// per-right-hand-side scaling y_i = c_i x_i and y_i += c_i x_i (c on the device, FP64)
+template <typename T>
+__global__ __launch_bounds__(RED_THREADS) void k_scale_b(const cplx<T>* x, cplx<T>* y, int64_t n, const double* c,
+                                                         int accumulate) {
+  const int64_t off = (int64_t)blockIdx.y * n;
+  const T s = (T)c[blockIdx.y];
+  for (int64_t i = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * RED_THREADS)
+    y[off + i] = accumulate ? y[off + i] + s * x[off + i] : s * x[off + i];
+}
+
+template <typename T>
+void batch_scale(Engine<T>& E, int nrhs, const cplx<T>* x, cplx<T>* y, const double* c_dev, bool accumulate) {
+  hipLaunchKernelGGL((k_scale_b<T>), dim3(E.grid, nrhs), dim3(RED_THREADS), 0, E.ctx->stream, x, y, E.n, c_dev,
+                     accumulate ? 1 : 0);
+  VS_HIP(hipGetLastError());
+}
+template void batch_scale<float>(Engine<float>&, int, const c64*, c64*, const double*, bool);
+template void batch_scale<double>(Engine<double>&, int, const c128*, c128*, const double*, bool);
+
+// y = x - y over nrhs * n values (r = b - A x for a whole batch)
+template <typename T>
+__global__ void k_sub_all(const cplx<T>* x, cplx<T>* y, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = x[i] - y[i];
+}
+template <typename T>
+void batch_sub(Engine<T>& E, int nrhs, const cplx<T>* x, cplx<T>* y) {
+  hipLaunchKernelGGL((k_sub_all<T>), dim3(2048), dim3(256), 0, E.ctx->stream, x, y, E.n * nrhs);
+  VS_HIP(hipGetLastError());
+}
+template void batch_sub<float>(Engine<float>&, int, const c64*, c64*);
+template void batch_sub<double>(Engine<double>&, int, const c128*, c128*);
+
 template struct Engine<float>;
 template struct Engine<double>;
 

@@ -40,6 +40,7 @@ struct BiScal {
   int flag;                   // 0 running, 2 non-finite preconditioned vector, 4 breakdown
   int brk;                    // where the breakdown was detected
   int has_pc;
+  int active;                 // batched solves: 0 once this right-hand side is finished
 };
 
 template <typename T>
@@ -51,6 +52,25 @@ template <typename T>
 void bi_stage_omega_full(Engine<T>& E, const cplx<T>* t, const cplx<T>* s, const cplx<T>* sh, const cplx<T>* xh,
                          cplx<T>* x, cplx<T>* r, BiScal* sc);
 
+// batched forms (nrhs systems, vectors stored consecutively, one BiScal per system)
+template <typename T>
+void bib_stage_rho(Engine<T>& E, int nrhs, const cplx<T>* rt, const cplx<T>* r, const cplx<T>* v, cplx<T>* p,
+                   BiScal* sc, c128* part);
+template <typename T>
+void bib_stage_alpha_half(Engine<T>& E, int nrhs, const cplx<T>* rt, const cplx<T>* v, const cplx<T>* ph,
+                          const cplx<T>* x, const cplx<T>* r, cplx<T>* xh, cplx<T>* s, BiScal* sc, c128* part);
+template <typename T>
+void bib_stage_omega_full(Engine<T>& E, int nrhs, const cplx<T>* t, const cplx<T>* s, const cplx<T>* sh,
+                          const cplx<T>* xh, cplx<T>* x, cplx<T>* r, BiScal* sc, c128* part);
+template <typename T>
+void batch_norm2(Engine<T>& E, int nrhs, const cplx<T>* x, c128* part, c128* out_dev);
+
+template <typename T>
+void batch_scale(Engine<T>& E, int nrhs, const cplx<T>* x, cplx<T>* y, const double* c_dev, bool accumulate);
+template <typename T>
+void batch_sub(Engine<T>& E, int nrhs, const cplx<T>* x, cplx<T>* y);
+
+
 enum { UPD_AXPBY_ = 0, UPD_BICG_P_ = 1, UPD_HALF_ = 2, UPD_SUB_ = 3, UPD_COPY_ = 4, UPD_SCALE_ = 5 };
 
 // device linear operator y = A x and preconditioner y = M^{-1} x (M may be empty)
@@ -58,7 +78,19 @@ template <typename T>
 struct LinOp {
   std::function<void(const cplx<T>*, cplx<T>*)> A;
   std::function<void(const cplx<T>*, cplx<T>*)> M;
+  // batched forms over nrhs consecutive vectors
+  std::function<void(const cplx<T>*, cplx<T>*, int)> Ab;
+  std::function<void(const cplx<T>*, cplx<T>*, int)> Mb;
 };
+
+struct BatchResult {
+  std::vector<int> flag;
+  std::vector<double> relres, iter;
+  int iterations = 0;
+};
+template <typename T>
+BatchResult solve_bicgstab_batch(Engine<T>& E, const LinOp<T>& op, int nrhs, const cplx<T>* B, cplx<T>* X, double tol,
+                                 int maxit);
 
 struct SolveResult {
   int flag = 1;
@@ -81,6 +113,10 @@ SolveResult solve_pgmres(Engine<T>& E, const LinOp<T>& op, int side, const cplx<
 SolveResult solve_refine(Engine<double>& E64, Engine<float>& E32, const LinOp<double>& A64, const LinOp<float>& A32,
                          const c128* b, c128* x, bool has_x0, double tol, int max_outer, int inner_solver,
                          double inner_tol, int inner_maxit, int restart, int* inner_total);
+
+BatchResult solve_refine_batch(Engine<double>& E64, Engine<float>& E32, const LinOp<double>& A64,
+                               const LinOp<float>& A32, int nrhs, const c128* B, c128* X, double tol, int max_outer,
+                               double inner_tol, int inner_maxit, int* outer_out, int* inner_total);
 
 void convert_dev(vs_ctx* ctx, const void* in, int pin, void* out, int pout, int64_t n);
 void mvp_dev(vs_op* op, const void* x, void* y, int transp, int precision, int nrhs);

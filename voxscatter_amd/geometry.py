@@ -293,3 +293,89 @@ def perturbed_hex_plate(L=96, M=96, N=24, m=1.31 + 0.01j, rel=0.05, seed=0) -> S
     s.eps_r = (complex(m) ** 2) * (1.0 + rel * u)
     s.name = "perturbed_hex_plate"
     return s
+
+
+# --------------------------------------------------------------------------------------------------
+# batches of particles and incident fields (multi-RHS workloads)
+# --------------------------------------------------------------------------------------------------
+
+def fibonacci_sphere(n: int) -> np.ndarray:
+    """n nearly uniform unit vectors (golden-angle spiral)."""
+    i = np.arange(n) + 0.5
+    z = 1 - 2 * i / n
+    phi = math.pi * (3 - math.sqrt(5)) * i
+    s = np.sqrt(1 - z * z)
+    return np.stack([s * np.cos(phi), s * np.sin(phi), z], axis=-1)
+
+
+def polarisations(khat) -> tuple[np.ndarray, np.ndarray]:
+    """Two real unit vectors orthogonal to khat and to each other."""
+    khat = np.asarray(khat, float)
+    a = np.array([0.0, 0.0, 1.0]) if abs(khat[2]) < 0.9 else np.array([1.0, 0.0, 0.0])
+    e1 = np.cross(khat, a)
+    e1 /= np.linalg.norm(e1)
+    e2 = np.cross(khat, e1)
+    return e1, e2
+
+
+def plane_wave_batch(scat: Scatterer, ndir: int = 32):
+    """Right-hand sides [Ex; Ey; Ez](idx) for ndir Fibonacci directions x 2 polarisations: (2 ndir, 3 nvox)."""
+    from .excitation import PlaneWavePointWise
+    rows = []
+    meta = []
+    for khat in fibonacci_sphere(ndir):
+        for e in polarisations(khat):
+            E = PlaneWavePointWise(e, scat.k0 * khat, scat.r)
+            rows.append(np.concatenate([E[..., c].reshape(-1, order="F")[scat.idx] for c in range(3)]))
+            meta.append((khat, e))
+    return np.array(rows), meta
+
+
+def _rotation(rng) -> np.ndarray:
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    w, x, y, z = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def random_particle(rng, n: int, shape: str, m: complex) -> Scatterer:
+    """Hex plate (aspect 4:1), hexagonal column (length 2 x diameter) or ellipsoid (1 : 0.7 : 0.5), randomly
+    oriented, inscribed in an n^3 grid of spacing lambda/(10|m|)."""
+    k0 = 2 * math.pi
+    d = 1.0 / (10 * abs(m))
+    r = centred_grid(n, n, n, d)
+    R = _rotation(rng)
+    q = r.reshape(-1, 3) @ R          # body-frame coordinates
+    rb = 0.5 * n * d * 0.98           # bounding-sphere radius
+    x, y, z = q[:, 0], q[:, 1], q[:, 2]
+    s3 = math.sqrt(3.0)
+    if shape == "ellipsoid":
+        a = rb
+        mask = (x / a) ** 2 + (y / (0.7 * a)) ** 2 + (z / (0.5 * a)) ** 2 <= 1
+    else:
+        if shape == "plate":     # diameter 2a, thickness a/2 -> bounding radius sqrt(a^2 + (a/4)^2)
+            a = rb / math.sqrt(1 + 1 / 16)
+            h = a / 4
+        else:                    # column: diameter 2a, length 4a
+            a = rb / math.sqrt(1 + 4)
+            h = 2 * a
+        hexm = (np.abs(y) <= s3 / 2 * a) & (np.abs(s3 * x) + np.abs(y) <= s3 * a)
+        mask = hexm & (np.abs(z) <= h)
+    mask = mask.reshape(n, n, n)
+    idx = find_F(mask)
+    return Scatterer(f"{shape}{n}", r, idx, complex(m), k0)
+
+
+def particle_batch(count: int = 96, seed: int = 0, sizes=(24, 32, 48, 64, 96)):
+    """Seeded particles: size uniform over ``sizes``, shape over plate/column/ellipsoid,
+    Re m ~ U[1.31, 1.6], Im m = 0.005, random orientation."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for _ in range(count):
+        n = int(rng.choice(sizes))
+        shape = str(rng.choice(["plate", "column", "ellipsoid"]))
+        m = complex(rng.uniform(1.31, 1.6), 0.005)
+        out.append(random_particle(rng, n, shape, m))
+    return out
