@@ -121,3 +121,56 @@ def test_mvp_full_size_linearity_and_adjoint(ctx):
     Ax = op.mvp(x)
     AHy = op.mvp(y, transp=True)
     assert abs(np.vdot(y, Ax) - np.vdot(AHy, x)) <= 1e-12 * abs(np.vdot(y, Ax))
+
+
+@pytest.mark.parametrize("shape", [(24, 24, 24), (32, 32, 32), (64, 64, 16), (32, 32, 8)])
+@pytest.mark.parametrize("transp", ["notransp", "transp"])
+def test_fused_slab_mvp_matches_oracle_and_unfused(ctx, shape, transp, monkeypatch):
+    """Grids whose (2M, 2N) has a fused y-z slab kernel: FP64 fused matvec against the oracle, and fused
+    against the five-pass path in FP64 and FP32."""
+    print(f"[fused mvp] shape={shape} {transp}", flush=True)
+    k = 2 * math.pi
+    r, vox, idx3, x = _setup(shape, seed=6)
+    L, M, N = shape
+    fref = O.fft_operator(O.circulant_nop_const(O.toeplitz_dda(r, k)))
+    dV = 0.07 ** 3
+    chi = np.zeros(L * M * N)
+    chi[vox] = 1
+    chi = chi.reshape((L, M, N), order="F")
+    alpha = O.alpha_ldr(1.31 + 0.02j, k, 0.07, [0, 0, 1], [1, 0, 0])
+    yref = OM.MVP(x, fref, np.ones((L, M, N)), dV * chi, 1 / alpha, transp, idx3)
+    op = api.Operator.dda(r, k, precisions=capi.VS_FP64 | capi.VS_FP32, ctx=ctx)
+    op.set_domain(idx3, 1 / alpha, 1.0, dV * chi.reshape(-1, order="F"))
+    out = {}
+    for fused in ("1", "0"):
+        monkeypatch.setenv("VS_FUSED", fused)
+        out[("fp64", fused)] = op.mvp(x, transp=transp == "transp")
+        xt = api.DeviceArray.from_numpy(x[None], np.complex64, ctx)
+        yt = api.DeviceArray((1, x.size), np.complex64, ctx)
+        op.mvp_dev(xt.ptr, yt.ptr, transp == "transp", capi.VS_FP32, 1)
+        out[("fp32", fused)] = yt.numpy()[0].astype(np.complex128)
+    assert _rel(out[("fp64", "1")], yref) <= 1e-12
+    assert _rel(out[("fp64", "0")], yref) <= 1e-12
+    assert _rel(out[("fp32", "1")], yref) <= 2e-5
+    assert _rel(out[("fp32", "1")], out[("fp32", "0")]) <= 2e-5
+
+
+def test_fused_slab_mvp_koch_full_size_fp32(ctx, monkeypatch):
+    """128x128x16 Koch prism in FP32 (fused y-z slab kernel for (256, 32)) against the FP64 matvec."""
+    s = G.koch_prism()
+    from voxscatter_amd.dda import DDASystem
+    sysd = DDASystem.build(s, precisions=capi.VS_FP64 | capi.VS_FP32, ctx=ctx)
+    op = sysd.op
+    n = op.n
+    rng = np.random.default_rng(7)
+    X = rng.normal(size=(2, n)) + 1j * rng.normal(size=(2, n))
+    yref = np.stack([op.mvp(X[i]) for i in range(2)])
+    xt = api.DeviceArray.from_numpy(X, np.complex64, ctx)
+    yt = api.DeviceArray(X.shape, np.complex64, ctx)
+    res = {}
+    for fused in ("1", "0"):
+        monkeypatch.setenv("VS_FUSED", fused)
+        op.mvp_dev(xt.ptr, yt.ptr, False, capi.VS_FP32, 2)
+        res[fused] = yt.numpy().astype(np.complex128)
+        assert _rel(res[fused], yref) <= 2e-5
+    assert _rel(res["1"], res["0"]) <= 2e-5

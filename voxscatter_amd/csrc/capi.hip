@@ -12,7 +12,8 @@ void op_build_fN(vs_op* op);
 void op_compress_tensor(vs_op* op);
 void build_circ_L(vs_ctx* ctx, const c128* toep, c128* cL, int L, int M, int N);
 void build_circ_M(vs_ctx* ctx, const c128* cL, c128* F, int L, int M, int N);
-void assemble2(vs_ctx* ctx, const c128* F, int L, int M, int N, c128 a, const c128* w, c128 ws, c128* out);
+void assemble2(vs_ctx* ctx, const c128* F, int L, int M, int N, int Lb, int Mb, c128 a, const c128* w, c128 ws,
+               c128* out);
 void assemble1(vs_ctx* ctx, const c128* cL, int L, int M, int N, c128 a, const c128* w, c128 ws, c128* out);
 void invert_blocks_dev(vs_ctx* ctx, c128* blocks, int64_t nblk, int64_t n);
 void pc_store_inverse(vs_pc* pc, const c128* inv);
@@ -52,7 +53,7 @@ void set_coef(vs_op* op, Coef& c, bool scalar, c128 s, const std::vector<c128>* 
 }
 
 std::vector<int> build_map3(const int64_t* idx, int64_t nidx, int64_t V3) {
-  VS_CHECK(nidx >= 0 && nidx <= V3, VS_ERR_ARG, "nidx out of range");
+  VS_CHECK(nidx >= 1 && nidx <= V3, VS_ERR_ARG, "idx must hold between 1 and 3*L*M*N unknowns");
   VS_CHECK(V3 < (int64_t)INT32_MAX, VS_ERR_UNSUPPORTED, "volume too large for 32-bit unknown map");
   std::vector<int> map(V3, -1);
   for (int64_t i = 0; i < nidx; ++i) {
@@ -378,7 +379,7 @@ int vs_circ_2_level(vs_ctx* ctx, int L, int M, int N, const double* circ_L, doub
     if (blocks) {
       const int64_t n = 3 * (int64_t)N;
       DevBuf B(sizeof(c128) * V / N * n * n);
-      assemble2(ctx, F.as<c128>(), L, M, N, mk<double>(0, 0), nullptr, mk<double>(-1, 0), B.as<c128>());
+      assemble2(ctx, F.as<c128>(), L, M, N, L, M, mk<double>(0, 0), nullptr, mk<double>(-1, 0), B.as<c128>());
       d2h(ctx, blocks, B.p, sizeof(c128) * (size_t)L * M * n * n);
     }
     if (circ_M) {
@@ -443,7 +444,11 @@ int vs_pc_create_circ(vs_op* op, int levels, const double a[2], const double* w,
     pc->map3.ensure(sizeof(int) * 3 * V);
     VS_HIP(hipMemcpyAsync(pc->map3.p, op->map3.p, sizeof(int) * 3 * V, hipMemcpyDeviceToDevice, ctx->stream));
     const int64_t nb = levels == 2 ? 3LL * N : 3LL * M * N;
-    const int64_t nblk = levels == 2 ? (int64_t)L * M : L;
+    // 2-level: only the blocks kx <= L/2, ky <= M/2 are built, inverted and kept (mirror symmetry)
+    pc->mirror = levels == 2;
+    pc->Lb = levels == 2 ? L / 2 + 1 : L;
+    pc->Mb = levels == 2 ? M / 2 + 1 : 1;
+    const int64_t nblk = levels == 2 ? (int64_t)pc->Lb * pc->Mb : L;
     VS_CHECK(nb <= INT32_MAX, VS_ERR_UNSUPPORTED, "block too large");
     const double bytes = (double)nblk * nb * nb * 16.0;
     VS_CHECK(bytes < 64e9, VS_ERR_OOM, "preconditioner blocks exceed the 64 GB build limit");
@@ -462,7 +467,7 @@ int vs_pc_create_circ(vs_op* op, int levels, const double a[2], const double* w,
     if (levels == 2) {
       DevBuf F(sizeof(c128) * 6 * V);
       build_circ_M(ctx, cL.as<c128>(), F.as<c128>(), L, M, N);
-      assemble2(ctx, F.as<c128>(), L, M, N, av, w ? wdev.as<c128>() : nullptr, ws, B.as<c128>());
+      assemble2(ctx, F.as<c128>(), L, M, N, pc->Lb, pc->Mb, av, w ? wdev.as<c128>() : nullptr, ws, B.as<c128>());
     } else {
       assemble1(ctx, cL.as<c128>(), L, M, N, av, w ? wdev.as<c128>() : nullptr, ws, B.as<c128>());
     }
@@ -493,6 +498,9 @@ int vs_pc_create_blocks(vs_ctx* ctx, int levels, int L, int M, int N, const doub
     pc->V = (int64_t)L * M * N;
     pc->precisions = precisions;
     const int64_t nb = levels == 2 ? 3LL * N : 3LL * M * N;
+    // caller-supplied blocks: the full set, no symmetry assumed
+    pc->Lb = L;
+    pc->Mb = levels == 2 ? M : 1;
     const int64_t nblk = levels == 2 ? (int64_t)L * M : L;
     VS_CHECK(levels == 2 || nb <= 4096, VS_ERR_UNSUPPORTED, "1-level blocks limited to 3MN <= 4096");
     pc->nb = (int)nb;

@@ -52,6 +52,7 @@ struct vs_op {
   // octant of the tensor actually read by the matvec: kx <= L, ky <= M, kz <= N, 6 * Pc values; the
   // other octants follow from the parity of each component (fN(2L-kx) = s_x fN(kx), etc.)
   vs::DevBuf fc64, fc32;
+  vs::DevBuf fs64, fs32;  // the octant stored slab-major, for the fused slab matvec (when its shape has one)
   int64_t Pc = 0;
   // domain
   int64_t nunk = 0;
@@ -71,6 +72,8 @@ struct vs_pc {
   int precisions = 0;
   int nb = 0;            // block size: 3N (2-level) or 3MN (1-level)
   bool sym = false;      // 2-level blocks stored as packed upper triangles (B^T = D B D)
+  bool mirror = false;   // 2-level blocks stored for kx <= L/2, ky <= M/2 only (the rest by sign symmetry)
+  int Lb = 0, Mb = 0;    // 2-level stored block grid (L x M, or L/2+1 x M/2+1 when mirrored)
   vs::DevBuf inv64, inv32;  // apply layout (2-level: [ky][r][s][kx]; 1-level: [kx][r][s])
   vs::DevBuf map3;       // own copy of the unknown map
   int64_t nunk = 0;

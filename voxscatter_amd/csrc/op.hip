@@ -9,11 +9,17 @@
 //   mv_ix   : inverse FFT along x, keep x<L, y = Gram Mr J - Mc conv / P, scatter to the compressed vector
 // The zero padding is never materialised in HBM: pad values are produced in LDS and cropped values are
 // never written.
+//
+// Fused slab path (shapes in VS_YZ_SHAPES whose padded y spectrum fits in LDS): the x pass writes the
+// volume slab-major, Sx(kx)[c][z][y], and one workgroup per kx slab does the y FFT, the z FFT, the tensor
+// product, the inverse z FFT and the inverse y FFT in LDS (k_yz), so HBM sees the slab once in and once
+// out plus the octant of the tensor for that kx, stored slab-major as well.
 #include "fft2.hpp"
 #include "fft_device.hpp"
 #include "handles.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 
 namespace vs {
@@ -90,16 +96,24 @@ __global__ __launch_bounds__(FFT_THREADS) void k_col_fft(FftDev<T> plan, cplx<T>
 // ------------------------------------------------------------------------------------------------
 // IO functors
 // ------------------------------------------------------------------------------------------------
+// Every IO functor also offers pre(line, k, rhs): the loads a store depends on (unknown map, solution
+// value), issued for all of a thread's outputs before any store so that they are in flight together;
+// store(line, k, rhs, v, pre) then only computes and writes.  Loads are branch-free (clamped addresses,
+// the result selected afterwards) for the same reason.
+struct NoPre {};
+
 template <typename T>
 struct IoDense {
   cplx<T>* in;
   cplx<T>* out;
   int64_t ld_in, ld_out, rhs_stride;
   T scale;
+  using Pre = NoPre;
   __device__ __forceinline__ cplx<T> load(int64_t g, int x, int rhs) const {
     return in[rhs * rhs_stride + g * ld_in + x];
   }
-  __device__ __forceinline__ void store(int64_t g, int k, int rhs, cplx<T> v) const {
+  __device__ __forceinline__ Pre pre(int64_t, int, int) const { return {}; }
+  __device__ __forceinline__ void store(int64_t g, int k, int rhs, cplx<T> v, const Pre& = {}) const {
     out[rhs * rhs_stride + g * ld_out + k] = scale * v;
   }
 };
@@ -113,11 +127,14 @@ struct IoGather {
   int L;
   cplx<T>* out;
   int64_t out_rhs;
+  using Pre = NoPre;
   __device__ __forceinline__ cplx<T> load(int64_t g, int xp, int rhs) const {
     const int m = map3[g * L + xp];
-    return m >= 0 ? x[rhs * nunk + m] : mk<T>(0, 0);
+    const cplx<T> v = x[rhs * nunk + (m >= 0 ? m : 0)];
+    return m >= 0 ? v : mk<T>(0, 0);
   }
-  __device__ __forceinline__ void store(int64_t g, int k, int rhs, cplx<T> v) const {
+  __device__ __forceinline__ Pre pre(int64_t, int, int) const { return {}; }
+  __device__ __forceinline__ void store(int64_t g, int k, int rhs, cplx<T> v, const Pre& = {}) const {
     out[rhs * out_rhs + g * L + k] = v;
   }
 };
@@ -131,13 +148,15 @@ struct IoScatter {
   cplx<T>* y;
   int64_t in_rhs;
   T scale;
+  struct Pre { int m; };
   __device__ __forceinline__ cplx<T> load(int64_t g, int xp, int rhs) const {
     return in[rhs * in_rhs + g * L + xp];
   }
-  __device__ __forceinline__ void store(int64_t g, int k, int rhs, cplx<T> v) const {
-    const int m = map3[g * L + k];
-    if (m >= 0) y[rhs * nunk + m] = scale * v;
+  __device__ __forceinline__ Pre pre(int64_t g, int k, int) const { return {map3[g * L + k]}; }
+  __device__ __forceinline__ void store(int64_t g, int k, int rhs, cplx<T> v, const Pre& p) const {
+    if (p.m >= 0) y[rhs * nunk + p.m] = scale * v;
   }
+  __device__ __forceinline__ void store(int64_t g, int k, int rhs, cplx<T> v) const { store(g, k, rhs, v, pre(g, k, rhs)); }
 };
 
 // matvec first pass: gather (+ transp pre-op Mc conj(J)), pad, x FFT -> S rows
@@ -151,15 +170,16 @@ struct IoMvFx {
   CoefDev<T> mc;
   cplx<T>* S;
   int64_t S_rhs;
+  using Pre = NoPre;
   __device__ __forceinline__ cplx<T> load(int64_t g, int xp, int rhs) const {
     const int64_t gi = g * L + xp;
     const int m = map3[gi];
-    if (m < 0) return mk<T>(0, 0);
-    cplx<T> J = x[rhs * nunk + m];
+    cplx<T> J = x[rhs * nunk + (m >= 0 ? m : 0)];
     if (transp) J = mc.at(gi % V) * cconj(J);
-    return J;
+    return m >= 0 ? J : mk<T>(0, 0);
   }
-  __device__ __forceinline__ void store(int64_t g, int k, int rhs, cplx<T> v) const {
+  __device__ __forceinline__ Pre pre(int64_t, int, int) const { return {}; }
+  __device__ __forceinline__ void store(int64_t g, int k, int rhs, cplx<T> v, const Pre& = {}) const {
     const int64_t y = g % M, zc = g / M;
     S[rhs * S_rhs + k + (int64_t)2 * L * (y + (int64_t)2 * M * zc)] = v;
   }
@@ -178,24 +198,61 @@ struct IoMvIx {
   const cplx<T>* S;
   int64_t S_rhs;
   T invP;
+  struct Pre {
+    int m;
+    cplx<T> J;
+  };
   __device__ __forceinline__ cplx<T> load(int64_t g, int kx, int rhs) const {
     const int64_t yy = g % M, zc = g / M;
     return S[rhs * S_rhs + kx + (int64_t)2 * L * (yy + (int64_t)2 * M * zc)];
   }
-  __device__ __forceinline__ void store(int64_t g, int xp, int rhs, cplx<T> v) const {
-    const int64_t gi = g * L + xp;
-    const int m = map3[gi];
-    if (m < 0) return;
-    const int64_t vox = gi % V;
-    const cplx<T> J = x[rhs * nunk + m];
+  __device__ __forceinline__ Pre pre(int64_t g, int xp, int rhs) const {
+    const int m = map3[g * L + xp];
+    return {m, x[rhs * nunk + (m >= 0 ? m : 0)]};
+  }
+  __device__ __forceinline__ void store(int64_t g, int xp, int rhs, cplx<T> v, const Pre& p) const {
+    if (p.m < 0) return;
+    const int64_t vox = (g * L + xp) % V;
     const cplx<T> conv = invP * v;
     cplx<T> out;
     if (!transp)
-      out = a.at(vox) * J - b.at(vox) * conv;
+      out = a.at(vox) * p.J - b.at(vox) * conv;
     else
-      out = at.at(vox) * J - cconj(conv);
-    y[rhs * nunk + m] = out;
+      out = at.at(vox) * p.J - cconj(conv);
+    y[rhs * nunk + p.m] = out;
   }
+  __device__ __forceinline__ void store(int64_t g, int xp, int rhs, cplx<T> v) const {
+    store(g, xp, rhs, v, pre(g, xp, rhs));
+  }
+};
+
+// slab-major variants: the x pass stores Sx(kx)[g] with g = y + M (z + N c) the line index, so that a
+// kx slab of 3MN values is contiguous (stores staged through LDS: consecutive threads, consecutive lines)
+template <typename T>
+struct IoMvFxSlab {
+  IoMvFx<T> in;
+  int64_t SL;  // slab length 3MN
+  using Pre = NoPre;
+  __device__ __forceinline__ cplx<T> load(int64_t g, int xp, int rhs) const { return in.load(g, xp, rhs); }
+  __device__ __forceinline__ Pre pre(int64_t, int, int) const { return {}; }
+  __device__ __forceinline__ void store(int64_t g, int k, int rhs, cplx<T> v, const Pre& = {}) const {
+    in.S[rhs * in.S_rhs + k * SL + g] = v;
+  }
+};
+
+template <typename T>
+struct IoMvIxSlab {
+  IoMvIx<T> out;
+  int64_t SL;
+  using Pre = typename IoMvIx<T>::Pre;
+  __device__ __forceinline__ cplx<T> load(int64_t g, int kx, int rhs) const {
+    return out.S[rhs * out.S_rhs + kx * SL + g];
+  }
+  __device__ __forceinline__ Pre pre(int64_t g, int xp, int rhs) const { return out.pre(g, xp, rhs); }
+  __device__ __forceinline__ void store(int64_t g, int xp, int rhs, cplx<T> v, const Pre& p) const {
+    out.store(g, xp, rhs, v, p);
+  }
+  __device__ __forceinline__ void store(int64_t g, int xp, int rhs, cplx<T> v) const { out.store(g, xp, rhs, v); }
 };
 
 // Compressed tensor: the octant kx <= L, ky <= M, kz <= N of fN.  Mirrored points pick up the parity
@@ -215,6 +272,25 @@ struct TensorC {
     out[3] = f[3 * Pc + o];
     out[4] = f[4 * Pc + o];
     out[5] = f[5 * Pc + o];
+    if (fx != fy) out[1] = -out[1];
+    if (fx != fz) out[2] = -out[2];
+    if (fy != fz) out[4] = -out[4];
+  }
+};
+
+// The same octant stored slab-major: value (ix, iy, iz) of component p at ((ix*6 + p)*(N+1) + iz)*(M+1) + iy,
+// so that the tensor of one kx slab is contiguous and consecutive ky are consecutive addresses.
+template <typename T>
+struct TensorS {
+  const cplx<T>* f;
+  int L, M, N;
+  __device__ __forceinline__ void load6(int kx, int ky, int kz, cplx<T>* out) const {
+    const bool fx = kx > L, fy = ky > M, fz = kz > N;
+    const int ix = fx ? 2 * L - kx : kx, iy = fy ? 2 * M - ky : ky, iz = fz ? 2 * N - kz : kz;
+    const int64_t cs = (int64_t)(N + 1) * (M + 1);
+    const cplx<T>* b = f + (int64_t)ix * 6 * cs + (int64_t)iz * (M + 1) + iy;
+#pragma unroll
+    for (int p = 0; p < 6; ++p) out[p] = b[p * cs];
     if (fx != fy) out[1] = -out[1];
     if (fx != fz) out[2] = -out[2];
     if (fy != fz) out[4] = -out[4];
@@ -285,6 +361,22 @@ __global__ void k_compress_tensor(const c128* __restrict__ fN, cplx<To>* __restr
   fc[t] = ccast<To>(fN[p * P + ix + 2LL * L * (iy + 2LL * M * iz)]);
 }
 
+template <typename To>
+__global__ void k_compress_tensor_slab(const c128* __restrict__ fN, cplx<To>* __restrict__ fs, int L, int M, int N) {
+  const int64_t L1 = L + 1, M1 = M + 1, N1 = N + 1;
+  const int64_t P = 8LL * L * M * N;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 6 * L1 * M1 * N1) return;
+  // destination order ((ix*6 + p)*N1 + iz)*M1 + iy
+  const int64_t iy = t % M1;
+  int64_t q = t / M1;
+  const int64_t iz = q % N1;
+  q /= N1;
+  const int p = (int)(q % 6);
+  const int64_t ix = q / 6;
+  fs[t] = ccast<To>(fN[p * P + ix + 2LL * L * (iy + 2LL * M * iz)]);
+}
+
 // ------------------------------------------------------------------------------------------------
 // compile-time two-pass FFT kernels (lengths in VS_FFT2_SIZES)
 // ------------------------------------------------------------------------------------------------
@@ -300,8 +392,10 @@ struct RowCfg {
   static constexpr int THREADS = TPL * LPW;
 };
 
-// lines along the contiguous axis; IO supplies load(line, pos, rhs) / store(line, k, rhs, v)
-template <typename T, int N, bool INV, class IO>
+// lines along the contiguous axis; IO supplies load(line, pos, rhs) / store(line, k, rhs, v).
+// TIN / TOUT stage the load / store through LDS so that consecutive threads touch consecutive lines at
+// one position (for line-contiguous layouts such as the slab-major matvec volume).
+template <typename T, int N, bool INV, class IO, bool TIN = false, bool TOUT = false>
 __global__ __launch_bounds__(256) void k_row2(const cplx<T>* __restrict__ tw, IO io, int64_t nlines, int n_in,
                                               int n_out) {
   using C = RowCfg<T, N>;
@@ -309,17 +403,45 @@ __global__ __launch_bounds__(256) void k_row2(const cplx<T>* __restrict__ tw, IO
   __shared__ cplx<T> lds[N * LP];
   const int t = threadIdx.x;
   const int ll = t / TPL, s = t - ll * TPL;
-  const int64_t g = (int64_t)blockIdx.x * LPW + ll;
+  const int64_t g0 = (int64_t)blockIdx.x * LPW;
+  const int64_t g = g0 + ll;
   const int rhs = blockIdx.y;
   const bool live = g < nlines;
+  constexpr int NI = N / TPL;  // staged elements per thread (THREADS = TPL * LPW, TPL divides N)
+  if (TIN) {
+    cplx<T> st[NI];
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+      const int e = t + i * C::THREADS;
+      const int l = e % LPW, n = e / LPW;
+      const bool ok = g0 + l < nlines && n < n_in;
+      st[i] = io.load(ok ? g0 + l : 0, ok ? n : 0, rhs);
+      if (!ok) st[i] = mk<T>(0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+      const int e = t + i * C::THREADS;
+      lds[(e / LPW) * LP + e % LPW] = st[i];
+    }
+    __syncthreads();
+  }
+  cplx<T> v[R2];
   if (s < R1) {
-    cplx<T> v[R2];
 #pragma unroll
     for (int n2 = 0; n2 < R2; ++n2) {
       const int n = s + R1 * n2;
-      v[n2] = (live && n < n_in) ? io.load(g, n, rhs) : mk<T>(0, 0);
+      if (TIN) {
+        v[n2] = lds[n * LP + ll];
+      } else {
+        const bool ok = live && n < n_in;
+        v[n2] = io.load(ok ? g : 0, ok ? n : 0, rhs);
+        if (!ok) v[n2] = mk<T>(0, 0);
+      }
     }
     fft2_step1<T, N, INV>(v, s, tw);
+  }
+  if (TIN) __syncthreads();  // every thread has read its inputs before the exchange overwrites them
+  if (s < R1) {
 #pragma unroll
     for (int k2 = 0; k2 < R2; ++k2) lds[(s * R2 + k2) * LP + ll] = v[k2];
   }
@@ -329,10 +451,48 @@ __global__ __launch_bounds__(256) void k_row2(const cplx<T>* __restrict__ tw, IO
 #pragma unroll
     for (int n1 = 0; n1 < R1; ++n1) u[n1] = lds[(n1 * R2 + s) * LP + ll];
     fft2_step2<T, N, INV>(u);
+    if (TOUT) {
 #pragma unroll
-    for (int k1 = 0; k1 < R1; ++k1) {
-      const int k = s + R2 * k1;
-      if (live && k < n_out) io.store(g, k, rhs, u[k1]);
+      for (int k1 = 0; k1 < R1; ++k1) lds[(s + R2 * k1) * LP + ll] = u[k1];  // the locations just read
+    } else {
+      // the loads the stores depend on, PB outputs at a time (bounded register use)
+      constexpr int PB = sizeof(T) == 8 ? 4 : 8;
+#pragma unroll
+      for (int k0 = 0; k0 < R1; k0 += PB) {
+        typename IO::Pre pr[PB];
+#pragma unroll
+        for (int j = 0; j < PB; ++j) {
+          const int k = s + R2 * (k0 + j);
+          const bool ok = k0 + j < R1 && live && k < n_out;
+          pr[j] = io.pre(ok ? g : 0, ok ? k : 0, rhs);
+        }
+#pragma unroll
+        for (int j = 0; j < PB; ++j) {
+          const int k = s + R2 * (k0 + j);
+          if (k0 + j < R1 && live && k < n_out) io.store(g, k, rhs, u[k0 + j], pr[j]);
+        }
+      }
+    }
+  }
+  if (TOUT) {
+    __syncthreads();
+    constexpr int PB = sizeof(T) == 8 ? 4 : 8;
+#pragma unroll
+    for (int i0 = 0; i0 < NI; i0 += PB) {
+      typename IO::Pre pr[PB];
+#pragma unroll
+      for (int j = 0; j < PB; ++j) {
+        const int e = t + (i0 + j) * C::THREADS;
+        const int l = e % LPW, k = e / LPW;
+        const bool ok = i0 + j < NI && g0 + l < nlines && k < n_out;
+        pr[j] = io.pre(ok ? g0 + l : 0, ok ? k : 0, rhs);
+      }
+#pragma unroll
+      for (int j = 0; j < PB; ++j) {
+        const int e = t + (i0 + j) * C::THREADS;
+        const int l = e % LPW, k = e / LPW;
+        if (i0 + j < NI && g0 + l < nlines && k < n_out) io.store(g0 + l, k, rhs, lds[k * LP + l], pr[j]);
+      }
     }
   }
 }
@@ -471,6 +631,243 @@ __global__ __launch_bounds__(512) void k_fz2(const cplx<T>* __restrict__ tw, cpl
 }
 
 // ------------------------------------------------------------------------------------------------
+// fused y-z slab pass: one workgroup per kx slab, everything between the two x passes in LDS
+// ------------------------------------------------------------------------------------------------
+// (2M, 2N) pairs with a compiled fused kernel; a shape runs fused when its LDS footprint also fits
+#define VS_YZ_SHAPES(X) X(256, 32) X(128, 32) X(128, 16) X(64, 16) X(64, 64) X(48, 48) X(96, 96) X(192, 48)
+
+constexpr int VS_LDS_MAX = 160 * 1024;
+
+template <typename T, int M2, int N2>
+struct YzCfg {
+  static constexpr int M = M2 / 2, N = N2 / 2, NL = 3 * N;
+  static constexpr int NLP = NL + 1;  // LDS pitch of one ky row of the spectrum [ky][c][z]
+  static constexpr int RY1 = Split<M2>::R1, RY2 = Split<M2>::R2;
+  static constexpr int RZ1 = Split<N2>::R1, RZ2 = Split<N2>::R2, TPZ = cmax(RZ1, RZ2);
+  static constexpr int THREADS = 1024;
+  static constexpr int IY1 = (NL * RY1 + THREADS - 1) / THREADS;  // y-FFT step-1 items per thread
+  static constexpr int IY2 = (NL * RY2 + THREADS - 1) / THREADS;
+  static constexpr int SPEC = M2 * NLP;
+  static constexpr int INB = NL * (M + 1);  // input staging, overlapping the spectrum
+  static constexpr int SPECMAX = SPEC > INB ? SPEC : INB;
+  static constexpr int fit_wz(int w) {
+    return (w > 1 && (int64_t)sizeof(cplx<T>) * (SPECMAX + 6 * N2 * w) > VS_LDS_MAX) ? fit_wz(w / 2) : w;
+  }
+  static constexpr int WZ0 = pow2_floor(THREADS / (3 * TPZ));
+  static constexpr int WZ = fit_wz(WZ0 > M2 ? M2 : WZ0);  // ky columns per z pass
+  static constexpr int ZB = 3 * N2 * WZ;
+  static constexpr size_t BYTES = sizeof(cplx<T>) * (size_t)(SPECMAX + 2 * ZB);
+};
+
+template <typename T, int M2, int N2>
+__global__ __launch_bounds__(1024) void k_yz(const cplx<T>* __restrict__ twy, const cplx<T>* __restrict__ twz,
+                                             cplx<T>* S, TensorS<T> fN, int64_t S_rhs) {
+  using C = YzCfg<T, M2, N2>;
+  constexpr int M = C::M, N = C::N, NL = C::NL, NLP = C::NLP, RY1 = C::RY1, RY2 = C::RY2;
+  constexpr int RZ1 = C::RZ1, RZ2 = C::RZ2, TPZ = C::TPZ, WZ = C::WZ, TH = C::THREADS;
+  cplx<T>* sp = reinterpret_cast<cplx<T>*>(vs_smem);
+  cplx<T>* xch = sp + C::SPECMAX;
+  cplx<T>* spec = xch + C::ZB;
+  const int t = threadIdx.x;
+  const int L = fN.L, L2 = 2 * L;
+  // consecutive logical ids are the mirror slabs kx, 2L - kx (same tensor octant); hardware dispatch
+  // sends workgroup i to XCD i mod 8, so logical ids are laid out XCD by XCD to share that XCD's L2
+  const int per_xcd = (L2 + 7) / 8;
+  const int q = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+  if (q >= L2) return;
+  const int kx = q == 0 ? 0 : (q == 1 ? L : ((q & 1) == 0 ? 1 + (q - 2) / 2 : L2 - 1 - (q - 3) / 2));
+  cplx<T>* g = S + (int64_t)blockIdx.y * S_rhs + (int64_t)kx * NL * M;
+
+  // 0. slab -> LDS, line-major with pitch M + 1 (all of a thread's loads in flight together)
+  {
+    constexpr int I0 = (NL * M + TH - 1) / TH;
+    cplx<T> st[I0];
+#pragma unroll
+    for (int i = 0; i < I0; ++i) {
+      const int e = t + i * TH;
+      st[i] = g[e < NL * M ? e : 0];
+    }
+#pragma unroll
+    for (int i = 0; i < I0; ++i) {
+      const int e = t + i * TH;
+      if (e < NL * M) sp[(e / M) * (M + 1) + e % M] = st[i];
+    }
+  }
+  __syncthreads();
+  // 1. y FFT of the zero-padded lines, step 1 (results held in registers across the barrier)
+  cplx<T> v[C::IY1][RY2];
+#pragma unroll
+  for (int i = 0; i < C::IY1; ++i) {
+    const int it = t + i * TH;
+    if (it < NL * RY1) {
+      const int line = it % NL, n1 = it / NL;
+#pragma unroll
+      for (int n2 = 0; n2 < RY2; ++n2) {
+        const int n = n1 + RY1 * n2;
+        v[i][n2] = n < M ? sp[line * (M + 1) + n] : mk<T>(0, 0);
+      }
+      fft2_step1<T, M2, false>(v[i], n1, twy);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < C::IY1; ++i) {
+    const int it = t + i * TH;
+    if (it < NL * RY1) {
+      const int line = it % NL, n1 = it / NL;
+#pragma unroll
+      for (int k2 = 0; k2 < RY2; ++k2) sp[(n1 * RY2 + k2) * NLP + line] = v[i][k2];
+    }
+  }
+  __syncthreads();
+  // 2. step 2, in place (each item reads and writes the same locations)
+#pragma unroll
+  for (int i = 0; i < C::IY2; ++i) {
+    const int it = t + i * TH;
+    if (it < NL * RY2) {
+      const int line = it % NL, k2 = it / NL;
+      cplx<T> u[RY1];
+#pragma unroll
+      for (int n1 = 0; n1 < RY1; ++n1) u[n1] = sp[(n1 * RY2 + k2) * NLP + line];
+      fft2_step2<T, M2, false>(u);
+#pragma unroll
+      for (int k1 = 0; k1 < RY1; ++k1) sp[(k2 + RY2 * k1) * NLP + line] = u[k1];
+    }
+  }
+  __syncthreads();
+  // 3. z stage on WZ ky columns at a time: zero-pad z to 2N, FFT, tensor product, inverse FFT, crop.
+  // The tensor values of the next pass are loaded while the current one runs.
+  {
+    const int col = t % WZ, r = t / WZ;
+    const int c = r % 3, sz = r / 3;
+    const bool act = t < 3 * TPZ * WZ;
+    constexpr int IZ = (WZ * N2 + TH - 1) / TH;
+    cplx<T> fnext[IZ][6];
+    auto load_f = [&](int ky0) {
+#pragma unroll
+      for (int j = 0; j < IZ; ++j) {
+        const int e = t + j * TH;
+        const int cc = e % WZ, kz = e / WZ;
+        const int ky = min(ky0 + cc, M2 - 1), kzc = min(kz, N2 - 1);
+        fN.load6(kx, ky, kzc, fnext[j]);
+      }
+    };
+    load_f(0);
+    for (int ky0 = 0; ky0 < M2; ky0 += WZ) {
+      cplx<T> fcur[IZ][6];
+#pragma unroll
+      for (int j = 0; j < IZ; ++j)
+#pragma unroll
+        for (int p = 0; p < 6; ++p) fcur[j][p] = fnext[j][p];
+      if (ky0 + WZ < M2) load_f(ky0 + WZ);
+      const int ky = ky0 + col;
+      const bool kyv = ky < M2;
+      if (act && sz < RZ1) {
+        cplx<T> vz[RZ2];
+#pragma unroll
+        for (int n2 = 0; n2 < RZ2; ++n2) {
+          const int n = sz + RZ1 * n2;
+          vz[n2] = (kyv && n < N) ? sp[ky * NLP + c * N + n] : mk<T>(0, 0);
+        }
+        fft2_step1<T, N2, false>(vz, sz, twz);
+#pragma unroll
+        for (int k2 = 0; k2 < RZ2; ++k2) xch[(c * N2 + sz * RZ2 + k2) * WZ + col] = vz[k2];
+      }
+      __syncthreads();
+      if (act && sz < RZ2) {
+        cplx<T> u[RZ1];
+#pragma unroll
+        for (int n1 = 0; n1 < RZ1; ++n1) u[n1] = xch[(c * N2 + n1 * RZ2 + sz) * WZ + col];
+        fft2_step2<T, N2, false>(u);
+#pragma unroll
+        for (int k1 = 0; k1 < RZ1; ++k1) spec[(c * N2 + sz + RZ2 * k1) * WZ + col] = u[k1];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < IZ; ++j) {
+        const int e = t + j * TH;
+        const int cc = e % WZ, kz = e / WZ;
+        if (e < WZ * N2 && ky0 + cc < M2) {
+          const cplx<T>* f = fcur[j];
+          cplx<T>* s3 = spec + kz * WZ + cc;
+          const cplx<T> jx = s3[0], jy = s3[N2 * WZ], jz = s3[2 * N2 * WZ];
+          s3[0] = f[0] * jx + f[1] * jy + f[2] * jz;
+          s3[N2 * WZ] = f[1] * jx + f[3] * jy + f[4] * jz;
+          s3[2 * N2 * WZ] = f[2] * jx + f[4] * jy + f[5] * jz;
+        }
+      }
+      __syncthreads();
+      if (act && sz < RZ1) {
+        cplx<T> vz[RZ2];
+#pragma unroll
+        for (int n2 = 0; n2 < RZ2; ++n2) vz[n2] = spec[(c * N2 + sz + RZ1 * n2) * WZ + col];
+        fft2_step1<T, N2, true>(vz, sz, twz);
+#pragma unroll
+        for (int k2 = 0; k2 < RZ2; ++k2) xch[(c * N2 + sz * RZ2 + k2) * WZ + col] = vz[k2];
+      }
+      __syncthreads();
+      if (act && sz < RZ2) {
+        cplx<T> u[RZ1];
+#pragma unroll
+        for (int n1 = 0; n1 < RZ1; ++n1) u[n1] = xch[(c * N2 + n1 * RZ2 + sz) * WZ + col];
+        fft2_step2<T, N2, true>(u);
+#pragma unroll
+        for (int k1 = 0; k1 < RZ1; ++k1) {
+          const int k = sz + RZ2 * k1;
+          if (kyv && k < N) sp[ky * NLP + c * N + k] = u[k1];
+        }
+      }
+      __syncthreads();
+    }
+  }
+  // 4. inverse y FFT, step 1 over all 2M spectral values
+#pragma unroll
+  for (int i = 0; i < C::IY1; ++i) {
+    const int it = t + i * TH;
+    if (it < NL * RY1) {
+      const int line = it % NL, n1 = it / NL;
+#pragma unroll
+      for (int n2 = 0; n2 < RY2; ++n2) v[i][n2] = sp[(n1 + RY1 * n2) * NLP + line];
+      fft2_step1<T, M2, true>(v[i], n1, twy);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < C::IY1; ++i) {
+    const int it = t + i * TH;
+    if (it < NL * RY1) {
+      const int line = it % NL, n1 = it / NL;
+#pragma unroll
+      for (int k2 = 0; k2 < RY2; ++k2) sp[(n1 * RY2 + k2) * NLP + line] = v[i][k2];
+    }
+  }
+  __syncthreads();
+  // 5. step 2, keep y < M (in place)
+#pragma unroll
+  for (int i = 0; i < C::IY2; ++i) {
+    const int it = t + i * TH;
+    if (it < NL * RY2) {
+      const int line = it % NL, k2 = it / NL;
+      cplx<T> u[RY1];
+#pragma unroll
+      for (int n1 = 0; n1 < RY1; ++n1) u[n1] = sp[(n1 * RY2 + k2) * NLP + line];
+      fft2_step2<T, M2, true>(u);
+#pragma unroll
+      for (int k1 = 0; k1 < RY1; ++k1) {
+        const int y = k2 + RY2 * k1;
+        if (y < M) sp[y * NLP + line] = u[k1];
+      }
+    }
+  }
+  __syncthreads();
+  // 6. LDS -> slab
+  for (int e = t; e < NL * M; e += TH) {
+    const int line = e / M, y = e - line * M;
+    g[e] = sp[y * NLP + line];
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
 // operator assembly kernels (FP64)
 // ------------------------------------------------------------------------------------------------
 __device__ __forceinline__ void green6(double dx, double dy, double dz, double ko, c128* out) {
@@ -585,7 +982,7 @@ static int col_tile(int n, int64_t fast, int64_t nbatch, size_t eb) {
   return W;
 }
 
-template <typename T, bool INV, class IO>
+template <typename T, bool INV, class IO, bool TIN = false, bool TOUT = false>
 static void run_rows(vs_ctx* ctx, const IO& io, int n, int64_t nlines, int n_in, int n_out, int nrhs) {
   const FftPlanHost& ph = ctx->plan(n);
   const FftDev<T> pd = ph.dev<T>();
@@ -594,8 +991,8 @@ static void run_rows(vs_ctx* ctx, const IO& io, int n, int64_t nlines, int n_in,
   case NN: {                                                                                             \
     using Cf = RowCfg<T, NN>;                                                                            \
     dim3 grid((unsigned)((nlines + Cf::LPW - 1) / Cf::LPW), (unsigned)nrhs);                            \
-    hipLaunchKernelGGL((k_row2<T, NN, INV, IO>), grid, dim3(Cf::THREADS), 0, ctx->stream, pd.tw, io,   \
-                       nlines, n_in, n_out);                                                             \
+    hipLaunchKernelGGL((k_row2<T, NN, INV, IO, TIN, TOUT>), grid, dim3(Cf::THREADS), 0, ctx->stream,    \
+                       pd.tw, io, nlines, n_in, n_out);                                                  \
     VS_HIP(hipGetLastError());                                                                           \
     return;                                                                                              \
   }
@@ -603,6 +1000,7 @@ static void run_rows(vs_ctx* ctx, const IO& io, int n, int64_t nlines, int n_in,
 #undef VS_ROW_CASE
     default: break;
   }
+  VS_CHECK(!TIN && !TOUT, VS_ERR_INTERNAL, "line-transposed row FFT needs a compiled length");
   VS_CHECK(n <= tile_budget(sizeof(cplx<T>)), VS_ERR_UNSUPPORTED, "FFT length exceeds the LDS tile");
   const int nl = row_tile(n, nlines, sizeof(cplx<T>));
   const size_t lds = sizeof(cplx<T>) * (size_t)(nl + 1) * n;
@@ -610,6 +1008,15 @@ static void run_rows(vs_ctx* ctx, const IO& io, int n, int64_t nlines, int n_in,
   hipLaunchKernelGGL((k_row_fft<T, IO>), grid, dim3(FFT_THREADS), lds, ctx->stream, pd, io, nlines, nl, n_in,
                      n_out, INV ? 1 : 0);
   VS_HIP(hipGetLastError());
+}
+
+static bool fft2_length(int n) {
+  switch (n) {
+#define VS_LEN_CASE(NN) case NN: return true;
+    VS_FFT2_SIZES(VS_LEN_CASE)
+#undef VS_LEN_CASE
+    default: return false;
+  }
 }
 
 template <typename T, bool INV>
@@ -724,6 +1131,68 @@ static bool launch_fz2(vs_ctx* ctx, cplx<T>* S, const TensorC<T>& fN, int L2, in
   }
 }
 
+// the fused slab kernel for (2M, 2N), if compiled and within the LDS limit (and 2L has a compiled x FFT)
+template <typename T>
+static size_t yz_bytes(int M2, int N2) {
+  switch (M2 * 4096 + N2) {
+#define VS_YZ_BYTES(A, B) case A * 4096 + B: return YzCfg<T, A, B>::BYTES;
+    VS_YZ_SHAPES(VS_YZ_BYTES)
+#undef VS_YZ_BYTES
+    default: return 0;
+  }
+}
+
+template <typename T>
+bool yz_fused_available(int L, int M, int N) {
+  const size_t b = yz_bytes<T>(2 * M, 2 * N);
+  return b > 0 && b <= (size_t)VS_LDS_MAX && fft2_length(2 * L);
+}
+template bool yz_fused_available<float>(int, int, int);
+template bool yz_fused_available<double>(int, int, int);
+
+static bool fused_enabled() {
+  const char* e = std::getenv("VS_FUSED");
+  return !(e && e[0] == '0');
+}
+
+// raise the dynamic-LDS limit of the fused kernel for (2M, 2N) above the 64 KB default (done when the
+// operator's slab tensor is built, never inside a stream capture)
+template <typename T>
+static void yz_set_attr(int M2, int N2) {
+  switch (M2 * 4096 + N2) {
+#define VS_YZ_ATTR(A, B)                                                                                     \
+  case A * 4096 + B:                                                                                         \
+    VS_HIP(hipFuncSetAttribute((const void*)k_yz<T, A, B>, hipFuncAttributeMaxDynamicSharedMemorySize,       \
+                               (int)YzCfg<T, A, B>::BYTES));                                                 \
+    return;
+    VS_YZ_SHAPES(VS_YZ_ATTR)
+#undef VS_YZ_ATTR
+    default: return;
+  }
+}
+
+template <typename T>
+static void launch_yz(vs_ctx* ctx, cplx<T>* S, const TensorS<T>& fN, int L, int M, int N, int nrhs, int64_t S_rhs) {
+  const int M2 = 2 * M, N2 = 2 * N;
+  const cplx<T>* twy = ctx->plan(M2).dev<T>().tw;
+  const cplx<T>* twz = ctx->plan(N2).dev<T>().tw;
+  const int per_xcd = (2 * L + 7) / 8;
+  dim3 grid((unsigned)(per_xcd * 8), (unsigned)nrhs);
+  switch (M2 * 4096 + N2) {
+#define VS_YZ_CASE(A, B)                                                                                   \
+  case A * 4096 + B: {                                                                                     \
+    using Cf = YzCfg<T, A, B>;                                                                             \
+    hipLaunchKernelGGL((k_yz<T, A, B>), grid, dim3(Cf::THREADS), Cf::BYTES, ctx->stream, twy, twz, S, fN,  \
+                       S_rhs);                                                                             \
+    VS_HIP(hipGetLastError());                                                                             \
+    return;                                                                                                \
+  }
+    VS_YZ_SHAPES(VS_YZ_CASE)
+#undef VS_YZ_CASE
+    default: throw Error{VS_ERR_INTERNAL, "no fused slab kernel for this shape"};
+  }
+}
+
 template <typename T>
 static void mvp_impl(vs_op* op, const cplx<T>* x, cplx<T>* y, int transp, int nrhs) {
   vs_ctx* ctx = op->ctx;
@@ -742,6 +1211,19 @@ static void mvp_impl(vs_op* op, const cplx<T>* x, cplx<T>* y, int transp, int nr
   const int* map3 = op->map3.as<int>();
   VS_CHECK(map3 != nullptr, VS_ERR_ARG, "operator domain not set (vs_op_set_domain)");
   const int64_t lines = (int64_t)M * N * 3;
+  const cplx<T>* fsl = (sizeof(T) == 8) ? op->fs64.as<cplx<T>>() : op->fs32.as<cplx<T>>();
+  if (fsl != nullptr && fused_enabled()) {
+    // slab-major volume 2L x (3MN): x pass, fused y-z slab pass, inverse x pass
+    const int64_t SL = lines, Sr = (int64_t)L2 * SL;
+    IoMvFx<T> fx0{x, map3, op->nunk, op->V, L, M, transp, coef_dev<T>(op->b), S, Sr};
+    run_rows<T, false, IoMvFxSlab<T>, false, true>(ctx, IoMvFxSlab<T>{fx0, SL}, L2, lines, L, L2, nrhs);
+    TensorS<T> ts{fsl, L, M, N};
+    launch_yz<T>(ctx, S, ts, L, M, N, nrhs, Sr);
+    IoMvIx<T> ix0{x, y, map3, op->nunk, op->V, L, M, transp, coef_dev<T>(op->a), coef_dev<T>(op->at),
+                  coef_dev<T>(op->b), S, Sr, (T)(1.0 / (double)op->P)};
+    run_rows<T, true, IoMvIxSlab<T>, true, false>(ctx, IoMvIxSlab<T>{ix0, SL}, L2, lines, L2, L, nrhs);
+    return;
+  }
   // 1. gather + x FFT
   IoMvFx<T> fx{x, map3, op->nunk, op->V, L, M, transp, coef_dev<T>(op->b), S, S_rhs};
   run_rows<T, false>(ctx, fx, L2, lines, L, L2, nrhs);
@@ -823,6 +1305,21 @@ void op_compress_tensor(vs_op* op) {
     hipLaunchKernelGGL((k_compress_tensor<float>), dim3(g), dim3(256), 0, ctx->stream, op->fN64.as<c128>(),
                        op->fc32.as<c64>(), op->L, op->M, op->N);
     VS_HIP(hipGetLastError());
+  }
+  // slab-major copies for the fused matvec, in the precisions whose shape has one
+  if ((op->precisions & VS_FP64) && yz_fused_available<double>(op->L, op->M, op->N)) {
+    op->fs64.ensure(sizeof(c128) * tot);
+    hipLaunchKernelGGL((k_compress_tensor_slab<double>), dim3(g), dim3(256), 0, ctx->stream, op->fN64.as<c128>(),
+                       op->fs64.as<c128>(), op->L, op->M, op->N);
+    VS_HIP(hipGetLastError());
+    yz_set_attr<double>(2 * op->M, 2 * op->N);
+  }
+  if ((op->precisions & VS_FP32) && yz_fused_available<float>(op->L, op->M, op->N)) {
+    op->fs32.ensure(sizeof(c64) * tot);
+    hipLaunchKernelGGL((k_compress_tensor_slab<float>), dim3(g), dim3(256), 0, ctx->stream, op->fN64.as<c128>(),
+                       op->fs32.as<c64>(), op->L, op->M, op->N);
+    VS_HIP(hipGetLastError());
+    yz_set_attr<float>(2 * op->M, 2 * op->N);
   }
 }
 

@@ -72,18 +72,19 @@ __device__ __forceinline__ int tcomp(int ca, int cb) {
   return lo == 0 ? hi : (lo == 1 ? 2 + hi : 5);
 }
 
-// 2-level: F = y-FFT'd Chan volume (L x M x N x 6, comp 2 not negated); block (kx,ky) at kx + L*ky
-__global__ void k_assemble2(const c128* __restrict__ F, int L, int M, int N, c128 a, const c128* __restrict__ w,
-                            c128 ws, c128* __restrict__ out) {
+// 2-level: F = y-FFT'd Chan volume (L x M x N x 6, comp 2 not negated); blocks of the points
+// kx < Lb, ky < Mb, block (kx,ky) at kx + Lb*ky (Lb = L, Mb = M for the full set)
+__global__ void k_assemble2(const c128* __restrict__ F, int L, int M, int N, int Lb, int Mb, c128 a,
+                            const c128* __restrict__ w, c128 ws, c128* __restrict__ out) {
   const int n = 3 * N;
   const int64_t nn = (int64_t)n * n;
-  const int64_t nblk = (int64_t)L * M;
+  const int64_t nblk = (int64_t)Lb * Mb;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nblk * nn) return;
   const int64_t blk = t / nn;
   const int e = (int)(t - blk * nn);
   const int ra = e % n, cb_ = e / n;  // row, column (column-major block)
-  const int kx = (int)(blk % L), ky = (int)(blk / L);
+  const int kx = (int)(blk % Lb), ky = (int)(blk / Lb);
   const int ca = ra / N, za = ra % N, cb = cb_ / N, zb = cb_ % N;
   const int p = tcomp(ca, cb);
   const int64_t V = (int64_t)L * M * N;
@@ -330,6 +331,27 @@ __global__ void k_relayout2_sym(const c128* __restrict__ in, cplx<To>* __restric
   out[((int64_t)ky * NT + t) * L + kx] = ccast<To>(in[blk * nn + q]);
 }
 
+// column-major blocks of the Lb x Mb stored grid -> tiled mirrored apply layout: columns kx in tiles of
+// Wt (128 bytes of To), entry t of block (kx, ky) at (((kx / Wt) * Mb + ky) * NT + t) * Wt + kx % Wt, with
+// t = r*(2n - r + 1)/2 + (s - r) over the packed upper triangle (SYM) or t = r*n + s (full).  Every
+// (tile, ky) is one contiguous run of NT rows of 128 bytes.
+template <typename To, bool SYM>
+__global__ void k_relayout2_tiled(const c128* __restrict__ in, cplx<To>* __restrict__ out, int Lb, int Mb, int n,
+                                  int Wt) {
+  const int64_t nn = (int64_t)n * n;
+  const int64_t NT = SYM ? (int64_t)n * (n + 1) / 2 : nn;
+  const int64_t tot = (int64_t)Lb * Mb * nn;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= tot) return;
+  const int64_t blk = e / nn;
+  const int q = (int)(e - blk * nn);
+  const int r = q % n, sidx = q / n;  // column-major source entry (r, s)
+  if (SYM && r > sidx) return;
+  const int kx = (int)(blk % Lb), ky = (int)(blk / Lb);
+  const int64_t t = SYM ? (int64_t)r * (2 * n - r + 1) / 2 + (sidx - r) : (int64_t)r * n + sidx;
+  out[(((int64_t)(kx / Wt) * Mb + ky) * NT + t) * Wt + kx % Wt] = ccast<To>(in[blk * nn + q]);
+}
+
 // column-major blocks -> 1-level apply layout (row-major per block)
 template <typename To>
 __global__ void k_relayout1(const c128* __restrict__ in, cplx<To>* __restrict__ out, int64_t nblk, int64_t n) {
@@ -435,6 +457,123 @@ __global__ __launch_bounds__(256) void k_blockmv2_sym(const cplx<T>* __restrict_
   }
 }
 
+// Mirrored 2-level apply.  The Chan eigenvalues of a component odd in x change sign under kx -> L - kx
+// (c(L-i) = -c(i) for the xy and xz vectors), those of a component odd in y under ky -> M - ky (xy, yz),
+// so the blocks obey B(L-kx, ky) = Dx B(kx, ky) Dx and B(kx, M-ky) = Dy B(kx, ky) Dy with Dx (Dy) = -1
+// on the x (y) unknowns; the same holds for the inverses.  Only kx <= L/2, ky <= M/2 are stored (grid
+// Lb x Mb), and one workgroup applies a tile of W stored blocks to the up to four points each stands for:
+// variant q (bit 0: x mirror, bit 1: y mirror) of right-hand side rc is u[(4 rc + q)][s][w].  The rows of
+// a tile are split over RS sibling workgroups (out of place: U -> U2); siblings get consecutive logical
+// ids, and logical ids are laid out so that consecutive ones land on the same XCD (hardware dispatch
+// sends workgroup i to XCD i mod 8), so the siblings' reads of U share that XCD's L2.
+template <typename T, bool SYM, int RC>
+__global__ __launch_bounds__(512) void k_blockmv2m(const cplx<T>* __restrict__ B, const cplx<T>* __restrict__ U,
+                                                   cplx<T>* __restrict__ U2, int L, int M, int Lb, int Mb, int n,
+                                                   int Wt, int W, int RS, int64_t rhs_stride, int nrhs) {
+  constexpr int NV = 4 * RC;
+  cplx<T>* u = reinterpret_cast<cplx<T>*>(vs_smem);
+  const int tiles = (Lb + Wt - 1) / Wt, subs = Wt / W;
+  const int64_t G = (int64_t)tiles * subs * Mb * RS;
+  const int64_t per_xcd = (G + 7) / 8;
+  const int64_t lid = (int64_t)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+  if (lid >= G) return;
+  const int rsi = (int)(lid % RS);
+  int64_t rest = lid / RS;
+  const int sub = (int)(rest % subs);
+  rest /= subs;
+  const int tile = (int)(rest % tiles), kyb = (int)(rest / tiles);
+  const int kx0 = tile * Wt + sub * W;
+  const int wv = max(0, min(W, Lb - kx0));
+  const int r0 = blockIdx.y * RC;
+  const int nr = min(RC, nrhs - r0);
+  const int N = n / 3;
+  const int64_t sstride = (int64_t)L * M;
+  const bool ymir = kyb != 0 && 2 * kyb != M;
+  const cplx<T>* Ui = U + (int64_t)r0 * rhs_stride;
+  cplx<T>* Uo = U2 + (int64_t)r0 * rhs_stride;
+  // branch-free gather (clamped addresses, sign 0 for dead entries) so that the loads of a batch are
+  // all in flight before the first use
+  const int tot = NV * n * W;
+  for (int e0 = threadIdx.x; e0 < tot; e0 += 4 * blockDim.x) {
+    cplx<T> val[4];
+    T sg[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int e = min(e0 + j * (int)blockDim.x, tot - 1);
+      const int w = e % W, q = e / W;
+      const int s = q % n, v = q / n;
+      const int rc = v >> 2, xm = v & 1, ym = (v >> 1) & 1;
+      const int kx = kx0 + w;
+      const bool live = e0 + j * (int)blockDim.x < tot && w < wv && rc < nr &&
+                        (!xm || (kx != 0 && 2 * kx != L)) && (!ym || ymir);
+      const int gx = live ? (xm ? L - kx : kx) : 0, gy = live ? (ym ? M - kyb : kyb) : 0;
+      const int comp = s / N;
+      val[j] = Ui[live ? rc * rhs_stride + s * sstride + gx + (int64_t)L * gy : 0];
+      sg[j] = live ? (((xm && comp == 0) || (ym && comp == 1)) ? (T)-1 : (T)1) : (T)0;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int e = e0 + j * (int)blockDim.x;
+      if (e < tot) u[e] = sg[j] * val[j];
+    }
+  }
+  __syncthreads();
+  const int64_t NT = SYM ? (int64_t)n * (n + 1) / 2 : (int64_t)n * n;
+  const int nz0 = 2 * N;
+  const cplx<T>* Bk = B + ((int64_t)tile * Mb + kyb) * NT * Wt + sub * W;
+  const int rows = (n + RS - 1) / RS;
+  const int rlo = rsi * rows, rhi = min(n, rlo + rows);
+  for (int e = threadIdx.x; e < (rhi - rlo) * W; e += blockDim.x) {
+    const int w = e % W, r = rlo + e / W;
+    if (w >= wv) continue;
+    cplx<T> acc[NV];
+#pragma unroll
+    for (int c = 0; c < NV; ++c) acc[c] = mk<T>(0, 0);
+    // batches of BU independent block loads in flight per lane
+    constexpr int BU = 8;
+    const int64_t t0 = SYM ? (int64_t)r * (2 * n - r + 1) / 2 : (int64_t)r * n;
+    const bool rz = r >= nz0;
+    for (int s0 = 0; s0 < n; s0 += BU) {
+      cplx<T> b[BU];
+      T sg[BU];
+#pragma unroll
+      for (int j = 0; j < BU; ++j) {
+        const int s = min(s0 + j, n - 1);
+        int64_t t;
+        T g = (T)1;
+        if (SYM) {
+          // s < r: transposed entry (s, r) with the z sign pattern; s >= r: row r of the upper triangle
+          const bool lo = s < r;
+          t = lo ? (int64_t)s * (2 * n - s + 1) / 2 + (r - s) : t0 + (s - r);
+          if (lo && rz != (s >= nz0)) g = (T)-1;
+        } else {
+          t = t0 + s;
+        }
+        b[j] = Bk[t * Wt + w];
+        sg[j] = s0 + j < n ? g : (T)0;
+      }
+#pragma unroll
+      for (int j = 0; j < BU; ++j) {
+        const int s = min(s0 + j, n - 1);
+        const cplx<T> bj = sg[j] * b[j];
+#pragma unroll
+        for (int c = 0; c < NV; ++c) acc[c] = acc[c] + bj * u[(c * n + s) * W + w];
+      }
+    }
+    const int kx = kx0 + w;
+    const int comp = r / N;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      const int rc = v >> 2, xm = v & 1, ym = (v >> 1) & 1;
+      const bool live = rc < nr && (!xm || (kx != 0 && 2 * kx != L)) && (!ym || ymir);
+      if (!live) continue;
+      const int gx = xm ? L - kx : kx, gy = ym ? M - kyb : kyb;
+      const bool neg = (xm && comp == 0) || (ym && comp == 1);
+      Uo[rc * rhs_stride + r * sstride + gx + (int64_t)L * gy] = neg ? -acc[v] : acc[v];
+    }
+  }
+}
+
 // per kx: U2(:, kx) = Binv(kx) U(:, kx), one wavefront per row
 template <typename T>
 __global__ __launch_bounds__(256) void k_blockmv1(const cplx<T>* __restrict__ B, const cplx<T>* __restrict__ U,
@@ -469,6 +608,9 @@ __global__ __launch_bounds__(256) void k_blockmv1(const cplx<T>* __restrict__ B,
 // ------------------------------------------------------------------------------------------------
 static unsigned nblocks(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 
+// column tile of the mirrored block layout: 128 bytes of complex values
+template <typename T> constexpr int mirror_tile() { return 128 / (int)sizeof(cplx<T>); }
+
 // Toep (device, FP64) -> circ_L (device): Chan in x + FFT along x
 void build_circ_L(vs_ctx* ctx, const c128* toep, c128* cL, int L, int M, int N) {
   const int64_t V = (int64_t)L * M * N;
@@ -485,9 +627,10 @@ void build_circ_M(vs_ctx* ctx, const c128* cL, c128* F, int L, int M, int N) {
   launch_col_fft(ctx, VS_FP64, F, L, L, (int64_t)N * 6, (int64_t)L * M, M, M, M, false, 1.0, 1, 0);
 }
 
-void assemble2(vs_ctx* ctx, const c128* F, int L, int M, int N, c128 a, const c128* w, c128 ws, c128* out) {
-  const int64_t tot = (int64_t)L * M * 9 * N * N;
-  hipLaunchKernelGGL(k_assemble2, dim3(nblocks(tot)), dim3(256), 0, ctx->stream, F, L, M, N, a, w, ws, out);
+void assemble2(vs_ctx* ctx, const c128* F, int L, int M, int N, int Lb, int Mb, c128 a, const c128* w, c128 ws,
+               c128* out) {
+  const int64_t tot = (int64_t)Lb * Mb * 9 * N * N;
+  hipLaunchKernelGGL(k_assemble2, dim3(nblocks(tot)), dim3(256), 0, ctx->stream, F, L, M, N, Lb, Mb, a, w, ws, out);
   VS_HIP(hipGetLastError());
 }
 
@@ -522,18 +665,34 @@ void invert_blocks_dev(vs_ctx* ctx, c128* blocks, int64_t nblk, int64_t n) {
 void pc_store_inverse(vs_pc* pc, const c128* inv) {
   vs_ctx* ctx = pc->ctx;
   const int64_t n = pc->nb;
-  const int64_t nblk = pc->levels == 2 ? (int64_t)pc->L * pc->M : pc->L;
+  const int64_t nblk = pc->levels == 2 ? (int64_t)pc->Lb * pc->Mb : pc->L;
   const int64_t tot = nblk * n * n;
   auto go = [&](auto tag, DevBuf& dst) {
     using To = decltype(tag);
+    if (pc->levels == 2 && pc->mirror) {
+      const int Wt = mirror_tile<To>();
+      const int64_t tiles = (pc->Lb + Wt - 1) / Wt;
+      const int64_t NT = pc->sym ? n * (n + 1) / 2 : n * n;
+      const size_t bytes = sizeof(cplx<To>) * (size_t)(tiles * Wt * pc->Mb * NT);
+      dst.ensure(bytes);
+      VS_HIP(hipMemsetAsync(dst.p, 0, bytes, ctx->stream));  // padding columns of the last tile
+      if (pc->sym)
+        hipLaunchKernelGGL((k_relayout2_tiled<To, true>), dim3(nblocks(tot)), dim3(256), 0, ctx->stream, inv,
+                           dst.as<cplx<To>>(), pc->Lb, pc->Mb, (int)n, Wt);
+      else
+        hipLaunchKernelGGL((k_relayout2_tiled<To, false>), dim3(nblocks(tot)), dim3(256), 0, ctx->stream, inv,
+                           dst.as<cplx<To>>(), pc->Lb, pc->Mb, (int)n, Wt);
+      VS_HIP(hipGetLastError());
+      return;
+    }
     const int64_t stored = (pc->levels == 2 && pc->sym) ? nblk * n * (n + 1) / 2 : tot;
     dst.ensure(sizeof(cplx<To>) * stored);
     if (pc->levels == 2 && pc->sym)
       hipLaunchKernelGGL((k_relayout2_sym<To>), dim3(nblocks(tot)), dim3(256), 0, ctx->stream, inv,
-                         dst.as<cplx<To>>(), pc->L, pc->M, (int)n);
+                         dst.as<cplx<To>>(), pc->Lb, pc->Mb, (int)n);
     else if (pc->levels == 2)
       hipLaunchKernelGGL((k_relayout2<To>), dim3(nblocks(tot)), dim3(256), 0, ctx->stream, inv, dst.as<cplx<To>>(),
-                         pc->L, pc->M, (int)n);
+                         pc->Lb, pc->Mb, (int)n);
     else
       hipLaunchKernelGGL((k_relayout1<To>), dim3(nblocks(tot)), dim3(256), 0, ctx->stream, inv, dst.as<cplx<To>>(),
                          nblk, n);
@@ -559,6 +718,43 @@ static void pc_apply_impl(vs_pc* pc, const cplx<T>* x, cplx<T>* y, int nrhs) {
   if (pc->levels == 2) {
     launch_col_fft(ctx, prec, U, L, L, (int64_t)N * 3, (int64_t)L * M, M, M, M, false, 1.0, nrhs, V3);
     const int n = pc->nb;
+    if (pc->mirror) {
+      // right-hand sides per block pass (times the four mirror points): the largest of 4, 2, 1 that keeps
+      // a tile of at least 8 stored blocks within 64 KB of LDS
+      const int Lb = pc->Lb, Mb = pc->Mb;
+      auto fits = [&](int rc, int w) { return (size_t)4 * rc * n * w * sizeof(cplx<T>) <= 65536; };
+      int RC = nrhs >= 4 ? 4 : (nrhs >= 2 ? 2 : 1);
+      while (RC > 1 && !fits(RC, std::min(8, Lb))) RC /= 2;
+      VS_CHECK(fits(RC, 1), VS_ERR_UNSUPPORTED, "2-level block too large for LDS");
+      // workgroups cover W columns of a Wt-wide layout tile (W | Wt); rows are split over sibling workgroups
+      // only for small grids
+      const int Wt = mirror_tile<T>();
+      int W = Wt;
+      while (W > 1 && !fits(RC, W)) W /= 2;
+      const int tiles = (Lb + Wt - 1) / Wt, subs = Wt / W;
+      const int chunks = (nrhs + RC - 1) / RC;
+      const int64_t base = (int64_t)tiles * subs * Mb * chunks;
+      int RS = base >= 512 ? 1 : (int)std::min<int64_t>((512 + base - 1) / base, std::max(1, n / 4));
+      const int rows = (n + RS - 1) / RS;
+      RS = (n + rows - 1) / rows;
+      const size_t lds = sizeof(cplx<T>) * (size_t)4 * RC * n * W;
+      const int threads = (int)std::min<int64_t>(512, ((int64_t)rows * W + 63) / 64 * 64);
+      const int64_t G = (int64_t)tiles * subs * Mb * RS;
+      dim3 grid((unsigned)(((G + 7) / 8) * 8), (unsigned)chunks);
+      pc->U2.ensure(sizeof(cplx<T>) * V3 * nrhs);
+      cplx<T>* U2 = pc->U2.as<cplx<T>>();
+#define VS_BMM(SYMV, RCV)                                                                                 \
+  hipLaunchKernelGGL((k_blockmv2m<T, SYMV, RCV>), grid, dim3(threads), lds, ctx->stream, B, U, U2, L, M, Lb, Mb, n, \
+                     Wt, W, RS, V3, nrhs)
+      if (pc->sym) {
+        if (RC == 4) VS_BMM(true, 4); else if (RC == 2) VS_BMM(true, 2); else VS_BMM(true, 1);
+      } else {
+        if (RC == 4) VS_BMM(false, 4); else if (RC == 2) VS_BMM(false, 2); else VS_BMM(false, 1);
+      }
+#undef VS_BMM
+      VS_HIP(hipGetLastError());
+      U = U2;
+    } else {
     // right-hand sides per block-matrix pass: 1, 4 or 8
     int RC = nrhs >= 8 ? 8 : (nrhs >= 4 ? 4 : 1);
     while (RC > 1 && (size_t)RC * n * sizeof(cplx<T>) > 65536 / 4) RC = RC == 8 ? 4 : 1;
@@ -583,6 +779,7 @@ static void pc_apply_impl(vs_pc* pc, const cplx<T>* x, cplx<T>* y, int nrhs) {
     }
 #undef VS_BMV
     VS_HIP(hipGetLastError());
+    }
     launch_col_fft(ctx, prec, U, L, L, (int64_t)N * 3, (int64_t)L * M, M, M, M, true, 1.0, nrhs, V3);
     launch_row_ifft_scatter(ctx, prec, U, map3, pc->nunk, y, L, lines, 1.0 / ((double)L * M), nrhs, V3);
   } else {
