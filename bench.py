@@ -33,7 +33,49 @@ def _parse():
     ap.add_argument("--inner-tol", type=float, default=1e-3)
     ap.add_argument("--inner", default="bicgstab", choices=["bicgstab", "gmres"])
     ap.add_argument("--sweep", action="store_true", help="also time other inner solvers/tolerances (stderr)")
+    ap.add_argument("--matvec-only", action="store_true",
+                    help="only run 20 FP32 and 20 FP64 matvecs, 20 preconditioner applies of each precision and "
+                         "5 device-to-device copies of 256 MiB (a fixed launch sequence for hardware-counter runs)")
     return ap.parse_args()
+
+
+COPY_BYTES = 256 << 20
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                            "matvec_traffic_koch128x128x16.txt")
+
+
+def measured_traffic():
+    """HBM bytes per FP32 matvec from the committed hardware-counter summary (FETCH_SIZE + WRITE_SIZE of
+    the matvec kernels, scaled by the same counters on a device copy of known size), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("fp32_matvec_hbm_bytes:"):
+                    return int(line.split(":")[1])
+    except OSError:
+        return None
+    return None
+
+
+def matvec_only(op, pc, ctx, api, VS_FP32, VS_FP64):
+    n = op.n
+    rng = np.random.default_rng(0)
+    xr = rng.normal(size=n) + 1j * rng.normal(size=n)
+    for prec, dt in ((VS_FP32, np.complex64), (VS_FP64, np.complex128)):
+        xv = api.DeviceArray.from_numpy(xr, dt, ctx)
+        yv = api.DeviceArray(n, dt, ctx)
+        for _ in range(20):
+            op.mvp_dev(xv.ptr, yv.ptr, False, prec, 1)
+        for _ in range(20):
+            pc.apply_dev(xv.ptr, yv.ptr, prec, 1)
+        ctx.synchronize()
+        print(f"[bench] matvec-only: 20 matvecs + 20 pc applies, precision {prec}", file=sys.stderr, flush=True)
+    a = api.DeviceArray(COPY_BYTES // 8, np.complex64, ctx)
+    c = api.DeviceArray(COPY_BYTES // 8, np.complex64, ctx)
+    for _ in range(5):
+        api.check(ctx.lib.vs_dev_copy(ctx.h, api.C.c_void_p(c.ptr), api.C.c_void_p(a.ptr), COPY_BYTES, 2))
+    print(json.dumps({"matvec_only": True, "matvecs_per_precision": 20, "pc_applies_per_precision": 20,
+                      "copy_bytes": COPY_BYTES, "copies": 5}), flush=True)
 
 
 def cpu_baseline(scat, sysd, gpu_iters: float, n_iters: int):
@@ -95,6 +137,9 @@ def main():
     pc = sysd.preconditioner(2, precisions=VS_FP64 | VS_FP32)
     op = sysd.op
     n = op.n
+    if args.matvec_only:
+        matvec_only(op, pc, ctx, api, VS_FP32, VS_FP64)
+        return
     b = api.DeviceArray.from_numpy(sysd.b, np.complex128, ctx)
     x = api.DeviceArray(n, np.complex128, ctx)
 
@@ -126,14 +171,15 @@ def main():
     x_ref = x.numpy()
     assert flag == 0 and relres <= TOL, (flag, relres)
 
-    # plain FP64 solve of the same system (timed beside, one warm run + one timed)
+    # plain FP64 solve of the same system (timed beside: one warm run, then as many timed runs as steps)
     x64 = api.DeviceArray(n, np.complex128, ctx)
     api.bicgstab_dev(op, b.ptr, x64.ptr, TOL, 2000, pc)
     ctx.synchronize()
     t0 = time.perf_counter()
-    f64, rr64, it64, nres64 = api.bicgstab_dev(op, b.ptr, x64.ptr, TOL, 2000, pc)
+    for _ in range(args.steps):
+        f64, rr64, it64, nres64 = api.bicgstab_dev(op, b.ptr, x64.ptr, TOL, 2000, pc)
     ctx.synchronize()
-    t_64 = time.perf_counter() - t0
+    t_64 = (time.perf_counter() - t0) / args.steps
     print(f"[bench] fp64: flag={f64} iter={it64} {t_64 * 1e3:.1f} ms", file=sys.stderr, flush=True)
     # plain FP64 GMRES(50) of the same system
     xg = api.DeviceArray(n, np.complex128, ctx)
@@ -156,20 +202,29 @@ def main():
     mv = {}
     rng = np.random.default_rng(0)
     xr = rng.normal(size=n) + 1j * rng.normal(size=n)
+    # algorithmic bytes per matvec: s_c (6 P + 12 V) -- the padded 3-component spectrum written once and
+    # read once (6 P), J read and y written (6 V), and the tensor read once; the tensor is exploited through
+    # its octant parity, so its term is the octant, 6 (L+1)(M+1)(N+1) ~ 6 V, not 6 P.
+    # Preconditioner apply: the stored blocks (mirror- and transpose-symmetric: (L/2+1)(M/2+1) packed
+    # triangles of n(n+1)/2 entries, n = 3N) plus 18 V of vector traffic in the FFT stages.
+    nb = 3 * N
+    blk_stored = (L // 2 + 1) * (M // 2 + 1) * nb * (nb + 1) // 2
     for prec, dt, sc in ((VS_FP32, np.complex64, 8), (VS_FP64, np.complex128, 16)):
         xv = api.DeviceArray.from_numpy(xr, dt, ctx)
         yv = api.DeviceArray(n, dt, ctx)
         ms = api.bench_mvp(op, xv.ptr, yv.ptr, False, prec, 1, 50)
         ms_pc = api.bench_pc(pc, xv.ptr, yv.ptr, prec, 1, 50)
-        B_mv = sc * (12 * P + 6 * V)
-        B_pc = sc * (9 * L * M * N * N + 18 * V)
+        B_mv = sc * (6 * P + 12 * V)
+        B_pc = sc * (blk_stored + 18 * V)
         mv[prec] = dict(ms=ms, gbs=B_mv / (ms * 1e-3) / 1e9, bytes=B_mv, pc_ms=ms_pc,
                         pc_gbs=B_pc / (ms_pc * 1e-3) / 1e9)
 
     m32 = mv[VS_FP32]
+    traffic = measured_traffic()
     roof = {"bound": "hbm", "achieved": round(m32["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(m32["gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "FP32 FFT matvec (5 kernels: mv_fx, col FFT y, mv_fzmul, col IFFT y, mv_ix)",
+            "frac": round(m32["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "kernel": "FP32 FFT matvec, one vs_mvp_dev call = 3 kernels (x FFT + gather, fused y-z slab, "
+                      "inverse x FFT + diagonal + scatter)",
             "bytes_per_launch": m32["bytes"], "ms_per_launch": round(m32["ms"], 5)}
 
     cpu = None
@@ -209,7 +264,9 @@ def main():
         "matvec_fp64_ms": round(mv[VS_FP64]["ms"], 5),
         "matvec_fp64_gbs": round(mv[VS_FP64]["gbs"], 1),
         "pc_apply_fp32_ms": round(m32["pc_ms"], 5),
+        "pc_apply_fp32_gbs": round(m32["pc_gbs"], 1),
         "pc_apply_fp64_ms": round(mv[VS_FP64]["pc_ms"], 5),
+        "pc_apply_fp64_gbs": round(mv[VS_FP64]["pc_gbs"], 1),
         "roofline": roof,
         "cpu_baseline": cpu,
     }
