@@ -223,8 +223,8 @@ def main():
     traffic = measured_traffic()
     roof = {"bound": "hbm", "achieved": round(m32["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(m32["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "kernel": "FP32 FFT matvec, one vs_mvp_dev call = 3 kernels (x FFT + gather, fused y-z slab, "
-                      "inverse x FFT + diagonal + scatter)",
+            "kernel": f"FP32 FFT matvec, one vs_mvp_dev call (variant '{op.matvec_variant(VS_FP32)}': "
+                      "x FFT + gather, fused y-z slab kernel, inverse x FFT + diagonal + scatter)",
             "bytes_per_launch": m32["bytes"], "ms_per_launch": round(m32["ms"], 5)}
 
     cpu = None
@@ -263,6 +263,8 @@ def main():
         "matvec_fp32_gbs": round(m32["gbs"], 1),
         "matvec_fp64_ms": round(mv[VS_FP64]["ms"], 5),
         "matvec_fp64_gbs": round(mv[VS_FP64]["gbs"], 1),
+        "matvec_variant_fp32": op.matvec_variant(VS_FP32),
+        "matvec_variant_fp64": op.matvec_variant(VS_FP64),
         "pc_apply_fp32_ms": round(m32["pc_ms"], 5),
         "pc_apply_fp32_gbs": round(m32["pc_gbs"], 1),
         "pc_apply_fp64_ms": round(mv[VS_FP64]["pc_ms"], 5),

@@ -85,6 +85,10 @@ int vs_op_get_fN(vs_op* op, double* fN);         /* 2L*2M*2N*6 complex128 */
 int vs_op_set_domain(vs_op* op, const int64_t* idx, int64_t nidx, const double* Gram, int64_t gram_len,
                      const double* Mr, int64_t mr_len, const double* Mc, int64_t mc_len);
 int64_t vs_op_nunknowns(const vs_op* op);
+/* The matvec variant used in a precision (VS_FP64 / VS_FP32): 0 five FFT passes, 1 fused padded-slab
+ * kernel, 2 fused parity-split slab kernel.  Picked by timing at the operator's first domain (no
+ * reference counterpart: the reference has one FFT path). */
+int vs_op_matvec_variant(const vs_op* op, int precision, int* variant);
 
 /* ---------------------------------------------------------------------------------------------- */
 /* matvec (replaces MVP.m:1-145 / mv_AN_const.m, both transp_flag branches)                        */

@@ -123,11 +123,12 @@ def test_mvp_full_size_linearity_and_adjoint(ctx):
     assert abs(np.vdot(y, Ax) - np.vdot(AHy, x)) <= 1e-12 * abs(np.vdot(y, Ax))
 
 
-@pytest.mark.parametrize("shape", [(24, 24, 24), (32, 32, 32), (64, 64, 16), (32, 32, 8)])
+@pytest.mark.parametrize("shape", [(24, 24, 24), (32, 32, 32), (64, 64, 16), (32, 32, 8), (5, 16, 4)])
 @pytest.mark.parametrize("transp", ["notransp", "transp"])
 def test_fused_slab_mvp_matches_oracle_and_unfused(ctx, shape, transp, monkeypatch):
-    """Grids whose (2M, 2N) has a fused y-z slab kernel: FP64 fused matvec against the oracle, and fused
-    against the five-pass path in FP64 and FP32."""
+    """Grids whose shape has fused slab kernels: every matvec variant (five passes, padded slab, parity
+    split; VS_MATVEC pins one, an unavailable one falls back to the autotuned choice) against the oracle
+    in FP64, and in FP32 against the oracle and the five-pass FP32 result."""
     print(f"[fused mvp] shape={shape} {transp}", flush=True)
     k = 2 * math.pi
     r, vox, idx3, x = _setup(shape, seed=6)
@@ -142,21 +143,22 @@ def test_fused_slab_mvp_matches_oracle_and_unfused(ctx, shape, transp, monkeypat
     op = api.Operator.dda(r, k, precisions=capi.VS_FP64 | capi.VS_FP32, ctx=ctx)
     op.set_domain(idx3, 1 / alpha, 1.0, dV * chi.reshape(-1, order="F"))
     out = {}
-    for fused in ("1", "0"):
-        monkeypatch.setenv("VS_FUSED", fused)
-        out[("fp64", fused)] = op.mvp(x, transp=transp == "transp")
+    for variant in ("pass5", "slab", "split"):
+        monkeypatch.setenv("VS_MATVEC", variant)
+        out[("fp64", variant)] = op.mvp(x, transp=transp == "transp")
         xt = api.DeviceArray.from_numpy(x[None], np.complex64, ctx)
         yt = api.DeviceArray((1, x.size), np.complex64, ctx)
         op.mvp_dev(xt.ptr, yt.ptr, transp == "transp", capi.VS_FP32, 1)
-        out[("fp32", fused)] = yt.numpy()[0].astype(np.complex128)
-    assert _rel(out[("fp64", "1")], yref) <= 1e-12
-    assert _rel(out[("fp64", "0")], yref) <= 1e-12
-    assert _rel(out[("fp32", "1")], yref) <= 2e-5
-    assert _rel(out[("fp32", "1")], out[("fp32", "0")]) <= 2e-5
+        out[("fp32", variant)] = yt.numpy()[0].astype(np.complex128)
+    for variant in ("pass5", "slab", "split"):
+        assert _rel(out[("fp64", variant)], yref) <= 1e-12, variant
+        assert _rel(out[("fp32", variant)], yref) <= 2e-5, variant
+        assert _rel(out[("fp32", variant)], out[("fp32", "pass5")]) <= 2e-5, variant
 
 
 def test_fused_slab_mvp_koch_full_size_fp32(ctx, monkeypatch):
-    """128x128x16 Koch prism in FP32 (fused y-z slab kernel for (256, 32)) against the FP64 matvec."""
+    """128x128x16 Koch prism in FP32 (fused slab kernels for this shape) against the FP64 matvec, for
+    every variant."""
     s = G.koch_prism()
     from voxscatter_amd.dda import DDASystem
     sysd = DDASystem.build(s, precisions=capi.VS_FP64 | capi.VS_FP32, ctx=ctx)
@@ -168,9 +170,9 @@ def test_fused_slab_mvp_koch_full_size_fp32(ctx, monkeypatch):
     xt = api.DeviceArray.from_numpy(X, np.complex64, ctx)
     yt = api.DeviceArray(X.shape, np.complex64, ctx)
     res = {}
-    for fused in ("1", "0"):
-        monkeypatch.setenv("VS_FUSED", fused)
+    for variant in ("pass5", "slab", "split"):
+        monkeypatch.setenv("VS_MATVEC", variant)
         op.mvp_dev(xt.ptr, yt.ptr, False, capi.VS_FP32, 2)
-        res[fused] = yt.numpy().astype(np.complex128)
-        assert _rel(res[fused], yref) <= 2e-5
-    assert _rel(res["1"], res["0"]) <= 2e-5
+        res[variant] = yt.numpy().astype(np.complex128)
+        assert _rel(res[variant], yref) <= 2e-5, variant
+        assert _rel(res[variant], res["pass5"]) <= 2e-5, variant

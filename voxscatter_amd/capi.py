@@ -58,6 +58,7 @@ SIGNATURES = {
     "vs_op_get_fN": (INT, [P, D]),
     "vs_op_set_domain": (INT, [P, I64, C.c_int64, D, C.c_int64, D, C.c_int64, D, C.c_int64]),
     "vs_op_nunknowns": (C.c_int64, [P]),
+    "vs_op_matvec_variant": (C.c_int, [P, C.c_int, C.POINTER(C.c_int)]),
     "vs_mvp": (INT, [P, D, D, INT]),
     "vs_mvp_dev": (INT, [P, P, P, INT, INT, INT]),
     "vs_circ_1_level": (INT, [P, INT, INT, INT, D, D, D]),

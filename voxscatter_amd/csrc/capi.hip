@@ -12,6 +12,8 @@ void op_build_fN(vs_op* op);
 void op_compress_tensor(vs_op* op);
 void build_circ_L(vs_ctx* ctx, const c128* toep, c128* cL, int L, int M, int N);
 void build_circ_M(vs_ctx* ctx, const c128* cL, c128* F, int L, int M, int N);
+template <typename T>
+int matvec_variant(const vs_op* op);
 void assemble2(vs_ctx* ctx, const c128* F, int L, int M, int N, int Lb, int Mb, c128 a, const c128* w, c128 ws,
                c128* out);
 void assemble1(vs_ctx* ctx, const c128* cL, int L, int M, int N, c128 a, const c128* w, c128 ws, c128* out);
@@ -319,11 +321,20 @@ int vs_op_set_domain(vs_op* op, const int64_t* idx, int64_t nidx, const double* 
       for (int64_t v = 0; v < V; ++v) vb[v] = at(Mc, mc_len, v);
       set_coef(op, op->b, false, mk<double>(0, 0), &vb);
     }
+    op_autotune(op);  // first domain: pick the fastest matvec variant per precision
     VS_HIP(hipStreamSynchronize(op->ctx->stream));
   });
 }
 
 int64_t vs_op_nunknowns(const vs_op* op) { return op ? op->nunk : -1; }
+
+int vs_op_matvec_variant(const vs_op* op, int precision, int* variant) {
+  return guarded([&] {
+    VS_CHECK(op && variant, VS_ERR_ARG, "null pointer");
+    VS_CHECK(precision == VS_FP64 || precision == VS_FP32, VS_ERR_ARG, "precision must be VS_FP64 or VS_FP32");
+    *variant = precision == VS_FP64 ? matvec_variant<double>(op) : matvec_variant<float>(op);
+  });
+}
 
 int vs_mvp(vs_op* op, const double* x, double* y, int transp) {
   return guarded([&] {

@@ -32,6 +32,7 @@ void launch_col_fft(vs_ctx* ctx, int prec, void* data, int64_t fast, int64_t axi
                     int64_t rhs_stride);
 void launch_row_fft_dense(vs_ctx* ctx, int prec, void* data, int64_t nlines, int n_in, int n, int n_out,
                           int64_t ld_in, int64_t ld_out, bool inv, double scale, int nrhs, int64_t rhs_stride);
+void op_autotune(vs_op* op);
 // compressed vector -> dense L x (lines) volume with an x FFT of length L (forward)
 void launch_row_fft_gather(vs_ctx* ctx, int prec, const void* x, const int* map3, int64_t nunk, void* out,
                            int L, int64_t nlines, int nrhs, int64_t out_rhs_stride);
@@ -53,6 +54,9 @@ struct vs_op {
   // other octants follow from the parity of each component (fN(2L-kx) = s_x fN(kx), etc.)
   vs::DevBuf fc64, fc32;
   vs::DevBuf fs64, fs32;  // the octant stored slab-major, for the fused slab matvec (when its shape has one)
+  vs::DevBuf fq64, fq32;  // the octant split by (iy, iz) parity, for the parity-split slab matvec
+  int mv_variant[2] = {-1, -1};  // matvec variant per precision (FP64, FP32) picked by the autotuner
+  bool tuned = false;
   int64_t Pc = 0;
   // domain
   int64_t nunk = 0;

@@ -297,6 +297,27 @@ struct TensorS {
   }
 };
 
+// The octant split by the parity of iy and iz (which equal the parities of ky and kz, 2M and 2N being
+// even): value (ix, iy, iz) of component p at ((((ix*6 + p)*2 + iz%2)*Hz + iz/2)*2 + iy%2)*Hy + iy/2, with
+// Hy = M/2 + 1, Hz = N/2 + 1.  For one kx slab and one (ky, kz) parity quarter, consecutive ky/2 are
+// consecutive addresses.
+template <typename T>
+struct TensorQ {
+  const cplx<T>* f;
+  int L, M, N, Hy, Hz;
+  __device__ __forceinline__ void load6(int kx, int ky, int kz, cplx<T>* out) const {
+    const bool fx = kx > L, fy = ky > M, fz = kz > N;
+    const int ix = fx ? 2 * L - kx : kx, iy = fy ? 2 * M - ky : ky, iz = fz ? 2 * N - kz : kz;
+    const int64_t ps = (int64_t)2 * Hz * 2 * Hy;  // component stride
+    const cplx<T>* b = f + (int64_t)ix * 6 * ps + ((int64_t)((iz & 1) * Hz + (iz >> 1)) * 2 + (iy & 1)) * Hy + (iy >> 1);
+#pragma unroll
+    for (int p = 0; p < 6; ++p) out[p] = b[p * ps];
+    if (fx != fy) out[1] = -out[1];
+    if (fx != fz) out[2] = -out[2];
+    if (fy != fz) out[4] = -out[4];
+  }
+};
+
 // ------------------------------------------------------------------------------------------------
 // fused z pass: FFT along z, 3x3 tensor product, inverse FFT along z
 // ------------------------------------------------------------------------------------------------
@@ -375,6 +396,27 @@ __global__ void k_compress_tensor_slab(const c128* __restrict__ fN, cplx<To>* __
   const int p = (int)(q % 6);
   const int64_t ix = q / 6;
   fs[t] = ccast<To>(fN[p * P + ix + 2LL * L * (iy + 2LL * M * iz)]);
+}
+
+template <typename To>
+__global__ void k_compress_tensor_q(const c128* __restrict__ fN, cplx<To>* __restrict__ fq, int L, int M, int N,
+                                    int Hy, int Hz) {
+  const int64_t P = 8LL * L * M * N;
+  const int64_t tot = (int64_t)(L + 1) * 6 * 2 * Hz * 2 * Hy;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= tot) return;
+  const int iyh = (int)(t % Hy);
+  int64_t q = t / Hy;
+  const int py = (int)(q % 2);
+  q /= 2;
+  const int izh = (int)(q % Hz);
+  q /= Hz;
+  const int pz = (int)(q % 2);
+  q /= 2;
+  const int p = (int)(q % 6);
+  const int64_t ix = q / 6;
+  const int iy = 2 * iyh + py, iz = 2 * izh + pz;
+  fq[t] = (iy <= M && iz <= N) ? ccast<To>(fN[p * P + ix + 2LL * L * (iy + 2LL * M * iz)]) : mk<To>(0, 0);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -565,7 +607,16 @@ __global__ __launch_bounds__(512) void k_fz2(const cplx<T>* __restrict__ tw, cpl
   const int col = t % W, r = t / W;
   const int c = r % 3, s = r / 3;
   const int tiles = (L2 + W - 1) / W;
-  const int tile = blockIdx.x % tiles, ky = blockIdx.x / tiles;
+  // logical ids: the tiles of ky, then those of its mirror 2M - ky (same tensor rows), ky in the order
+  // 0, M, 1, 2M-1, 2, 2M-2, ...; laid out XCD by XCD (workgroup i runs on XCD i mod 8) so that one XCD's
+  // L2 serves all tiles of a mirror pair
+  const int G = tiles * M2;
+  const int per_xcd = (G + 7) / 8;
+  const int lid = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+  if (lid >= G) return;
+  const int tile = lid % tiles, qy = lid / tiles;
+  const int My = M2 / 2;
+  const int ky = qy == 0 ? 0 : (qy == 1 ? My : ((qy & 1) == 0 ? 1 + (qy - 2) / 2 : M2 - 1 - (qy - 3) / 2));
   const int kx0 = tile * W;
   const bool live = kx0 + col < L2;
   cplx<T>* Sb = S + (int64_t)blockIdx.y * S_rhs;
@@ -868,6 +919,215 @@ __global__ __launch_bounds__(1024) void k_yz(const cplx<T>* __restrict__ twy, co
 }
 
 // ------------------------------------------------------------------------------------------------
+// parity-split slab pass: the zero-padded transforms as pairs of unpadded ones
+// ------------------------------------------------------------------------------------------------
+// With x zero beyond M, the length-2M DFT splits by the parity of the output index:
+//   X(2j)   = DFT_M(x)(j),   X(2j+1) = DFT_M(x . w_2M^y)(j)                      (w_n = exp(-2 pi i/n))
+// and the inverse length-2M DFT cropped to y < M is
+//   y(y)    = IDFT_M(X_even)(y) + conj(w_2M^y) IDFT_M(X_odd)(y)
+// (the same along z with N).  The tensor product is pointwise in (ky, kz), so one workgroup runs the two
+// ky parities of its kx slab one after the other with an unpadded spectrum of M x 3N values (half the
+// padded one), keeping the first half's result in registers; the z stage is one thread per (ky, c) with
+// both kz parities in registers (length-N register butterflies).
+#define VS_YZQ_SHAPES(X) X(128, 16) X(64, 16) X(128, 8) X(64, 8) X(32, 8) X(32, 16) X(16, 8) X(16, 4)
+
+template <typename T, int M, int N>
+struct YzqCfg {
+  static constexpr int NL = 3 * N, NLP = NL + 1;
+  static constexpr int RY1 = Split<M>::R1, RY2 = Split<M>::R2;
+  static constexpr int TH = 512;
+  static constexpr int IY1 = (NL * RY1 + TH - 1) / TH, IY2 = (NL * RY2 + TH - 1) / TH;
+  static constexpr int IS = (NL * M + TH - 1) / TH;  // slab values per thread
+  static constexpr int IZ = (M * N + TH - 1) / TH;   // tensor-product points per thread
+  static constexpr int SP = M * NLP > NL * (M + 1) ? M * NLP : NL * (M + 1);
+  static constexpr size_t BYTES = sizeof(cplx<T>) * (size_t)SP;
+  static_assert(3 * M <= TH, "one thread per (ky, component) in the z stage");
+};
+
+template <typename T, int M, int N>
+__global__ __launch_bounds__(512) void k_yzq(const cplx<T>* __restrict__ twm, const cplx<T>* __restrict__ tw2m,
+                                             const cplx<T>* __restrict__ tw2n, cplx<T>* S, TensorQ<T> fN,
+                                             int64_t S_rhs) {
+  using C = YzqCfg<T, M, N>;
+  constexpr int NL = C::NL, NLP = C::NLP, RY1 = C::RY1, RY2 = C::RY2, TH = C::TH;
+  cplx<T>* sp = reinterpret_cast<cplx<T>*>(vs_smem);
+  const int t = threadIdx.x;
+  const int L = fN.L, L2 = 2 * L;
+  // mirror slabs kx, 2L - kx on consecutive logical ids, laid out XCD by XCD (as in k_yz)
+  const int per_xcd = (L2 + 7) / 8;
+  const int q = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+  if (q >= L2) return;
+  const int kx = q == 0 ? 0 : (q == 1 ? L : ((q & 1) == 0 ? 1 + (q - 2) / 2 : L2 - 1 - (q - 3) / 2));
+  cplx<T>* g = S + (int64_t)blockIdx.y * S_rhs + (int64_t)kx * NL * M;
+  cplx<T> acc[C::IS];  // result of the even-ky half, element t + i TH of the slab
+
+  for (int py = 0; py < 2; ++py) {
+    // 0. slab -> LDS [line][y] (pitch M + 1), twisted by w_2M^y for the odd half
+    {
+      cplx<T> st[C::IS];
+#pragma unroll
+      for (int i = 0; i < C::IS; ++i) {
+        const int e = t + i * TH;
+        st[i] = g[e < NL * M ? e : 0];
+      }
+#pragma unroll
+      for (int i = 0; i < C::IS; ++i) {
+        const int e = t + i * TH;
+        if (e < NL * M) {
+          const int line = e / M, y = e - line * M;
+          sp[line * (M + 1) + y] = py ? st[i] * tw2m[y] : st[i];
+        }
+      }
+    }
+    __syncthreads();
+    // 1. DFT_M along y, step 1 (results in registers across the barrier)
+    cplx<T> v[C::IY1][RY2];
+#pragma unroll
+    for (int i = 0; i < C::IY1; ++i) {
+      const int it = t + i * TH;
+      if (it < NL * RY1) {
+        const int line = it % NL, n1 = it / NL;
+#pragma unroll
+        for (int n2 = 0; n2 < RY2; ++n2) v[i][n2] = sp[line * (M + 1) + n1 + RY1 * n2];
+        fft2_step1<T, M, false>(v[i], n1, twm);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < C::IY1; ++i) {
+      const int it = t + i * TH;
+      if (it < NL * RY1) {
+        const int line = it % NL, n1 = it / NL;
+#pragma unroll
+        for (int k2 = 0; k2 < RY2; ++k2) sp[(n1 * RY2 + k2) * NLP + line] = v[i][k2];
+      }
+    }
+    __syncthreads();
+    // 2. step 2 in place -> spectrum [j][line], j = ky / 2
+#pragma unroll
+    for (int i = 0; i < C::IY2; ++i) {
+      const int it = t + i * TH;
+      if (it < NL * RY2) {
+        const int line = it % NL, k2 = it / NL;
+        cplx<T> u[RY1];
+#pragma unroll
+        for (int n1 = 0; n1 < RY1; ++n1) u[n1] = sp[(n1 * RY2 + k2) * NLP + line];
+        fft2_step2<T, M, false>(u);
+#pragma unroll
+        for (int k1 = 0; k1 < RY1; ++k1) sp[(k2 + RY2 * k1) * NLP + line] = u[k1];
+      }
+    }
+    __syncthreads();
+    // 3. z stage: thread (j, c) transforms its column both ways; the tensor product runs per kz parity
+    {
+      const bool zt = t < 3 * M;
+      const int j = t % M, c = t / M;
+      cplx<T>* col = sp + j * NLP + c * N;
+      cplx<T> a[N], o[N];
+      if (zt) {
+#pragma unroll
+        for (int z = 0; z < N; ++z) {
+          a[z] = col[z];
+          o[z] = a[z] * tw2n[z];
+        }
+        Bfly<T, N>::run(a, false);
+        Bfly<T, N>::run(o, false);
+#pragma unroll
+        for (int m = 0; m < N; ++m) col[m] = a[m];  // even kz
+      }
+      auto zmul = [&](int pz) {
+#pragma unroll
+        for (int i = 0; i < C::IZ; ++i) {
+          const int e = t + i * TH;
+          if (e < M * N) {
+            const int jj = e % M, m = e / M;
+            cplx<T> f[6];
+            fN.load6(kx, 2 * jj + py, 2 * m + pz, f);
+            cplx<T>* s3 = sp + jj * NLP + m;
+            const cplx<T> jx = s3[0], jy = s3[N], jz = s3[2 * N];
+            s3[0] = f[0] * jx + f[1] * jy + f[2] * jz;
+            s3[N] = f[1] * jx + f[3] * jy + f[4] * jz;
+            s3[2 * N] = f[2] * jx + f[4] * jy + f[5] * jz;
+          }
+        }
+      };
+      __syncthreads();
+      zmul(0);
+      __syncthreads();
+      if (zt) {
+#pragma unroll
+        for (int m = 0; m < N; ++m) a[m] = col[m];
+        Bfly<T, N>::run(a, true);
+#pragma unroll
+        for (int m = 0; m < N; ++m) col[m] = o[m];  // odd kz
+      }
+      __syncthreads();
+      zmul(1);
+      __syncthreads();
+      if (zt) {
+#pragma unroll
+        for (int m = 0; m < N; ++m) o[m] = col[m];
+        Bfly<T, N>::run(o, true);
+#pragma unroll
+        for (int z = 0; z < N; ++z) col[z] = a[z] + cconj(tw2n[z]) * o[z];
+      }
+      __syncthreads();
+    }
+    // 4. inverse DFT_M along y, step 1
+#pragma unroll
+    for (int i = 0; i < C::IY1; ++i) {
+      const int it = t + i * TH;
+      if (it < NL * RY1) {
+        const int line = it % NL, n1 = it / NL;
+#pragma unroll
+        for (int n2 = 0; n2 < RY2; ++n2) v[i][n2] = sp[(n1 + RY1 * n2) * NLP + line];
+        fft2_step1<T, M, true>(v[i], n1, twm);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < C::IY1; ++i) {
+      const int it = t + i * TH;
+      if (it < NL * RY1) {
+        const int line = it % NL, n1 = it / NL;
+#pragma unroll
+        for (int k2 = 0; k2 < RY2; ++k2) sp[(n1 * RY2 + k2) * NLP + line] = v[i][k2];
+      }
+    }
+    __syncthreads();
+    // 5. step 2 in place -> [y][line]
+#pragma unroll
+    for (int i = 0; i < C::IY2; ++i) {
+      const int it = t + i * TH;
+      if (it < NL * RY2) {
+        const int line = it % NL, k2 = it / NL;
+        cplx<T> u[RY1];
+#pragma unroll
+        for (int n1 = 0; n1 < RY1; ++n1) u[n1] = sp[(n1 * RY2 + k2) * NLP + line];
+        fft2_step2<T, M, true>(u);
+#pragma unroll
+        for (int k1 = 0; k1 < RY1; ++k1) sp[(k2 + RY2 * k1) * NLP + line] = u[k1];
+      }
+    }
+    __syncthreads();
+    // 6. even half: keep in registers; odd half: combine and write the slab
+#pragma unroll
+    for (int i = 0; i < C::IS; ++i) {
+      const int e = t + i * TH;
+      if (e < NL * M) {
+        const int line = e / M, y = e - line * M;
+        const cplx<T> r = sp[y * NLP + line];
+        if (py == 0)
+          acc[i] = r;
+        else
+          g[e] = acc[i] + cconj(tw2m[y]) * r;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
 // operator assembly kernels (FP64)
 // ------------------------------------------------------------------------------------------------
 __device__ __forceinline__ void green6(double dx, double dy, double dz, double ko, c128* out) {
@@ -1119,7 +1379,7 @@ static bool launch_fz2(vs_ctx* ctx, cplx<T>* S, const TensorC<T>& fN, int L2, in
     using Cf = FzCfg<T, NN>;                                                                             \
     if (3 * NN * Cf::W > lds_budget<T>()) return false;                                                  \
     const int tiles = (L2 + Cf::W - 1) / Cf::W;                                                          \
-    dim3 grid((unsigned)(tiles * M2), (unsigned)nrhs);                                                   \
+    dim3 grid((unsigned)(((tiles * M2 + 7) / 8) * 8), (unsigned)nrhs);                                   \
     hipLaunchKernelGGL((k_fz2<T, NN>), grid, dim3(Cf::THREADS), 0, ctx->stream, pd.tw, S, fN, L2, M2, N, \
                        S_rhs);                                                                           \
     VS_HIP(hipGetLastError());                                                                           \
@@ -1143,6 +1403,22 @@ static size_t yz_bytes(int M2, int N2) {
 }
 
 template <typename T>
+static size_t yzq_bytes(int M, int N) {
+  switch (M * 4096 + N) {
+#define VS_YZQ_BYTES(A, B) case A * 4096 + B: return YzqCfg<T, A, B>::BYTES;
+    VS_YZQ_SHAPES(VS_YZQ_BYTES)
+#undef VS_YZQ_BYTES
+    default: return 0;
+  }
+}
+
+template <typename T>
+bool yzq_available(int L, int M, int N) {
+  const size_t b = yzq_bytes<T>(M, N);
+  return b > 0 && b <= (size_t)VS_LDS_MAX && fft2_length(2 * L);
+}
+
+template <typename T>
 bool yz_fused_available(int L, int M, int N) {
   const size_t b = yz_bytes<T>(2 * M, 2 * N);
   return b > 0 && b <= (size_t)VS_LDS_MAX && fft2_length(2 * L);
@@ -1150,9 +1426,36 @@ bool yz_fused_available(int L, int M, int N) {
 template bool yz_fused_available<float>(int, int, int);
 template bool yz_fused_available<double>(int, int, int);
 
-static bool fused_enabled() {
-  const char* e = std::getenv("VS_FUSED");
-  return !(e && e[0] == '0');
+// matvec variants: 0 = five passes, 1 = padded-slab fused kernel, 2 = parity-split fused kernel.
+// VS_MATVEC=pass5|slab|split forces one (when the operator holds its tensor layout); VS_FUSED=0 is
+// pass5.  Otherwise the variant timed fastest for the operator at its first domain is used.
+static int forced_variant() {
+  const char* f = std::getenv("VS_FUSED");
+  if (f && f[0] == '0') return 0;
+  const char* e = std::getenv("VS_MATVEC");
+  if (!e) return -1;
+  if (!std::strcmp(e, "pass5")) return 0;
+  if (!std::strcmp(e, "slab")) return 1;
+  if (!std::strcmp(e, "split")) return 2;
+  return -1;
+}
+
+template <typename T>
+static bool variant_available(const vs_op* op, int v) {
+  if (v == 0) return true;
+  if (v == 1) return (sizeof(T) == 8 ? op->fs64.p : op->fs32.p) != nullptr;
+  return (sizeof(T) == 8 ? op->fq64.p : op->fq32.p) != nullptr;
+}
+
+template <typename T>
+static int choose_variant(const vs_op* op) {
+  const int f = forced_variant();
+  if (f >= 0 && variant_available<T>(op, f)) return f;
+  const int tuned = op->mv_variant[sizeof(T) == 8 ? 0 : 1];
+  if (tuned >= 0 && variant_available<T>(op, tuned)) return tuned;
+  for (int v = 2; v >= 1; --v)
+    if (variant_available<T>(op, v)) return v;
+  return 0;
 }
 
 // raise the dynamic-LDS limit of the fused kernel for (2M, 2N) above the 64 KB default (done when the
@@ -1168,6 +1471,42 @@ static void yz_set_attr(int M2, int N2) {
     VS_YZ_SHAPES(VS_YZ_ATTR)
 #undef VS_YZ_ATTR
     default: return;
+  }
+}
+
+template <typename T>
+static void yzq_set_attr(int M, int N) {
+  switch (M * 4096 + N) {
+#define VS_YZQ_ATTR(A, B)                                                                                    \
+  case A * 4096 + B:                                                                                         \
+    VS_HIP(hipFuncSetAttribute((const void*)k_yzq<T, A, B>, hipFuncAttributeMaxDynamicSharedMemorySize,      \
+                               (int)YzqCfg<T, A, B>::BYTES));                                                \
+    return;
+    VS_YZQ_SHAPES(VS_YZQ_ATTR)
+#undef VS_YZQ_ATTR
+    default: return;
+  }
+}
+
+template <typename T>
+static void launch_yzq(vs_ctx* ctx, cplx<T>* S, const TensorQ<T>& fN, int L, int M, int N, int nrhs, int64_t S_rhs) {
+  const cplx<T>* twm = ctx->plan(M).dev<T>().tw;
+  const cplx<T>* tw2m = ctx->plan(2 * M).dev<T>().tw;
+  const cplx<T>* tw2n = ctx->plan(2 * N).dev<T>().tw;
+  const int per_xcd = (2 * L + 7) / 8;
+  dim3 grid((unsigned)(per_xcd * 8), (unsigned)nrhs);
+  switch (M * 4096 + N) {
+#define VS_YZQ_CASE(A, B)                                                                                  \
+  case A * 4096 + B: {                                                                                     \
+    using Cf = YzqCfg<T, A, B>;                                                                            \
+    hipLaunchKernelGGL((k_yzq<T, A, B>), grid, dim3(Cf::TH), Cf::BYTES, ctx->stream, twm, tw2m, tw2n, S,   \
+                       fN, S_rhs);                                                                         \
+    VS_HIP(hipGetLastError());                                                                             \
+    return;                                                                                                \
+  }
+    VS_YZQ_SHAPES(VS_YZQ_CASE)
+#undef VS_YZQ_CASE
+    default: throw Error{VS_ERR_INTERNAL, "no parity-split slab kernel for this shape"};
   }
 }
 
@@ -1211,14 +1550,21 @@ static void mvp_impl(vs_op* op, const cplx<T>* x, cplx<T>* y, int transp, int nr
   const int* map3 = op->map3.as<int>();
   VS_CHECK(map3 != nullptr, VS_ERR_ARG, "operator domain not set (vs_op_set_domain)");
   const int64_t lines = (int64_t)M * N * 3;
+  const int variant = choose_variant<T>(op);
+  const cplx<T>* fq = (sizeof(T) == 8) ? op->fq64.as<cplx<T>>() : op->fq32.as<cplx<T>>();
   const cplx<T>* fsl = (sizeof(T) == 8) ? op->fs64.as<cplx<T>>() : op->fs32.as<cplx<T>>();
-  if (fsl != nullptr && fused_enabled()) {
+  if (variant > 0) {
     // slab-major volume 2L x (3MN): x pass, fused y-z slab pass, inverse x pass
     const int64_t SL = lines, Sr = (int64_t)L2 * SL;
     IoMvFx<T> fx0{x, map3, op->nunk, op->V, L, M, transp, coef_dev<T>(op->b), S, Sr};
     run_rows<T, false, IoMvFxSlab<T>, false, true>(ctx, IoMvFxSlab<T>{fx0, SL}, L2, lines, L, L2, nrhs);
-    TensorS<T> ts{fsl, L, M, N};
-    launch_yz<T>(ctx, S, ts, L, M, N, nrhs, Sr);
+    if (variant == 2) {
+      TensorQ<T> tq{fq, L, M, N, M / 2 + 1, N / 2 + 1};
+      launch_yzq<T>(ctx, S, tq, L, M, N, nrhs, Sr);
+    } else {
+      TensorS<T> ts{fsl, L, M, N};
+      launch_yz<T>(ctx, S, ts, L, M, N, nrhs, Sr);
+    }
     IoMvIx<T> ix0{x, y, map3, op->nunk, op->V, L, M, transp, coef_dev<T>(op->a), coef_dev<T>(op->at),
                   coef_dev<T>(op->b), S, Sr, (T)(1.0 / (double)op->P)};
     run_rows<T, true, IoMvIxSlab<T>, true, false>(ctx, IoMvIxSlab<T>{ix0, SL}, L2, lines, L2, L, nrhs);
@@ -1264,6 +1610,56 @@ void mvp_dev(vs_op* op, const void* x, void* y, int transp, int precision, int n
     throw Error{VS_ERR_ARG, "precision must be VS_FP64 or VS_FP32"};
 }
 
+template <typename T>
+int matvec_variant(const vs_op* op) {
+  return choose_variant<T>(op);
+}
+template int matvec_variant<float>(const vs_op*);
+template int matvec_variant<double>(const vs_op*);
+
+// Autotuning: time every available matvec variant in each precision the operator holds (one warm-up and
+// three timed applications on a unit vector, HIP events on the context stream) and keep the fastest.
+// Done once per operator, at its first domain (the matvec needs the unknown map).
+template <typename T>
+static void autotune_prec(vs_op* op) {
+  vs_ctx* ctx = op->ctx;
+  const int slot = sizeof(T) == 8 ? 0 : 1;
+  DevBuf x(sizeof(cplx<T>) * op->nunk), y(sizeof(cplx<T>) * op->nunk);
+  VS_HIP(hipMemsetAsync(x.p, 0, sizeof(cplx<T>) * op->nunk, ctx->stream));
+  hipEvent_t e0, e1;
+  VS_HIP(hipEventCreate(&e0));
+  VS_HIP(hipEventCreate(&e1));
+  float best = 1e30f;
+  int best_v = 0;
+  for (int v = 0; v <= 2; ++v) {
+    if (!variant_available<T>(op, v)) continue;
+    op->mv_variant[slot] = v;
+    mvp_impl<T>(op, x.as<cplx<T>>(), y.as<cplx<T>>(), 0, 1);
+    VS_HIP(hipEventRecord(e0, ctx->stream));
+    for (int r = 0; r < 3; ++r) mvp_impl<T>(op, x.as<cplx<T>>(), y.as<cplx<T>>(), 0, 1);
+    VS_HIP(hipEventRecord(e1, ctx->stream));
+    VS_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    VS_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best) {
+      best = ms;
+      best_v = v;
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  op->mv_variant[slot] = best_v;
+}
+
+void op_autotune(vs_op* op) {
+  if (op->tuned) return;
+  const char* e = std::getenv("VS_AUTOTUNE");
+  if (e && e[0] == '0') return;
+  if (op->precisions & VS_FP64) autotune_prec<double>(op);
+  if (op->precisions & VS_FP32) autotune_prec<float>(op);
+  op->tuned = true;
+}
+
 // Gauss-Legendre nodes/weights on [-1,1] by Newton iteration on P_n (long double)
 static void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w) {
   x.resize(n);
@@ -1306,7 +1702,28 @@ void op_compress_tensor(vs_op* op) {
                        op->fc32.as<c64>(), op->L, op->M, op->N);
     VS_HIP(hipGetLastError());
   }
-  // slab-major copies for the fused matvec, in the precisions whose shape has one
+  // slab-major copies for the fused matvec variants, in the precisions whose shape has them
+  const int Hy = op->M / 2 + 1, Hz = op->N / 2 + 1;
+  const int64_t totq = (int64_t)(op->L + 1) * 6 * 2 * Hz * 2 * Hy;
+  const unsigned gq = (unsigned)((totq + 255) / 256);
+  bool q64 = false, q32 = false;
+  if ((op->precisions & VS_FP64) && yzq_available<double>(op->L, op->M, op->N)) {
+    op->fq64.ensure(sizeof(c128) * totq);
+    hipLaunchKernelGGL((k_compress_tensor_q<double>), dim3(gq), dim3(256), 0, ctx->stream, op->fN64.as<c128>(),
+                       op->fq64.as<c128>(), op->L, op->M, op->N, Hy, Hz);
+    VS_HIP(hipGetLastError());
+    yzq_set_attr<double>(op->M, op->N);
+    q64 = true;
+  }
+  if ((op->precisions & VS_FP32) && yzq_available<float>(op->L, op->M, op->N)) {
+    op->fq32.ensure(sizeof(c64) * totq);
+    hipLaunchKernelGGL((k_compress_tensor_q<float>), dim3(gq), dim3(256), 0, ctx->stream, op->fN64.as<c128>(),
+                       op->fq32.as<c64>(), op->L, op->M, op->N, Hy, Hz);
+    VS_HIP(hipGetLastError());
+    yzq_set_attr<float>(op->M, op->N);
+    q32 = true;
+  }
+  if (q64 || q32) ctx->plan(op->M);  // twiddles of the unpadded y transform, created outside any capture
   if ((op->precisions & VS_FP64) && yz_fused_available<double>(op->L, op->M, op->N)) {
     op->fs64.ensure(sizeof(c128) * tot);
     hipLaunchKernelGGL((k_compress_tensor_slab<double>), dim3(g), dim3(256), 0, ctx->stream, op->fN64.as<c128>(),
