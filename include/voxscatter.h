@@ -74,6 +74,20 @@ int vs_op_create_dda(vs_ctx* ctx, int L, int M, int N, const double* r, double k
 
 /* Operator from a caller-supplied circulant FFT fN (2L*2M*2N*6 complex128), e.g. a Galerkin operator
  * (the fN argument of MVP.m:1).  Toep (L*M*N*6, nullable) enables vs_pc_create_circ. */
+/* Galerkin (piecewise-constant JVIE) N operator on an L x M x N grid of cubes of side dx
+ * (replaces getOPERATORS.m:71-76 with op = 'N', type = 'FFTofC': assembly_nop_const.m:1-174, then
+ * circulant_nop_const.m and fft_operator.m).  The kernel is exp(-i ko R)/R.  singular: the five
+ * singular face-pair integrals [ST, EAc, EAo, VAc, VAo] as interleaved complex doubles (the values
+ * singular_st_const.m / singular_ea_const.m / singular_va_const.m produce), or NULL to have the
+ * library evaluate them (vs_galerkin_singular).  Grids need L, M, N >= 2. */
+int vs_op_create_galerkin(vs_ctx* ctx, int L, int M, int N, double dx, double ko, const double* singular,
+                          int precisions, vs_op** out);
+/* The singular integrals int_S int_S' exp(-i ko R)/R dS dS' of squares of side dx: coincident (ST),
+ * edge-adjacent coplanar / orthogonal (EAc, EAo), vertex-adjacent coplanar / orthogonal (VAc, VAo),
+ * as 5 interleaved complex doubles (replaces singular_st_const.m, singular_ea_const.m,
+ * singular_va_const.m with this library's own semi-analytic method instead of DIRECTFN). */
+int vs_galerkin_singular(vs_ctx* ctx, double ko, double dx, double out[10]);
+
 int vs_op_create_fN(vs_ctx* ctx, int L, int M, int N, const double* fN, const double* Toep,
                     int precisions, vs_op** out);
 void vs_op_destroy(vs_op* op);

@@ -176,6 +176,20 @@ class Operator:
         return cls(h, ctx, (L, M, N), precisions)
 
     @classmethod
+    def galerkin(cls, shape, dx: float, ko: float, singular=None, precisions: int = VS_FP64,
+                 ctx: Context | None = None):
+        """Galerkin N operator (getOPERATORS(r, f, 'N', 'FFTofC')) on an L x M x N grid of cubes of side dx.
+        singular: the five singular face-pair integrals [ST, EAc, EAo, VAc, VAo]; None lets the library
+        evaluate them (``galerkin_singular``)."""
+        ctx = ctx or get_context()
+        L, M, N = (int(v) for v in shape)
+        sg = None if singular is None else cplx_arr(np.asarray(singular, dtype=np.complex128))
+        h = C.c_void_p()
+        check(ctx.lib.vs_op_create_galerkin(ctx.h, L, M, N, float(dx), float(ko), dptr(sg), int(precisions),
+                                            C.byref(h)))
+        return cls(h, ctx, (L, M, N), precisions)
+
+    @classmethod
     def from_fN(cls, fN, Toep=None, precisions: int = VS_FP64, ctx: Context | None = None):
         ctx = ctx or get_context()
         fN = np.asarray(fN)
@@ -587,6 +601,15 @@ def refine_dev(A: Operator, b_ptr: int, x_ptr: int, M1=None, tol=1e-6, max_outer
                                   int(inner_maxit), int(restart), C.byref(flag), C.byref(relres), C.byref(outer),
                                   C.byref(tot)))
     return flag.value, relres.value, outer.value, tot.value
+
+
+def galerkin_singular(ko: float, dx: float, ctx: Context | None = None) -> np.ndarray:
+    """[ST, EAc, EAo, VAc, VAo]: int_S int_S' exp(-i ko R)/R over pairs of squares of side dx (coincident,
+    edge-adjacent coplanar/orthogonal, vertex-adjacent coplanar/orthogonal), evaluated on the device."""
+    ctx = ctx or get_context()
+    out = np.zeros(10)
+    check(ctx.lib.vs_galerkin_singular(ctx.h, float(ko), float(dx), dptr(out)))
+    return out[0::2] + 1j * out[1::2]
 
 
 def bench_mvp(A: Operator, x_ptr: int, y_ptr: int, transp=False, precision=VS_FP64, nrhs=1, nrep=20) -> float:

@@ -14,6 +14,8 @@ void build_circ_L(vs_ctx* ctx, const c128* toep, c128* cL, int L, int M, int N);
 void build_circ_M(vs_ctx* ctx, const c128* cL, c128* F, int L, int M, int N);
 template <typename T>
 int matvec_variant(const vs_op* op);
+void galerkin_singular(vs_ctx* ctx, double ko, double dx, c128 out[5]);
+void galerkin_toeplitz(vs_ctx* ctx, int L, int M, int N, double dx, double ko, const c128 sing[5], c128* G);
 void assemble2(vs_ctx* ctx, const c128* F, int L, int M, int N, int Lb, int Mb, c128 a, const c128* w, c128 ws,
                c128* out);
 void assemble1(vs_ctx* ctx, const c128* cL, int L, int M, int N, c128 a, const c128* w, c128 ws, c128* out);
@@ -224,6 +226,54 @@ int vs_op_create_dda(vs_ctx* ctx, int L, int M, int N, const double* r, double k
     op->P = 8 * op->V;
     op->precisions = precisions;
     op_build_dda(op.get(), r, ko, nearby_quad);
+    *out = op.release();
+  });
+}
+
+int vs_galerkin_singular(vs_ctx* ctx, double ko, double dx, double out[10]) {
+  return guarded([&] {
+    VS_CHECK(ctx && out, VS_ERR_ARG, "null pointer");
+    VS_CHECK(dx > 0 && ko >= 0, VS_ERR_ARG, "dx must be positive and ko non-negative");
+    DeviceGuard g(ctx->device);
+    c128 v[5];
+    galerkin_singular(ctx, ko, dx, v);
+    for (int i = 0; i < 5; ++i) {
+      out[2 * i] = v[i].x;
+      out[2 * i + 1] = v[i].y;
+    }
+  });
+}
+
+int vs_op_create_galerkin(vs_ctx* ctx, int L, int M, int N, double dx, double ko, const double* singular,
+                          int precisions, vs_op** out) {
+  return guarded([&] {
+    VS_CHECK(ctx && out, VS_ERR_ARG, "null pointer");
+    *out = nullptr;
+    check_dims(L, M, N);
+    VS_CHECK(M >= 2 && N >= 2, VS_ERR_UNSUPPORTED, "Galerkin grids need L, M, N >= 2");
+    VS_CHECK(dx > 0 && ko >= 0, VS_ERR_ARG, "dx must be positive and ko non-negative");
+    VS_CHECK((precisions & (VS_FP64 | VS_FP32)) != 0, VS_ERR_ARG, "precisions must include VS_FP64 or VS_FP32");
+    DeviceGuard g(ctx->device);
+    auto op = std::make_unique<vs_op>();
+    op->ctx = ctx;
+    op->L = L;
+    op->M = M;
+    op->N = N;
+    op->V = (int64_t)L * M * N;
+    op->P = 8 * op->V;
+    op->precisions = precisions;
+    c128 sing[5];
+    if (singular) {
+      for (int i = 0; i < 5; ++i) sing[i] = mk<double>(singular[2 * i], singular[2 * i + 1]);
+    } else {
+      galerkin_singular(ctx, ko, dx, sing);
+    }
+    op->toep.ensure(sizeof(c128) * 6 * op->V);
+    galerkin_toeplitz(ctx, L, M, N, dx, ko, sing, op->toep.as<c128>());
+    op->has_toep = true;
+    op_build_fN(op.get());
+    VS_HIP(hipStreamSynchronize(ctx->stream));
+    if (!(op->precisions & VS_FP64)) op->fN64.release();
     *out = op.release();
   });
 }

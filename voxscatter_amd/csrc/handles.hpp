@@ -3,6 +3,8 @@
 
 #include "vs_internal.hpp"
 
+#include <vector>
+
 namespace vs {
 
 // Per-voxel or scalar complex coefficient of the MVP diagonal.
@@ -33,6 +35,8 @@ void launch_col_fft(vs_ctx* ctx, int prec, void* data, int64_t fast, int64_t axi
 void launch_row_fft_dense(vs_ctx* ctx, int prec, void* data, int64_t nlines, int n_in, int n, int n_out,
                           int64_t ld_in, int64_t ld_out, bool inv, double scale, int nrhs, int64_t rhs_stride);
 void op_autotune(vs_op* op);
+void op_build_fN(vs_op* op);
+void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w);
 // compressed vector -> dense L x (lines) volume with an x FFT of length L (forward)
 void launch_row_fft_gather(vs_ctx* ctx, int prec, const void* x, const int* map3, int64_t nunk, void* out,
                            int L, int64_t nlines, int nrhs, int64_t out_rhs_stride);

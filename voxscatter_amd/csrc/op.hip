@@ -1660,8 +1660,8 @@ void op_autotune(vs_op* op) {
   op->tuned = true;
 }
 
-// Gauss-Legendre nodes/weights on [-1,1] by Newton iteration on P_n (long double)
-static void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w) {
+// Gauss-Legendre nodes/weights on [-1,1] by Newton iteration on P_n (long double), ascending nodes
+void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w) {
   x.resize(n);
   w.resize(n);
   const long double PI = 3.141592653589793238462643383279502884L;
