@@ -156,6 +156,14 @@ int vs_pgmres(vs_op* op, vs_pc* pc, int side, const double* b, double* x, int re
               int maxit, const double* x0, int* flag, double* relres, int iter[2], double* resvec,
               int* nres);
 
+/* pgmresDR(A,b,restart,tol,maxit,ritz,L1,[],R1,[],x0) of pgmresDR.m:73-280 (GMRES with deflated
+ * restarting: ritz harmonic Ritz vectors carried between cycles; ritz <= 0 takes pgmresDR.m's
+ * default min(10, round(restart/2))).  side 0: pc as L1, 1: as R1.  resvec holds up to
+ * restart*(maxit+1)+1 values. */
+int vs_pgmresdr(vs_op* op, vs_pc* pc, int side, const double* b, double* x, int restart, double tol,
+                int maxit, int ritz, const double* x0, int* flag, double* relres, int iter[2],
+                double* resvec, int* nres);
+
 /* Device-pointer forms: b, x are FP64 device vectors of vs_op_nunknowns() values. */
 int vs_bicgstab_dev(vs_op* op, vs_pc* pc, const void* b, void* x, double tol, int maxit, int* flag,
                     double* relres, double* iter, int* nres);

@@ -339,3 +339,121 @@ def pgmres_iter(afun, x, r, m, L1, L2, R1, R2, tol):
     x = x + xupd
     r = V[:, :k + 1] @ res
     return x, r, k, resvec
+
+
+# ------------------------------------------------------------------------------------------------
+# GMRES with deflated restarting (pgmresDR.m, pgmresDR_iter.m, getHarmVecs1.m, getHarmVecs2.m)
+# ------------------------------------------------------------------------------------------------
+def _apply_side(afun, v, L1, R1):
+    if R1 is not None:
+        v = R1(v)
+    v = afun(v)
+    if L1 is not None:
+        v = L1(v)
+    return v
+
+
+def pgmresDR_iter(afun, r, m, L1, R1, C, tol):
+    """One Arnoldi cycle of pgmresDR_iter.m:23-141 (classical Gram-Schmidt after the deflation against C);
+    returns V (n x k+1), H (k+1 x k), B (size(C,2) x k), k, resvec (projected residual norms)."""
+    n = r.shape[0]
+    V = np.zeros((n, m + 1), dtype=np.complex128)
+    rn = _norm(r)
+    V[:, 0] = r / rn
+    kc = 0 if C is None else C.shape[1]
+    B = np.zeros((kc, m), dtype=np.complex128)
+    H = np.zeros((m + 1, m), dtype=np.complex128)
+    resvec = np.zeros(m)
+    for k in range(1, m + 1):
+        w = _apply_side(afun, V[:, k - 1], L1, R1)
+        if kc:
+            B[:, k - 1] = C.conj().T @ w
+            w = w - C @ B[:, k - 1]
+        H[:k, k - 1] = V[:, :k].conj().T @ w
+        w = w - V[:, :k] @ H[:k, k - 1]
+        H[k, k - 1] = _norm(w)
+        V[:, k] = w / H[k, k - 1]
+        rhs = np.zeros(k + 1, dtype=np.complex128)
+        rhs[0] = rn
+        y = np.linalg.lstsq(H[:k + 1, :k], rhs, rcond=None)[0]
+        resvec[k - 1] = _norm(rhs - H[:k + 1, :k] @ y)
+        if resvec[k - 1] < tol:
+            return V[:, :k + 1], H[:k + 1, :k], B[:, :k], k, resvec[:k]
+    return V, H, B, m, resvec
+
+
+def getHarmVecs1(m, k, H):
+    """getHarmVecs1.m:14-29: harmonic Ritz vectors of H (m+1 x m) with the k smallest |theta|."""
+    F = np.linalg.solve(H[:m, :].conj().T, np.eye(m))
+    F[:, :m - 1] = 0
+    Hr = H[:m, :] + H[m, m - 1] ** 2 * F
+    vals, vecs = np.linalg.eig(Hr)
+    order = np.argsort(np.abs(vals), kind="stable")
+    return vecs[:, order[:k]]
+
+
+def getHarmVecs2(m, k, H2, V, U, C):
+    """getHarmVecs2.m:15-36: generalized harmonic Ritz vectors, the k of largest |lambda| of eig(A, B)."""
+    import scipy.linalg
+    B = H2.conj().T @ H2
+    A = np.zeros((m + 1, m), dtype=np.complex128)
+    A[:k, :k] = C.conj().T @ U
+    A[k:m + 1, :k] = V.conj().T @ U
+    A[k:m, k:m] = np.eye(m - k)
+    A = H2.conj().T @ A
+    vals, vecs = scipy.linalg.eig(A, B)
+    order = np.argsort(np.abs(vals), kind="stable")
+    return vecs[:, order[m - k:]]
+
+
+def pgmresDR(afun, b, restart=50, tol=1e-3, maxit=10, ritz=None, L1=None, R1=None, x0=None):
+    """``[x,flag,relres,iter,resvec] = pgmresDR(A,b,restart,tol,maxit,ritz,L1,[],R1,[],x0)``
+    (pgmresDR.m:73-280; one left factor L1 or right factor R1)."""
+    b = np.asarray(b, dtype=np.complex128)
+    ritz = min(10, int(math.floor(restart / 2 + 0.5))) if ritz is None else min(ritz, restart - 1)
+    x0 = np.zeros_like(b) if x0 is None else np.array(x0, dtype=np.complex128)
+    r = b - afun(x0)
+    bp = b.copy()
+    if L1 is not None:
+        r = L1(r)
+        bp = L1(bp)
+    bnorm = _norm(bp)
+    resvec = [_norm(r) / bnorm]
+    V, H, _, p, rin = pgmresDR_iter(afun, r, restart, L1, R1, None, tol * bnorm)
+    resvec += list(rin / bnorm)
+    y = np.linalg.lstsq(H, V.conj().T @ r, rcond=None)[0]
+    x = V[:, :p] @ y
+    r = r - V @ (H @ y)
+    U = C = None
+    if resvec[-1] > tol:
+        k = p - 1 if ritz > p - 1 else ritz
+        P = getHarmVecs1(p, k, H)
+        U = V[:, :p] @ P
+        C, R = np.linalg.qr(H @ P)
+        C = V @ C
+        U = np.linalg.solve(R.T, U.T).T
+    outit = 0
+    while resvec[-1] > tol and outit < maxit:
+        outit += 1
+        V, H, B, p, rin = pgmresDR_iter(afun, r, restart, L1, R1, C, tol * bnorm)
+        resvec += list(rin / bnorm)
+        d = np.array([_norm(U[:, i]) for i in range(ritz)])
+        D = np.diag(1.0 / d)
+        U = U @ D
+        H2 = np.block([[D, B], [np.zeros((H.shape[0], D.shape[1])), H]])
+        CV = np.hstack([C, V])
+        y = np.linalg.lstsq(H2, CV.conj().T @ r, rcond=None)[0]
+        x = x + np.hstack([U, V[:, :p]]) @ y
+        r = r - CV @ (H2 @ y)
+        if resvec[-1] > tol:
+            P = getHarmVecs2(p + ritz, ritz, H2, V, U, C)
+            U = np.hstack([U, V[:, :p]]) @ P
+            Q, R = np.linalg.qr(H2 @ P)
+            C = CV @ Q
+            U = np.linalg.solve(R.T, U.T).T
+    if R1 is not None:
+        x = R1(x)
+    x = x0 + x
+    relres = _norm(b - afun(x)) / _norm(b)
+    flag = 0 if relres <= tol else 1
+    return x, flag, relres, (p, outit), np.array(resvec)

@@ -148,3 +148,21 @@ def test_refinement_reaches_fp64_tolerance(c1, ctx):
         assert flag == 0 and relres <= TOL
         assert np.linalg.norm(c1["afun"](x) - sysd.b) / np.linalg.norm(sysd.b) <= TOL
         assert _rel(x, xr) <= 50 * TOL
+
+
+@pytest.mark.parametrize("side", ["R1", "L1"])
+def test_pgmresdr_parity(c1, side):
+    """GMRES with deflated restarting against the oracle: identical iteration counts, residual history
+    and solution (the harmonic Ritz vectors are basis-invariant, so the eigenvector scaling and QR sign
+    conventions of the two implementations do not change the iterates)."""
+    print(f"[pgmresDR] side={side}", flush=True)
+    sysd = c1["sys"]
+    pc = sysd.preconditioner(2)
+    mfun = _oracle_pc(c1, 2)
+    x, flag, relres, it, resvec = api.pgmresDR(sysd.op, sysd.b, 8, 1e-8, 20, 3, **{side: pc})
+    xr, flagr, relresr, itr, resvecr = OK.pgmresDR(c1["afun"], sysd.b, 8, 1e-8, 20, 3, **{side: mfun})
+    print(f"  gpu: flag={flag} iter={it} nres={len(resvec)}  cpu: flag={flagr} iter={itr} nres={len(resvecr)}")
+    assert flag == flagr == 0
+    assert tuple(it) == tuple(itr) and len(resvec) == len(resvecr)
+    assert np.allclose(resvec, resvecr, rtol=1e-7, atol=1e-14)
+    assert _rel(x, xr) <= 1e-9

@@ -89,3 +89,37 @@ def test_pgmres_matches_direct_solve(side):
     assert np.linalg.norm(x - np.linalg.solve(A, b)) <= 1e-9 * np.linalg.norm(x)
     assert resvec[0] == pytest.approx(1.0)
     assert np.all(np.diff(resvec[:p + 1]) <= 1e-12)  # GMRES residuals are monotone within a cycle
+
+
+def _outlier_system(n=120, seed=0):
+    rng = np.random.default_rng(seed)
+    Q = np.linalg.qr(rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)))[0]
+    ev = np.concatenate([[1e-3, 2e-3, 5e-3, 1e-2], 1 + 0.5 * rng.uniform(size=n - 4)])
+    A = Q @ np.diag(ev) @ Q.conj().T + 0.01 * rng.normal(size=(n, n))
+    b = rng.normal(size=n) + 0j
+    return A, b
+
+
+def test_pgmresdr_solves_and_first_cycle_is_pgmres():
+    A, b = _outlier_system()
+    af = lambda v: A @ v  # noqa: E731
+    xs = np.linalg.solve(A, b)
+    x, flag, relres, it, rv = K.pgmresDR(af, b, 20, 1e-10, 30, 6)
+    assert flag == 0 and np.linalg.norm(x - xs) / np.linalg.norm(xs) < 1e-9
+    # without deflated cycles (maxit = 0) the method is the first GMRES(m) cycle of pgmres
+    x1, f1, r1, i1, rv1 = K.pgmresDR(af, b, 20, 1e-10, 0, 6)
+    xp, fp, rp, ip, rvp = K.pgmres(af, b, 20, 1e-10, 30)
+    assert np.allclose(rv1, rvp[:len(rv1)], rtol=1e-12)
+    # deflating the four small eigenvalues beats plain restarting
+    assert len(rv) < len(rvp)
+
+
+def test_pgmresdr_preconditioned_sides():
+    A, b = _outlier_system(60, 1)
+    Minv = np.linalg.inv(np.diag(np.diag(A)))
+    af = lambda v: A @ v  # noqa: E731
+    mf = lambda v: Minv @ v  # noqa: E731
+    xs = np.linalg.solve(A, b)
+    for kw in ({"L1": mf}, {"R1": mf}):
+        x, flag, relres, it, rv = K.pgmresDR(af, b, 15, 1e-10, 20, 4, **kw)
+        assert flag == 0 and np.linalg.norm(x - xs) / np.linalg.norm(xs) < 1e-8

@@ -548,6 +548,29 @@ def pgmres(A, b, restart=50, tol=1e-3, maxit=10, L1=None, L2=None, R1=None, R2=N
     return x, flag.value, relres.value, (it[0], it[1]), resvec[:nres.value].copy()
 
 
+def pgmresDR(A, b, restart=50, tol=1e-3, maxit=10, ritz=None, L1=None, L2=None, R1=None, R2=None, x0=None):
+    """``[x, flag, relres, iter, resvec] = pgmresDR(A, b, restart, tol, maxit, ritz, L1, L2, R1, R2, x0)``:
+    GMRES with deflated restarting (ritz harmonic Ritz vectors kept between cycles).  One preconditioner
+    factor is supported, either as L1 (left) or R1 (right)."""
+    A = _check_A(A)
+    if L2 is not None or R2 is not None or (L1 is not None and R1 is not None):
+        raise VoxScatterError(capi.VS_ERR_UNSUPPORTED, "only one of L1 / R1 is supported")
+    pc, side = (L1, 0) if L1 is not None else (R1, 1)
+    n = A.n
+    b = cplx_arr(b, n)
+    x = np.zeros_like(b)
+    x0a = None if x0 is None else cplx_arr(x0, n)
+    flag = C.c_int()
+    relres = C.c_double()
+    it = (C.c_int * 2)()
+    nres = C.c_int()
+    resvec = np.zeros(int(restart) * (int(maxit) + 1) + 1)
+    check(A.ctx.lib.vs_pgmresdr(A.h, None if pc is None else pc.h, side, dptr(b), dptr(x), int(restart), float(tol),
+                                int(maxit), 0 if ritz is None else int(ritz), dptr(x0a), C.byref(flag),
+                                C.byref(relres), it, dptr(resvec), C.byref(nres)))
+    return x, flag.value, relres.value, (it[0], it[1]), resvec[:nres.value].copy()
+
+
 # --------------------------------------------------------------------------------------------------
 # precision form and device-pointer entry points
 # --------------------------------------------------------------------------------------------------

@@ -696,6 +696,32 @@ int vs_pgmres(vs_op* op, vs_pc* pc, int side, const double* b, double* x, int re
   });
 }
 
+int vs_pgmresdr(vs_op* op, vs_pc* pc, int side, const double* b, double* x, int restart, double tol, int maxit,
+                int ritz, const double* x0, int* flag, double* relres, int iter[2], double* resvec, int* nres) {
+  return guarded([&] {
+    VS_CHECK(op && b && x && flag && relres && iter && resvec && nres, VS_ERR_ARG, "null pointer");
+    VS_CHECK(maxit >= 0 && restart >= 2, VS_ERR_ARG, "maxit must be >= 0 and restart >= 2");
+    VS_CHECK(side == 0 || side == 1, VS_ERR_ARG, "side must be 0 (L1) or 1 (R1)");
+    check_solver_args(op, pc, VS_FP64);
+    vs_ctx* ctx = op->ctx;
+    DeviceGuard g(ctx->device);
+    const int64_t n = op->nunk;
+    DevBuf db(sizeof(c128) * n), dx(sizeof(c128) * n);
+    h2d(ctx, db.p, b, sizeof(c128) * n);
+    if (x0) h2d(ctx, dx.p, x0, sizeof(c128) * n);
+    Engine<double> E(ctx, n);
+    SolveResult R = solve_pgmresdr<double>(E, make_linop<double>(op, pc), side, db.as<c128>(), dx.as<c128>(),
+                                           x0 != nullptr, restart, tol, maxit, ritz);
+    d2h(ctx, x, dx.p, sizeof(c128) * n);
+    *flag = R.flag;
+    *relres = R.relres;
+    iter[0] = R.inner;
+    iter[1] = R.outer;
+    *nres = (int)R.resvec.size();
+    std::copy(R.resvec.begin(), R.resvec.end(), resvec);
+  });
+}
+
 int vs_gmres_dev(vs_op* op, vs_pc* pc, const void* b, void* x, int restart, double tol, int maxit, int* flag,
                  double* relres, int iter[2], int* nres) {
   return guarded([&] {

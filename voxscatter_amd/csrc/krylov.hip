@@ -1,4 +1,5 @@
 // Krylov solver control logic (host) over device-resident vectors.
+#include "dense.hpp"
 #include "solver.hpp"
 
 #include <cfloat>
@@ -524,6 +525,308 @@ SolveResult solve_pgmres(Engine<T>& E, const LinOp<T>& op, int side, const cplx<
   VS_HIP(hipStreamSynchronize(E.ctx->stream));
   return R;
 }
+
+// ------------------------------------------------------------------------------------------------
+// GMRES with deflated restarting (pgmresDR.m:73-280, pgmresDR_iter.m:23-141, getHarmVecs1.m,
+// getHarmVecs2.m): after each cycle the k harmonic Ritz vectors of smallest harmonic Ritz value are
+// kept as U (with C = A U orthonormal); the next cycle's Arnoldi vectors are orthogonalised against C
+// and the iterate is the least-squares minimiser over span[U V].  The small projected problems run on
+// the host in FP64 (QR least squares, eigenvalues; eig(A, B) through a Cholesky reduction of the
+// Hermitian positive definite B = H2^H H2), the bases live on the device.
+// ------------------------------------------------------------------------------------------------
+namespace {
+
+using dense::Mat;
+using dense::zd;
+
+inline zd Z(c128 a) { return zd(a.x, a.y); }
+inline c128 Cz(zd a) { return mk<double>(a.real(), a.imag()); }
+
+std::vector<c128> to_c(const std::vector<zd>& v) {
+  std::vector<c128> o(v.size());
+  for (size_t i = 0; i < v.size(); ++i) o[i] = Cz(v[i]);
+  return o;
+}
+std::vector<c128> to_c(const Mat& M) {
+  std::vector<c128> o(M.a.size());
+  for (size_t i = 0; i < M.a.size(); ++i) o[i] = Cz(M.a[i]);
+  return o;
+}
+
+// indices of the eigenvalues sorted by modulus, ascending (stable)
+std::vector<int> order_by_modulus(const std::vector<zd>& w) {
+  std::vector<int> idx(w.size());
+  std::iota(idx.begin(), idx.end(), 0);
+  std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return std::abs(w[a]) < std::abs(w[b]); });
+  return idx;
+}
+
+// getHarmVecs1.m: harmonic Ritz vectors of H (m+1 x m), the k of smallest |theta|
+Mat harm_vecs1(int m, int k, const Mat& H) {
+  Mat Hm(m, m);
+  for (int j = 0; j < m; ++j)
+    for (int i = 0; i < m; ++i) Hm(i, j) = H(i, j);
+  // last column of inv(Hm^H): solve Hm^H f = e_m
+  Mat HmH = dense::adjoint(Hm);
+  std::vector<zd> em(m, zd(0));
+  em[m - 1] = 1;
+  std::vector<zd> f = dense::lstsq(HmH, em);
+  const zd h = H(m, m - 1);
+  Mat Hr = Hm;
+  for (int i = 0; i < m; ++i) Hr(i, m - 1) += h * h * f[i];
+  std::vector<zd> w;
+  Mat X;
+  dense::eig(Hr, w, X);
+  const std::vector<int> idx = order_by_modulus(w);
+  Mat P(m, k);
+  for (int j = 0; j < k; ++j)
+    for (int i = 0; i < m; ++i) P(i, j) = X(i, idx[j]);
+  return P;
+}
+
+// getHarmVecs2.m: eig(H2^H A, H2^H H2) with A = [C'U 0; V'U [I; 0]], the k of largest |lambda|
+Mat harm_vecs2(int m, int k, const Mat& H2, const Mat& CU, const Mat& VU) {
+  Mat A(m + 1, m);
+  for (int j = 0; j < k; ++j)
+    for (int i = 0; i < k; ++i) A(i, j) = CU(i, j);
+  for (int j = 0; j < k; ++j)
+    for (int i = 0; i < m + 1 - k; ++i) A(k + i, j) = VU(i, j);
+  for (int i = k; i < m; ++i) A(i, i) = 1;
+  const Mat H2h = dense::adjoint(H2);
+  const Mat AA = dense::matmul(H2h, A), B = dense::matmul(H2h, H2);
+  // B = L L^H: eig(L^{-1} AA L^{-H}) has the same eigenvalues, eigenvectors z = L^{-H} w
+  const Mat L = dense::cholesky(B);
+  Mat T1 = dense::lower_solve(L, AA);                                     // L^{-1} AA
+  Mat T2 = dense::adjoint(dense::lower_solve(L, dense::adjoint(T1)));      // (L^{-1} (L^{-1} AA)^H)^H
+  std::vector<zd> w;
+  Mat Wv;
+  dense::eig(T2, w, Wv);
+  Mat Zv = dense::lower_adj_solve(L, Wv);
+  const std::vector<int> idx = order_by_modulus(w);
+  Mat P(m, k);
+  for (int j = 0; j < k; ++j)
+    for (int i = 0; i < m; ++i) P(i, j) = Zv(i, idx[m - k + j]);
+  return P;
+}
+
+}  // namespace
+
+template <typename T>
+SolveResult solve_pgmresdr(Engine<T>& E, const LinOp<T>& op, int side, const cplx<T>* b, cplx<T>* x, bool has_x0,
+                           int restart, double tol, int maxit, int ritz) {
+  using cv = cplx<T>;
+  const int64_t n = E.n;
+  SolveResult R;
+  VS_CHECK(restart >= 2 && restart <= 500, VS_ERR_UNSUPPORTED, "pgmresDR restart must be in 2..500");
+  ritz = ritz <= 0 ? std::min(10, (int)std::lround(restart / 2.0)) : std::min(ritz, restart - 1);
+  const bool left = op.M && side == 0, right = op.M && side == 1;
+  const int m = restart;
+  Vecs<T> Vb(n, m + 1), Cb(n, ritz), Ub(n, ritz), C2(n, ritz), U2(n, ritz), W(n, 5);
+  cv *r = W[0], *w = W[1], *tmp = W[2], *xa = W[3], *x0 = W[4];
+  cv *Cp = Cb[0], *Up = Ub[0], *Cn = C2[0], *Un = U2[0];
+  // r = b - A x0, bp = b, both left-preconditioned (pgmresDR.m:110-139)
+  if (has_x0) {
+    E.copy(x, x0);
+    op.A(x0, r);
+    E.update(UPD_SUB_, b, nullptr, nullptr, nullptr, r, nullptr, C(0), C(0), false);
+  } else {
+    E.copy(b, r);
+  }
+  double bnorm;
+  if (left) {
+    op.M(r, tmp);
+    E.copy(tmp, r);
+    op.M(b, tmp);
+    bnorm = E.norm(tmp);
+  } else {
+    bnorm = E.norm(b);
+  }
+  std::vector<double> resvec{E.norm(r) / bnorm};
+  VS_HIP(hipMemsetAsync(xa, 0, sizeof(cv) * n, E.ctx->stream));
+  int kc = 0;  // deflation vectors in C, U
+
+  // one Arnoldi cycle from r (pgmresDR_iter.m), orthogonalised against C(:, 1:kc); returns p and fills
+  // H (p+1 x p) and Bm (kc x p); appends the projected residual norms to resvec
+  auto arnoldi = [&](Mat& H, Mat& Bm) -> int {
+    const double rn = E.norm(r);
+    E.update(UPD_SCALE_, r, nullptr, nullptr, nullptr, Vb[0], nullptr, C(1.0 / rn), C(0), false);
+    H = Mat(m + 1, m);
+    Bm = Mat(kc, m);
+    std::vector<zd> g(m + 1, zd(0)), cs(m), sn(m);
+    g[0] = rn;
+    Mat Hg(m + 1, m);
+    int k = 1;
+    for (; k <= m; ++k) {
+      const cv* in = Vb[k - 1];
+      if (right) {
+        op.M(in, tmp);
+        op.A(tmp, w);
+      } else if (left) {
+        op.A(in, tmp);
+        op.M(tmp, w);
+      } else {
+        op.A(in, w);
+      }
+      if (kc > 0) {
+        std::vector<c128> bc(kc);
+        E.mdot(Cp, n, kc, w, bc.data());
+        E.gemv_acc(Cp, n, kc, bc.data(), -1.0, w);
+        for (int i = 0; i < kc; ++i) Bm(i, k - 1) = Z(bc[i]);
+      }
+      std::vector<c128> h(k);
+      E.mdot(Vb[0], n, k, w, h.data());
+      E.gemv_acc(Vb[0], n, k, h.data(), -1.0, w);
+      const double hn = E.norm(w);
+      for (int i = 0; i < k; ++i) H(i, k - 1) = Z(h[i]);
+      H(k, k - 1) = hn;
+      E.update(UPD_SCALE_, w, nullptr, nullptr, nullptr, Vb[k], nullptr, C(hn > 0 ? 1.0 / hn : 0.0), C(0), false);
+      // projected residual norm by Givens rotations (same as norm(rhs - H y), y = H \ rhs)
+      const int c = k - 1;
+      for (int i = 0; i <= k; ++i) Hg(i, c) = H(i, c);
+      for (int i = 0; i < c; ++i) {
+        const zd hi = Hg(i, c), hi1 = Hg(i + 1, c);
+        Hg(i, c) = std::conj(cs[i]) * hi + std::conj(sn[i]) * hi1;
+        Hg(i + 1, c) = -sn[i] * hi + cs[i] * hi1;
+      }
+      const zd a = Hg(c, c), bb = Hg(c + 1, c);
+      const double rr = std::sqrt(std::norm(a) + std::norm(bb));
+      cs[c] = rr > 0 ? a / rr : zd(1);
+      sn[c] = rr > 0 ? bb / rr : zd(0);
+      g[c + 1] = -sn[c] * g[c];
+      g[c] = std::conj(cs[c]) * g[c];
+      const double res = std::abs(g[c + 1]);
+      resvec.push_back(res / bnorm);
+      if (res < tol * bnorm) break;
+    }
+    const int p = std::min(k, m);
+    Mat Hp(p + 1, p), Bp(kc, p);
+    for (int j = 0; j < p; ++j) {
+      for (int i = 0; i <= p; ++i) Hp(i, j) = H(i, j);
+      for (int i = 0; i < kc; ++i) Bp(i, j) = Bm(i, j);
+    }
+    H = Hp;
+    Bm = Bp;
+    return p;
+  };
+
+  // first cycle (pgmresDR.m:152-185)
+  Mat H, Bm;
+  int p = arnoldi(H, Bm);
+  {
+    std::vector<c128> vr(p + 1);
+    E.mdot(Vb[0], n, p + 1, r, vr.data());
+    std::vector<zd> rhs(p + 1);
+    for (int i = 0; i <= p; ++i) rhs[i] = Z(vr[i]);
+    const std::vector<zd> y = dense::lstsq(H, rhs);
+    E.gemv_acc(Vb[0], n, p, to_c(y).data(), 1.0, xa);
+    std::vector<zd> hy(p + 1, zd(0));
+    for (int j = 0; j < p; ++j)
+      for (int i = 0; i <= p; ++i) hy[i] += H(i, j) * y[j];
+    E.gemv_acc(Vb[0], n, p + 1, to_c(hy).data(), -1.0, r);
+    if (resvec.back() > tol) {
+      const int k = ritz > p - 1 ? p - 1 : ritz;
+      const Mat P = harm_vecs1(p, k, H);
+      Mat Q, Rq;
+      dense::qr_econ(dense::matmul(H, P), Q, Rq);
+      const Mat PR = dense::right_div_upper(P, Rq);  // U = V(:,1:p) P R^{-1}
+      E.combine(Vb[0], n, p, to_c(PR).data(), k, Up, n, false);
+      E.combine(Vb[0], n, p + 1, to_c(Q).data(), k, Cp, n, false);
+      kc = k;
+    }
+  }
+  // deflated cycles (pgmresDR.m:187-255)
+  int outit = 0;
+  while (resvec.back() > tol && outit < maxit) {
+    ++outit;
+    p = arnoldi(H, Bm);
+    // U = U D, D = diag(1 / ||U(:, i)||)
+    std::vector<double> d(kc);
+    for (int i = 0; i < kc; ++i) {
+      cv* ui = Up + (int64_t)i * n;
+      d[i] = E.norm(ui);
+      E.update(UPD_SCALE_, ui, nullptr, nullptr, nullptr, ui, nullptr, C(1.0 / d[i]), C(0), false);
+    }
+    const int mm = kc + p;
+    Mat H2(mm + 1, mm);
+    for (int i = 0; i < kc; ++i) H2(i, i) = 1.0 / d[i];
+    for (int j = 0; j < p; ++j) {
+      for (int i = 0; i < kc; ++i) H2(i, kc + j) = Bm(i, j);
+      for (int i = 0; i <= p; ++i) H2(kc + i, kc + j) = H(i, j);
+    }
+    // y = H2 \ ([C V]' r); x += [U V] y; r -= [C V] (H2 y)
+    std::vector<c128> cr(std::max(kc, 1)), vr(p + 1);
+    if (kc > 0) E.mdot(Cp, n, kc, r, cr.data());
+    E.mdot(Vb[0], n, p + 1, r, vr.data());
+    std::vector<zd> rhs(mm + 1);
+    for (int i = 0; i < kc; ++i) rhs[i] = Z(cr[i]);
+    for (int i = 0; i <= p; ++i) rhs[kc + i] = Z(vr[i]);
+    const std::vector<zd> y = dense::lstsq(H2, rhs);
+    const std::vector<c128> yc = to_c(y);
+    if (kc > 0) E.gemv_acc(Up, n, kc, yc.data(), 1.0, xa);
+    E.gemv_acc(Vb[0], n, p, yc.data() + kc, 1.0, xa);
+    std::vector<zd> t(mm + 1, zd(0));
+    for (int j = 0; j < mm; ++j)
+      for (int i = 0; i <= mm; ++i) t[i] += H2(i, j) * y[j];
+    const std::vector<c128> tc = to_c(t);
+    if (kc > 0) E.gemv_acc(Cp, n, kc, tc.data(), -1.0, r);
+    E.gemv_acc(Vb[0], n, p + 1, tc.data() + kc, -1.0, r);
+    if (resvec.back() > tol) {
+      // C'U and V'U (getHarmVecs2.m:16-18)
+      Mat CU(kc, kc), VU(p + 1, kc);
+      for (int j = 0; j < kc; ++j) {
+        std::vector<c128> a(std::max(kc, 1)), bvec(p + 1);
+        E.mdot(Cp, n, kc, Up + (int64_t)j * n, a.data());
+        E.mdot(Vb[0], n, p + 1, Up + (int64_t)j * n, bvec.data());
+        for (int i = 0; i < kc; ++i) CU(i, j) = Z(a[i]);
+        for (int i = 0; i <= p; ++i) VU(i, j) = Z(bvec[i]);
+      }
+      const Mat P = harm_vecs2(mm, ritz, H2, CU, VU);
+      Mat Q, Rq;
+      dense::qr_econ(dense::matmul(H2, P), Q, Rq);
+      const Mat PR = dense::right_div_upper(P, Rq);
+      // U_new = [U V(:,1:p)] P R^{-1},  C_new = [C V] Q
+      const std::vector<c128> prc = to_c(PR), qc = to_c(Q);
+      std::vector<c128> top((size_t)kc * ritz), bot((size_t)p * ritz), qtop((size_t)kc * ritz),
+          qbot((size_t)(p + 1) * ritz);
+      for (int j = 0; j < ritz; ++j) {
+        for (int i = 0; i < kc; ++i) {
+          top[i + (size_t)kc * j] = prc[i + (size_t)mm * j];
+          qtop[i + (size_t)kc * j] = qc[i + (size_t)(mm + 1) * j];
+        }
+        for (int i = 0; i < p; ++i) bot[i + (size_t)p * j] = prc[kc + i + (size_t)mm * j];
+        for (int i = 0; i <= p; ++i) qbot[i + (size_t)(p + 1) * j] = qc[kc + i + (size_t)(mm + 1) * j];
+      }
+      E.combine(Up, n, kc, top.data(), ritz, Un, n, false);
+      E.combine(Vb[0], n, p, bot.data(), ritz, Un, n, true);
+      E.combine(Cp, n, kc, qtop.data(), ritz, Cn, n, false);
+      E.combine(Vb[0], n, p + 1, qbot.data(), ritz, Cn, n, true);
+      std::swap(Up, Un);
+      std::swap(Cp, Cn);
+      kc = ritz;
+    }
+  }
+  // x = x0 + R1 x (pgmresDR.m:257-271)
+  if (right) {
+    op.M(xa, tmp);
+    E.copy(tmp, xa);
+  }
+  if (has_x0) E.update(UPD_AXPBY_, x0, nullptr, nullptr, nullptr, xa, nullptr, C(1), C(1), false);
+  E.copy(xa, x);
+  op.A(x, tmp);
+  const double rr = E.update(UPD_SUB_, b, nullptr, nullptr, nullptr, tmp, nullptr, C(0), C(0), true);
+  R.relres = rr / E.norm(b);
+  R.flag = R.relres <= tol ? 0 : 1;
+  R.outer = outit;
+  R.inner = p;
+  R.resvec = resvec;
+  VS_HIP(hipStreamSynchronize(E.ctx->stream));
+  return R;
+}
+
+template SolveResult solve_pgmresdr<double>(Engine<double>&, const LinOp<double>&, int, const c128*, c128*, bool, int,
+                                            double, int, int);
+template SolveResult solve_pgmresdr<float>(Engine<float>&, const LinOp<float>&, int, const c64*, c64*, bool, int,
+                                           double, int, int);
 
 template SolveResult solve_bicgstab<double>(Engine<double>&, const LinOp<double>&, const c128*, c128*, bool, double, int);
 template SolveResult solve_bicgstab<float>(Engine<float>&, const LinOp<float>&, const c64*, c64*, bool, double, int);

@@ -346,6 +346,57 @@ void Engine<T>::gemv_acc(const cplx<T>* V, int64_t ldv, int k, const c128* h_hos
 }
 
 
+// Y(:, j) (+)= sum_i X(:, i) P(i, j) for up to 8 outputs j at once (P, q x kc column-major, in LDS):
+// one pass over the q basis vectors per chunk of outputs (deflation-space updates of GMRES-DR)
+template <typename T>
+__global__ __launch_bounds__(RED_THREADS) void k_combine(const cplx<T>* __restrict__ X, int64_t ldx, int q,
+                                                         const c128* __restrict__ P, int kc, cplx<T>* Y, int64_t ldy,
+                                                         int64_t n, int accumulate) {
+  extern __shared__ __align__(16) unsigned char vs_comb_smem[];
+  c128* Ps = reinterpret_cast<c128*>(vs_comb_smem);
+  for (int e = threadIdx.x; e < q * kc; e += blockDim.x) Ps[e] = P[e];
+  __syncthreads();
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    c128 acc[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) acc[c] = mk<double>(0, 0);
+    for (int i = 0; i < q; ++i) {
+      const cplx<T> xv = X[e + (int64_t)i * ldx];
+      const c128 x = mk<double>((double)xv.x, (double)xv.y);
+#pragma unroll
+      for (int c = 0; c < 8; ++c)
+        if (c < kc) acc[c] = acc[c] + x * Ps[i + q * c];
+    }
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+      if (c < kc) {
+        cplx<T>* yp = Y + e + (int64_t)c * ldy;
+        c128 v = acc[c];
+        if (accumulate) v = v + mk<double>((double)yp->x, (double)yp->y);
+        *yp = ccast<T>(v);
+      }
+  }
+}
+
+template <typename T>
+void Engine<T>::combine(const cplx<T>* X, int64_t ldx, int q, const c128* P_host, int k, cplx<T>* Y, int64_t ldy,
+                        bool accumulate) {
+  VS_CHECK(q >= 1 && q <= 512 && k >= 0, VS_ERR_UNSUPPORTED, "basis combination of more than 512 vectors");
+  if (k == 0) return;
+  DevBuf dP(sizeof(c128) * (size_t)q * 8);
+  std::vector<c128> chunk((size_t)q * 8);
+  for (int j0 = 0; j0 < k; j0 += 8) {
+    const int kc = std::min(8, k - j0);
+    for (int c = 0; c < kc; ++c)
+      for (int i = 0; i < q; ++i) chunk[i + (size_t)q * c] = P_host[i + (size_t)q * (j0 + c)];
+    VS_HIP(hipMemcpyAsync(dP.p, chunk.data(), sizeof(c128) * (size_t)q * kc, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL((k_combine<T>), dim3(grid), dim3(RED_THREADS), sizeof(c128) * (size_t)q * kc, ctx->stream, X,
+                       ldx, q, dP.as<c128>(), kc, Y + (int64_t)j0 * ldy, ldy, n, accumulate ? 1 : 0);
+    VS_HIP(hipGetLastError());
+    VS_HIP(hipStreamSynchronize(ctx->stream));  // the host chunk and dP are reused by the next chunk
+  }
+}
+
 // ------------------------------------------------------------------------------------------------
 // device-resident BiCGStab scalars: every scalar of an iteration is formed on the device, so a whole
 // iteration runs without host round trips (and can be replayed as one captured graph)

@@ -29,6 +29,9 @@ struct Engine {
   void norm2_dev(const cplx<T>* x, c128* out_dev);
   void scale_inv_norm_dev(const cplx<T>* w, cplx<T>* y, const c128* nrm2_dev);
   void read(const c128* dev, int K, c128* host);
+  // Y(:, j) (+)= sum_i X(:, i) P(i, j), P q x k column-major on the host, columns of X / Y at stride ldx / ldy
+  void combine(const cplx<T>* X, int64_t ldx, int q, const c128* P_host, int k, cplx<T>* Y, int64_t ldy,
+               bool accumulate);
 };
 
 // device-resident scalars of one BiCGStab iteration (read back once per iteration)
@@ -109,6 +112,10 @@ SolveResult solve_gmres(Engine<T>& E, const LinOp<T>& op, const cplx<T>* b, cplx
 template <typename T>
 SolveResult solve_pgmres(Engine<T>& E, const LinOp<T>& op, int side, const cplx<T>* b, cplx<T>* x, bool has_x0,
                          int restart, double tol, int maxit);
+
+template <typename T>
+SolveResult solve_pgmresdr(Engine<T>& E, const LinOp<T>& op, int side, const cplx<T>* b, cplx<T>* x, bool has_x0,
+                           int restart, double tol, int maxit, int ritz);
 
 SolveResult solve_refine(Engine<double>& E64, Engine<float>& E32, const LinOp<double>& A64, const LinOp<float>& A32,
                          const c128* b, c128* x, bool has_x0, double tol, int max_outer, int inner_solver,
