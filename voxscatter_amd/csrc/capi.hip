@@ -6,6 +6,8 @@
 #include "handles.hpp"
 #include "solver.hpp"
 
+#include <cstdlib>
+
 namespace vs {
 void op_build_dda(vs_op* op, const double* r_host, double ko, int nearby);
 void op_build_fN(vs_op* op);
@@ -122,6 +124,10 @@ int vs_ctx_create(int device, vs_ctx** out) {
     VS_CHECK(e == hipSuccess && c > 0, VS_ERR_NODEV, "no HIP device available");
     VS_CHECK(device >= 0 && device < c, VS_ERR_NODEV, "device index out of range");
     VS_HIP(hipSetDevice(device));
+    // the Krylov loops wait on the stream once per iteration: spin instead of yielding so the wait ends
+    // as soon as the work does (a device whose runtime is already active keeps its flags)
+    const char* sp = std::getenv("VS_SPIN");
+    if (!(sp && sp[0] == '0') && hipSetDeviceFlags(hipDeviceScheduleSpin) != hipSuccess) (void)hipGetLastError();
     auto ctx = std::make_unique<vs_ctx>();
     ctx->device = device;
     VS_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
