@@ -10,13 +10,54 @@
 
 namespace vs {
 
+// n-vectors carved from one of the context's reusable workspaces
 template <typename T>
 struct Vecs {
-  DevBuf buf;
+  cplx<T>* base;
   int64_t n;
-  int count;
-  Vecs(int64_t n_, int cnt) : buf(sizeof(cplx<T>) * n_ * cnt), n(n_), count(cnt) {}
-  cplx<T>* operator[](int i) { return buf.as<cplx<T>>() + (int64_t)i * n; }
+  Vecs(vs_ctx* ctx, int slot, int64_t n_, int cnt)
+      : base(static_cast<cplx<T>*>(ctx->workspace(slot, sizeof(cplx<T>) * (size_t)n_ * cnt))), n(n_) {}
+  Vecs(cplx<T>* b, int64_t n_) : base(b), n(n_) {}
+  cplx<T>* operator[](int i) { return base + (int64_t)i * n; }
+  Vecs sub(int first) { return Vecs(base + (int64_t)first * n, n); }
+};
+
+// a piece of a workspace
+struct Span {
+  void* p;
+  template <typename U> U* as() const { return static_cast<U*>(p); }
+};
+
+// consecutive 256-byte aligned pieces of one workspace: sizes first, then pointers
+struct Carve {
+  std::vector<size_t> sz;
+  size_t add(size_t bytes) {
+    sz.push_back(bytes);
+    return sz.size() - 1;
+  }
+  char* base = nullptr;
+  void take(vs_ctx* ctx, int slot) {
+    size_t tot = 0;
+    for (size_t b : sz) tot += (b + 255) & ~size_t(255);
+    base = static_cast<char*>(ctx->workspace(slot, tot));
+  }
+  Span at(size_t i) const {
+    size_t off = 0;
+    for (size_t j = 0; j < i; ++j) off += (sz[j] + 255) & ~size_t(255);
+    return Span{base + off};
+  }
+};
+
+// events of the pipelined BiCGStab, destroyed on every exit path
+struct EventPair {
+  hipEvent_t e[2] = {nullptr, nullptr};
+  EventPair() {
+    for (auto& x : e) VS_HIP(hipEventCreateWithFlags(&x, hipEventDisableTiming));
+  }
+  ~EventPair() {
+    for (auto& x : e)
+      if (x) (void)hipEventDestroy(x);
+  }
 };
 
 static inline c128 C(double r, double i = 0) { return mk<double>(r, i); }
@@ -30,12 +71,16 @@ static double eps_of() {
 }
 
 // MATLAB bicgstab semantics (the restatement of the built-in documented alongside the oracle), with
-// the scalars of each iteration formed on the device.  Fast path: both half steps are launched back to
-// back (replayed as one captured graph from the second iteration on) and the host reads the scalars
-// once per iteration, then replays MATLAB's control logic on them.  Whenever that logic takes the
-// half-step check branch (a recomputed true residual replaces s), the second half step is recomputed
-// from the true residual, exactly as MATLAB orders it; once a check has failed (moresteps > 0) every
-// half step is checked and the iterations run with one read-back per half step.
+// the scalars of each iteration formed on the device.  Fast path (no pending true-residual checks): both
+// half steps of iteration i are launched back to back, a copy of the scalars and an event follow, and
+// the first half step of iteration i+1 is launched before the host waits for that event, so the device
+// never idles while the host replays MATLAB's control logic on iteration i's scalars.  The vectors that
+// half step writes (p, v, xhalf, s) are kept per iteration parity, so iteration i's are intact; when the
+// logic leaves the fast path (a true-residual check, a breakdown, stagnation) the speculative half step
+// is cancelled by restoring the scalars read for iteration i.  Whenever the logic takes the half-step
+// check branch (a recomputed true residual replaces s), the second half step is recomputed from it,
+// exactly as MATLAB orders it; once a check has failed (moresteps > 0) every half step is checked and
+// read back on its own.
 template <typename T>
 SolveResult solve_bicgstab(Engine<T>& E, const LinOp<T>& op, const cplx<T>* b, cplx<T>* x, bool has_x0, double tol,
                            int maxit) {
@@ -54,11 +99,13 @@ SolveResult solve_bicgstab(Engine<T>& E, const LinOp<T>& op, const cplx<T>* b, c
     return R;
   }
   if (!has_x0) VS_HIP(hipMemsetAsync(x, 0, sizeof(cv) * n, ctx->stream));
-  Vecs<T> W(n, 10);
-  cv *r = W[0], *rt = W[1], *p = W[2], *v = W[3], *ph = W[4], *s = W[5], *sh = W[6], *t = W[7], *xh = W[8],
-     *xmin = W[9];
-  const cv* phv = op.M ? ph : p;
-  const cv* shv = op.M ? sh : s;
+  Vecs<T> W(ctx, vs_ctx::WS_KRYLOV, n, 14);
+  cv *r = W[0], *rt = W[1], *ph = W[2], *sh = W[3], *t = W[4], *xmin = W[5];
+  // p, v, xhalf, s of iteration i live in slot i & 1
+  auto P = [&](int i) { return W[6 + (i & 1)]; };
+  auto V = [&](int i) { return W[8 + (i & 1)]; };
+  auto XH = [&](int i) { return W[10 + (i & 1)]; };
+  auto S = [&](int i) { return W[12 + (i & 1)]; };
   int flag = 1;
   E.copy(x, xmin);
   double imin = 0;
@@ -84,9 +131,9 @@ SolveResult solve_bicgstab(Engine<T>& E, const LinOp<T>& op, const cplx<T>* b, c
   double iter = 0;
   int ii = 0;
 
-  DevBuf dsc(sizeof(BiScal));
-  BiScal* sc = dsc.as<BiScal>();
-  BiScal* hs = reinterpret_cast<BiScal*>(ctx->pinned);
+  BiScal* sc = static_cast<BiScal*>(ctx->workspace(vs_ctx::WS_KRYLOV_AUX, sizeof(BiScal)));
+  BiScal* hs = reinterpret_cast<BiScal*>(ctx->pinned);  // two read-back slots (iteration parity)
+  BiScal* up = reinterpret_cast<BiScal*>(ctx->pinned_up);  // [0] initial scalars, [1] restore staging
   BiScal h{};
   h.rho = C(1);
   h.rho1 = C(1);
@@ -95,83 +142,69 @@ SolveResult solve_bicgstab(Engine<T>& E, const LinOp<T>& op, const cplx<T>* b, c
   h.beta = C(0);
   h.first = 1;
   h.has_pc = op.M ? 1 : 0;
-  *reinterpret_cast<BiScal*>(ctx->pinned_up) = h;
-  VS_HIP(hipMemcpyAsync(sc, ctx->pinned_up, sizeof(BiScal), hipMemcpyHostToDevice, ctx->stream));
+  up[0] = h;
+  VS_HIP(hipMemcpyAsync(sc, up, sizeof(BiScal), hipMemcpyHostToDevice, ctx->stream));
+  bi_stage_init<T>(E, rt, r, sc);
+  EventPair ev;
 
-  auto first_half = [&] {
-    bi_stage_rho<T>(E, rt, r, v, p, sc);
-    if (op.M) op.M(p, ph);
-    op.A(phv, v);
-    bi_stage_alpha_half<T>(E, rt, v, phv, x, r, xh, s, sc);
+  auto first_half = [&](int i, bool finish_half) {
+    cv* pi = P(i);
+    bi_stage_rho<T>(E, r, V(i - 1), P(i - 1), pi, sc);
+    const cv* phv = op.M ? ph : pi;
+    if (op.M) op.M(pi, ph);
+    op.A(phv, V(i));
+    bi_stage_alpha_half<T>(E, rt, V(i), phv, x, r, XH(i), S(i), sc, finish_half);
   };
-  auto second_half = [&] {
-    if (op.M) op.M(s, sh);
+  // fold_half: the half step's ||s|| is reduced together with the omega stage
+  auto second_half = [&](int i, bool fold_half) {
+    const cv* shv = op.M ? sh : S(i);
+    if (op.M) op.M(S(i), sh);
     op.A(shv, t);
-    bi_stage_omega_full<T>(E, t, s, shv, xh, x, r, sc);
+    bi_stage_omega_full<T>(E, t, S(i), shv, XH(i), rt, x, r, sc, fold_half);
   };
-  auto readback = [&] {
-    VS_HIP(hipMemcpyAsync(hs, sc, sizeof(BiScal), hipMemcpyDeviceToHost, ctx->stream));
-    VS_HIP(hipStreamSynchronize(ctx->stream));
-    h = *hs;
+  auto snapshot = [&](int i) {
+    VS_HIP(hipMemcpyAsync(hs + (i & 1), sc, sizeof(BiScal), hipMemcpyDeviceToHost, ctx->stream));
+    VS_HIP(hipEventRecord(ev.e[i & 1], ctx->stream));
+  };
+  auto wait = [&](int i) {
+    VS_HIP(hipEventSynchronize(ev.e[i & 1]));
+    h = hs[i & 1];
+  };
+  bool spec = false;  // the first half step of iteration ii + 1 is in flight
+  auto cancel_spec = [&] {
+    if (!spec) return;
+    up[1] = h;  // the previous restore has completed: every wait on the stream since came after it
+    VS_HIP(hipMemcpyAsync(sc, up + 1, sizeof(BiScal), hipMemcpyHostToDevice, ctx->stream));
+    spec = false;
   };
   auto clear_flag = [&] {
     // a breakdown seen only by a speculative second half step must not block its recomputation
     VS_HIP(hipStreamSynchronize(ctx->stream));
-    *reinterpret_cast<int*>(ctx->pinned_up) = 0;
-    VS_HIP(hipMemcpyAsync(&sc->flag, ctx->pinned_up, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    *reinterpret_cast<int*>(up) = 0;
+    VS_HIP(hipMemcpyAsync(&sc->flag, up, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
   };
 
-  const char* ge = getenv("VS_GRAPH");
-  bool use_graph = !(ge && ge[0] == '0');
-  hipGraphExec_t gexec = nullptr;
-  hipGraph_t graph = nullptr;
-
   for (ii = 1; ii <= maxit; ++ii) {
-    const bool fast = moresteps == 0;
     bool second_done = false;
-    if (fast) {
-      if (use_graph && ii >= 2 && !gexec) {
-        // capture one full iteration (both half steps and the scalar read-back)
-        if (hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
-          bool ok = true;
-          try {
-            first_half();
-            second_half();
-            VS_HIP(hipMemcpyAsync(hs, sc, sizeof(BiScal), hipMemcpyDeviceToHost, ctx->stream));
-          } catch (const Error&) {
-            ok = false;
-          }
-          hipGraph_t g = nullptr;
-          const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
-          if (ok && ec == hipSuccess && g && hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0) == hipSuccess) {
-            graph = g;
-          } else {
-            if (g) hipGraphDestroy(g);
-            gexec = nullptr;
-            use_graph = false;
-            (void)hipGetLastError();
-          }
-        } else {
-          use_graph = false;
-          (void)hipGetLastError();
-        }
+    if (moresteps == 0) {
+      if (!spec) first_half(ii, false);
+      second_half(ii, true);
+      snapshot(ii);
+      spec = false;
+      if (ii < maxit) {
+        first_half(ii + 1, false);
+        spec = true;
       }
-      if (gexec) {
-        VS_HIP(hipGraphLaunch(gexec, ctx->stream));
-        VS_HIP(hipStreamSynchronize(ctx->stream));
-        h = *hs;
-      } else {
-        first_half();
-        second_half();
-        readback();
-      }
+      wait(ii);
       second_done = true;
     } else {
-      first_half();
-      readback();
+      first_half(ii, true);
+      snapshot(ii);
+      wait(ii);
     }
     // ---- first half step: rho, beta, alpha (breakdowns before the half step) ----
     if (h.flag && h.brk <= 4) {
+      cancel_spec();
       flag = h.flag;
       nres = 2 * ii - 1;
       break;
@@ -184,11 +217,12 @@ SolveResult solve_bicgstab(Engine<T>& E, const LinOp<T>& op, const cplx<T>* b, c
     normr_act = normr;
     resvec[2 * ii - 1] = normr;
     if (normr <= tolb || stag >= maxstagsteps || moresteps) {
-      op.A(xh, s);
-      normr_act = E.update(UPD_SUB_, b, nullptr, nullptr, nullptr, s, nullptr, C(0), C(0), true);
+      cancel_spec();
+      op.A(XH(ii), S(ii));
+      normr_act = E.update(UPD_SUB_, b, nullptr, nullptr, nullptr, S(ii), nullptr, C(0), C(0), true);
       resvec[2 * ii - 1] = normr_act;
       if (normr_act <= tolb) {
-        E.copy(xh, x);
+        E.copy(XH(ii), x);
         flag = 0;
         iter = ii - 0.5;
         nres = 2 * ii;
@@ -198,7 +232,7 @@ SolveResult solve_bicgstab(Engine<T>& E, const LinOp<T>& op, const cplx<T>* b, c
         ++moresteps;
         if (moresteps >= maxmsteps) {
           flag = 3;
-          E.copy(xh, x);
+          E.copy(XH(ii), x);
           nres = 2 * ii;
           break;
         }
@@ -207,16 +241,18 @@ SolveResult solve_bicgstab(Engine<T>& E, const LinOp<T>& op, const cplx<T>* b, c
     }
     if (normr_act < normrmin) {
       normrmin = normr_act;
-      E.copy(xh, xmin);
+      E.copy(XH(ii), xmin);
       imin = ii - 0.5;
     }
     if (!second_done) {
       if (h.flag) clear_flag();
-      second_half();
-      readback();
+      second_half(ii, false);
+      snapshot(ii);
+      wait(ii);
     }
     // ---- second half step: omega ----
     if (h.flag) {
+      cancel_spec();
       flag = h.flag;
       nres = 2 * ii;
       break;
@@ -229,6 +265,7 @@ SolveResult solve_bicgstab(Engine<T>& E, const LinOp<T>& op, const cplx<T>* b, c
     normr_act = normr;
     resvec[2 * ii] = normr;
     if (normr <= tolb || stag >= maxstagsteps || moresteps) {
+      cancel_spec();
       op.A(x, r);
       normr_act = E.update(UPD_SUB_, b, nullptr, nullptr, nullptr, r, nullptr, C(0), C(0), true);
       resvec[2 * ii] = normr_act;
@@ -246,6 +283,7 @@ SolveResult solve_bicgstab(Engine<T>& E, const LinOp<T>& op, const cplx<T>* b, c
           break;
         }
       }
+      bi_stage_init<T>(E, rt, r, sc);  // the next rho comes from the recomputed residual
     }
     if (normr_act < normrmin) {
       normrmin = normr_act;
@@ -253,13 +291,12 @@ SolveResult solve_bicgstab(Engine<T>& E, const LinOp<T>& op, const cplx<T>* b, c
       imin = ii;
     }
     if (stag >= maxstagsteps) {
+      cancel_spec();
       flag = 3;
       nres = 2 * ii + 1;
       break;
     }
   }
-  if (gexec) hipGraphExecDestroy(gexec);
-  if (graph) hipGraphDestroy(graph);
   if (ii > maxit) ii = maxit;
   if (nres < 0) nres = 2 * (int64_t)maxit + 1;
   double relres;
@@ -325,9 +362,9 @@ SolveResult solve_gmres(Engine<T>& E, const LinOp<T>& op, const cplx<T>* b, cplx
     return R;
   }
   if (!has_x0) VS_HIP(hipMemsetAsync(x, 0, sizeof(cv) * n, E.ctx->stream));
-  Vecs<T> Vb(n, restart + 1);
-  Vecs<T> W(n, 3);
-  DevBuf hdev(sizeof(c128) * 1025);
+  Vecs<T> Vb(E.ctx, vs_ctx::WS_KRYLOV, n, restart + 4);
+  Vecs<T> W = Vb.sub(restart + 1);
+  c128* hdev = static_cast<c128*>(E.ctx->workspace(vs_ctx::WS_KRYLOV_AUX, sizeof(c128) * 1025));
   cv *r = W[0], *w = W[1], *tmp = W[2];
   auto precond = [&](const cv* in, cv* out) {
     if (op.M)
@@ -362,7 +399,7 @@ SolveResult solve_gmres(Engine<T>& E, const LinOp<T>& op, const cplx<T>* b, cplx
       op.A(Vb[k], tmp);
       precond(tmp, w);
       // CGS2 with the projection coefficients kept on the device; one read-back per Arnoldi step
-      c128* h1d = hdev.as<c128>();
+      c128* h1d = hdev;
       c128* h2d = h1d + (k + 1);
       c128* nrd = h2d + (k + 1);
       E.mdot_dev(Vb[0], n, k + 1, w, h1d);
@@ -426,8 +463,8 @@ SolveResult solve_pgmres(Engine<T>& E, const LinOp<T>& op, int side, const cplx<
   SolveResult R;
   VS_CHECK(restart >= 1 && restart <= 511, VS_ERR_UNSUPPORTED, "pgmres restart must be in 1..511");
   const bool left = op.M && side == 0, right = op.M && side == 1;
-  Vecs<T> Vb(n, restart + 1);
-  Vecs<T> W(n, 3);
+  Vecs<T> Vb(E.ctx, vs_ctx::WS_KRYLOV, n, restart + 4);
+  Vecs<T> W = Vb.sub(restart + 1);
   cv *r = W[0], *w = W[1], *tmp = W[2];
   if (!has_x0) VS_HIP(hipMemsetAsync(x, 0, sizeof(cv) * n, E.ctx->stream));
   // r = b - A x0 (only if x0 is nonzero), bp = b; left preconditioning of both
@@ -621,7 +658,9 @@ SolveResult solve_pgmresdr(Engine<T>& E, const LinOp<T>& op, int side, const cpl
   ritz = ritz <= 0 ? std::min(10, (int)std::lround(restart / 2.0)) : std::min(ritz, restart - 1);
   const bool left = op.M && side == 0, right = op.M && side == 1;
   const int m = restart;
-  Vecs<T> Vb(n, m + 1), Cb(n, ritz), Ub(n, ritz), C2(n, ritz), U2(n, ritz), W(n, 5);
+  Vecs<T> Vb(E.ctx, vs_ctx::WS_KRYLOV, n, m + 1 + 4 * ritz + 5);
+  Vecs<T> Cb = Vb.sub(m + 1), Ub = Vb.sub(m + 1 + ritz), C2 = Vb.sub(m + 1 + 2 * ritz),
+          U2 = Vb.sub(m + 1 + 3 * ritz), W = Vb.sub(m + 1 + 4 * ritz);
   cv *r = W[0], *w = W[1], *tmp = W[2], *xa = W[3], *x0 = W[4];
   cv *Cp = Cb[0], *Up = Ub[0], *Cn = C2[0], *Un = U2[0];
   // r = b - A x0, bp = b, both left-preconditioned (pgmresDR.m:110-139)
@@ -858,7 +897,11 @@ SolveResult solve_refine(Engine<double>& E64, Engine<float>& E32, const LinOp<do
     R.resvec = {0.0};
     return R;
   }
-  DevBuf r64(sizeof(c128) * n), r32(sizeof(c64) * n), d32(sizeof(c64) * n), d64(sizeof(c128) * n);
+  Carve ws;
+  const size_t i64a = ws.add(sizeof(c128) * n), i32a = ws.add(sizeof(c64) * n), i32b = ws.add(sizeof(c64) * n),
+               i64b = ws.add(sizeof(c128) * n);
+  ws.take(ctx, vs_ctx::WS_REFINE);
+  const Span r64 = ws.at(i64a), r32 = ws.at(i32a), d32 = ws.at(i32b), d64 = ws.at(i64b);
   int tot = 0;
   double relres = 0;
   R.flag = 1;
@@ -922,11 +965,15 @@ BatchResult solve_bicgstab_batch(Engine<T>& E, const LinOp<T>& op, int k, const 
   R.flag.assign(k, 1);
   R.relres.assign(k, 0.0);
   R.iter.assign(k, 0.0);
-  Vecs<T> W(n * k, 9);
+  Vecs<T> W(ctx, vs_ctx::WS_KRYLOV, n * k, 9);
   cv *r = W[0], *rt = W[1], *p = W[2], *v = W[3], *ph = W[4], *s = W[5], *sh = W[6], *t = W[7], *xh = W[8];
   const cv* phv = op.Mb ? ph : p;
   const cv* shv = op.Mb ? sh : s;
-  DevBuf part(sizeof(c128) * (size_t)E.grid * k * 4), nrm(sizeof(c128) * k), dsc(sizeof(BiScal) * k);
+  Carve ws;
+  const size_t ip = ws.add(sizeof(c128) * (size_t)E.grid * k * 4), in_ = ws.add(sizeof(c128) * k),
+               isc = ws.add(sizeof(BiScal) * k);
+  ws.take(ctx, vs_ctx::WS_KRYLOV_AUX);
+  const Span part = ws.at(ip), nrm = ws.at(in_), dsc = ws.at(isc);
   BiScal* sc = dsc.as<BiScal>();
   VS_HIP(hipMemsetAsync(X, 0, sizeof(cv) * n * k, ctx->stream));
   batch_norm2<T>(E, k, B, part.as<c128>(), nrm.as<c128>());
@@ -1075,8 +1122,16 @@ BatchResult solve_refine_batch(Engine<double>& E64, Engine<float>& E32, const Li
   R.flag.assign(k, 1);
   R.relres.assign(k, 0.0);
   R.iter.assign(k, 0.0);
-  DevBuf R64(sizeof(c128) * n * k), R32(sizeof(c64) * n * k), D32(sizeof(c64) * n * k), D64(sizeof(c128) * n * k);
-  DevBuf part(sizeof(c128) * (size_t)E64.grid * k), nrm(sizeof(c128) * k), coef(sizeof(double) * k);
+  Carve ws;
+  const size_t i64a = ws.add(sizeof(c128) * n * k), i32a = ws.add(sizeof(c64) * n * k),
+               i32b = ws.add(sizeof(c64) * n * k), i64b = ws.add(sizeof(c128) * n * k);
+  ws.take(ctx, vs_ctx::WS_REFINE);
+  const Span R64 = ws.at(i64a), R32 = ws.at(i32a), D32 = ws.at(i32b), D64 = ws.at(i64b);
+  Carve wa;
+  const size_t ip = wa.add(sizeof(c128) * (size_t)E64.grid * k), in_ = wa.add(sizeof(c128) * k),
+               ic = wa.add(sizeof(double) * k);
+  wa.take(ctx, vs_ctx::WS_REFINE_AUX);
+  const Span part = wa.at(ip), nrm = wa.at(in_), coef = wa.at(ic);
   VS_HIP(hipMemsetAsync(X, 0, sizeof(c128) * n * k, ctx->stream));
   batch_norm2<double>(E64, k, B, part.as<c128>(), nrm.as<c128>());
   std::vector<c128> h(k);

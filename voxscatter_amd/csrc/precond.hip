@@ -11,6 +11,7 @@
 #include "handles.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 
 namespace vs {
 
@@ -461,17 +462,20 @@ __global__ __launch_bounds__(256) void k_blockmv2_sym(const cplx<T>* __restrict_
 // (c(L-i) = -c(i) for the xy and xz vectors), those of a component odd in y under ky -> M - ky (xy, yz),
 // so the blocks obey B(L-kx, ky) = Dx B(kx, ky) Dx and B(kx, M-ky) = Dy B(kx, ky) Dy with Dx (Dy) = -1
 // on the x (y) unknowns; the same holds for the inverses.  Only kx <= L/2, ky <= M/2 are stored (grid
-// Lb x Mb), and one workgroup applies a tile of W stored blocks to the up to four points each stands for:
-// variant q (bit 0: x mirror, bit 1: y mirror) of right-hand side rc is u[(4 rc + q)][s][w].  The rows of
-// a tile are split over RS sibling workgroups (out of place: U -> U2); siblings get consecutive logical
-// ids, and logical ids are laid out so that consecutive ones land on the same XCD (hardware dispatch
-// sends workgroup i to XCD i mod 8), so the siblings' reads of U share that XCD's L2.
+// Lb x Mb), and one workgroup applies a tile of W = 2^lw stored blocks to the up to four points each
+// stands for: variant q (bit 0: x mirror, bit 1: y mirror) of right-hand side rc is u[(4 rc + q)][s][w].
+// The rows of a tile are split over RS sibling workgroups (out of place: U -> U2); siblings get
+// consecutive logical ids, and logical ids are laid out so that consecutive ones land on the same XCD
+// (hardware dispatch sends workgroup i to XCD i mod 8), so the siblings' reads of U share that XCD's L2.
+// Index arithmetic is 32-bit within a block and the gather runs without integer division.
 template <typename T, bool SYM, int RC>
 __global__ __launch_bounds__(512) void k_blockmv2m(const cplx<T>* __restrict__ B, const cplx<T>* __restrict__ U,
                                                    cplx<T>* __restrict__ U2, int L, int M, int Lb, int Mb, int n,
-                                                   int Wt, int W, int RS, int64_t rhs_stride, int nrhs) {
+                                                   int Wt, int lw, int RS, int64_t rhs_stride, int nrhs) {
   constexpr int NV = 4 * RC;
+  constexpr int BU = 8;
   cplx<T>* u = reinterpret_cast<cplx<T>*>(vs_smem);
+  const int W = 1 << lw;
   const int tiles = (Lb + Wt - 1) / Wt, subs = Wt / W;
   const int64_t G = (int64_t)tiles * subs * Mb * RS;
   const int64_t per_xcd = (G + 7) / 8;
@@ -491,47 +495,43 @@ __global__ __launch_bounds__(512) void k_blockmv2m(const cplx<T>* __restrict__ B
   const bool ymir = kyb != 0 && 2 * kyb != M;
   const cplx<T>* Ui = U + (int64_t)r0 * rhs_stride;
   cplx<T>* Uo = U2 + (int64_t)r0 * rhs_stride;
-  // branch-free gather (clamped addresses, sign 0 for dead entries) so that the loads of a batch are
-  // all in flight before the first use
-  const int tot = NV * n * W;
-  for (int e0 = threadIdx.x; e0 < tot; e0 += 4 * blockDim.x) {
-    cplx<T> val[4];
-    T sg[4];
+  {
+    // gather: lanes over the W columns, rows over s; the NV loads of a row are issued together
+    // (clamped addresses, sign 0 for dead entries)
+    const int w = threadIdx.x & (W - 1);
+    const int kx = kx0 + w;
+    const bool wl = w < wv;
+    const bool xl = kx != 0 && 2 * kx != L;
+    for (int s = threadIdx.x >> lw; s < n; s += blockDim.x >> lw) {
+      const int comp = (s >= N) + (s >= 2 * N);
+      cplx<T> val[NV];
+      T sg[NV];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int e = min(e0 + j * (int)blockDim.x, tot - 1);
-      const int w = e % W, q = e / W;
-      const int s = q % n, v = q / n;
-      const int rc = v >> 2, xm = v & 1, ym = (v >> 1) & 1;
-      const int kx = kx0 + w;
-      const bool live = e0 + j * (int)blockDim.x < tot && w < wv && rc < nr &&
-                        (!xm || (kx != 0 && 2 * kx != L)) && (!ym || ymir);
-      const int gx = live ? (xm ? L - kx : kx) : 0, gy = live ? (ym ? M - kyb : kyb) : 0;
-      const int comp = s / N;
-      val[j] = Ui[live ? rc * rhs_stride + s * sstride + gx + (int64_t)L * gy : 0];
-      sg[j] = live ? (((xm && comp == 0) || (ym && comp == 1)) ? (T)-1 : (T)1) : (T)0;
-    }
+      for (int v = 0; v < NV; ++v) {
+        const int rc = v >> 2, xm = v & 1, ym = (v >> 1) & 1;
+        const bool live = wl && rc < nr && (!xm || xl) && (!ym || ymir);
+        const int gx = live ? (xm ? L - kx : kx) : 0, gy = live ? (ym ? M - kyb : kyb) : 0;
+        val[v] = Ui[live ? rc * rhs_stride + s * sstride + gx + (int64_t)L * gy : 0];
+        sg[v] = live ? (((xm && comp == 0) || (ym && comp == 1)) ? (T)-1 : (T)1) : (T)0;
+      }
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int e = e0 + j * (int)blockDim.x;
-      if (e < tot) u[e] = sg[j] * val[j];
+      for (int v = 0; v < NV; ++v) u[(v * n + s) * W + w] = sg[v] * val[v];
     }
   }
   __syncthreads();
-  const int64_t NT = SYM ? (int64_t)n * (n + 1) / 2 : (int64_t)n * n;
+  const int NT = SYM ? n * (n + 1) / 2 : n * n;
   const int nz0 = 2 * N;
   const cplx<T>* Bk = B + ((int64_t)tile * Mb + kyb) * NT * Wt + sub * W;
   const int rows = (n + RS - 1) / RS;
   const int rlo = rsi * rows, rhi = min(n, rlo + rows);
   for (int e = threadIdx.x; e < (rhi - rlo) * W; e += blockDim.x) {
-    const int w = e % W, r = rlo + e / W;
+    const int w = e & (W - 1);
+    const int r = rlo + (e >> lw);
     if (w >= wv) continue;
     cplx<T> acc[NV];
 #pragma unroll
     for (int c = 0; c < NV; ++c) acc[c] = mk<T>(0, 0);
-    // batches of BU independent block loads in flight per lane
-    constexpr int BU = 8;
-    const int64_t t0 = SYM ? (int64_t)r * (2 * n - r + 1) / 2 : (int64_t)r * n;
+    const int t0 = SYM ? r * (2 * n - r + 1) / 2 : r * n;
     const bool rz = r >= nz0;
     for (int s0 = 0; s0 < n; s0 += BU) {
       cplx<T> b[BU];
@@ -539,17 +539,17 @@ __global__ __launch_bounds__(512) void k_blockmv2m(const cplx<T>* __restrict__ B
 #pragma unroll
       for (int j = 0; j < BU; ++j) {
         const int s = min(s0 + j, n - 1);
-        int64_t t;
+        int t;
         T g = (T)1;
         if (SYM) {
           // s < r: transposed entry (s, r) with the z sign pattern; s >= r: row r of the upper triangle
           const bool lo = s < r;
-          t = lo ? (int64_t)s * (2 * n - s + 1) / 2 + (r - s) : t0 + (s - r);
+          t = lo ? s * (2 * n - s + 1) / 2 + (r - s) : t0 + (s - r);
           if (lo && rz != (s >= nz0)) g = (T)-1;
         } else {
           t = t0 + s;
         }
-        b[j] = Bk[t * Wt + w];
+        b[j] = Bk[(int64_t)t * Wt + w];
         sg[j] = s0 + j < n ? g : (T)0;
       }
 #pragma unroll
@@ -702,6 +702,15 @@ void pc_store_inverse(vs_pc* pc, const c128* inv) {
   if (pc->precisions & VS_FP32) go(float(0), pc->inv32);
 }
 
+// forced row split of the mirrored block product (a tuning knob; 0 = automatic)
+static int bmm_rows_split() {
+  static const int v = [] {
+    const char* e = std::getenv("VS_BMM_RS");
+    return e ? std::max(0, std::atoi(e)) : 0;
+  }();
+  return v;
+}
+
 template <typename T>
 static void pc_apply_impl(vs_pc* pc, const cplx<T>* x, cplx<T>* y, int nrhs) {
   vs_ctx* ctx = pc->ctx;
@@ -726,15 +735,16 @@ static void pc_apply_impl(vs_pc* pc, const cplx<T>* x, cplx<T>* y, int nrhs) {
       int RC = nrhs >= 4 ? 4 : (nrhs >= 2 ? 2 : 1);
       while (RC > 1 && !fits(RC, std::min(8, Lb))) RC /= 2;
       VS_CHECK(fits(RC, 1), VS_ERR_UNSUPPORTED, "2-level block too large for LDS");
-      // workgroups cover W columns of a Wt-wide layout tile (W | Wt); rows are split over sibling workgroups
-      // only for small grids
+      // workgroups cover W columns of a Wt-wide layout tile (W | Wt, both powers of two); rows are split over
+      // RS sibling workgroups of about 256 lanes (measured best on 48-row blocks in both precisions)
       const int Wt = mirror_tile<T>();
-      int W = Wt;
-      while (W > 1 && !fits(RC, W)) W /= 2;
+      int lw = 0;
+      while ((2 << lw) <= Wt && fits(RC, 2 << lw)) ++lw;
+      const int W = 1 << lw;
       const int tiles = (Lb + Wt - 1) / Wt, subs = Wt / W;
       const int chunks = (nrhs + RC - 1) / RC;
-      const int64_t base = (int64_t)tiles * subs * Mb * chunks;
-      int RS = base >= 512 ? 1 : (int)std::min<int64_t>((512 + base - 1) / base, std::max(1, n / 4));
+      const int rows_wg = std::max(1, 256 / W);
+      int RS = bmm_rows_split() > 0 ? std::min(bmm_rows_split(), n) : (n + rows_wg - 1) / rows_wg;
       const int rows = (n + RS - 1) / RS;
       RS = (n + rows - 1) / rows;
       const size_t lds = sizeof(cplx<T>) * (size_t)4 * RC * n * W;
@@ -745,7 +755,7 @@ static void pc_apply_impl(vs_pc* pc, const cplx<T>* x, cplx<T>* y, int nrhs) {
       cplx<T>* U2 = pc->U2.as<cplx<T>>();
 #define VS_BMM(SYMV, RCV)                                                                                 \
   hipLaunchKernelGGL((k_blockmv2m<T, SYMV, RCV>), grid, dim3(threads), lds, ctx->stream, B, U, U2, L, M, Lb, Mb, n, \
-                     Wt, W, RS, V3, nrhs)
+                     Wt, lw, RS, V3, nrhs)
       if (pc->sym) {
         if (RC == 4) VS_BMM(true, 4); else if (RC == 2) VS_BMM(true, 2); else VS_BMM(true, 1);
       } else {

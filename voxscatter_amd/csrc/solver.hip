@@ -405,46 +405,35 @@ __device__ __forceinline__ bool cz(c128 a) { return a.x == 0.0 && a.y == 0.0; }
 __device__ __forceinline__ bool cinf(c128 a) { return isinf(a.x) || isinf(a.y); }
 __device__ __forceinline__ bool cfin(c128 a) { return isfinite(a.x) && isfinite(a.y); }
 
-enum BiStage : int { BI_RHO = 0, BI_ALPHA = 1, BI_HALF = 2, BI_OMEGA = 3, BI_FULL = 4 };
+// BI_RHO: the batched kernels form rho in a stage of its own
+enum BiStage : int { BI_INIT = 0, BI_ALPHA = 1, BI_HALF = 2, BI_OMEGA = 3, BI_FULL = 4, BI_RHO = 5 };
 
-// reduce the partials of one stage (fixed order) and apply that stage's scalar logic
-__global__ __launch_bounds__(RED_THREADS) void k_bi_scalar(const c128* partial, int nblk, int K, int stage,
-                                                           BiScal* sc) {
-  __shared__ double sre[RED_THREADS], sim[RED_THREADS];
-  __shared__ c128 tot[4];
-  for (int k = 0; k < K; ++k) {
-    double a = 0, b = 0;
-    for (int i = threadIdx.x; i < nblk; i += RED_THREADS) {
-      a += partial[i * K + k].x;
-      b += partial[i * K + k].y;
-    }
-    sre[threadIdx.x] = a;
-    sim[threadIdx.x] = b;
-    __syncthreads();
-    for (int st = RED_THREADS / 2; st > 0; st >>= 1) {
-      if (threadIdx.x < st) {
-        sre[threadIdx.x] += sre[threadIdx.x + st];
-        sim[threadIdx.x] += sim[threadIdx.x + st];
-      }
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) tot[k] = mk<double>(sre[0], sim[0]);
-    __syncthreads();
+// rho of this iteration (conj(rt) . r, formed when r was) and beta; brk = 1 / 2 on a breakdown
+__device__ __forceinline__ int bi_rho_beta(const BiScal& S, c128& rho, c128& beta) {
+  rho = S.rho_n;
+  beta = mk<double>(0, 0);
+  if (cz(rho) || cinf(rho)) return 1;
+  if (!S.first) {
+    beta = cdiv(rho, S.rho1) * cdiv(S.alpha, S.omega);
+    if (cz(beta) || !cfin(beta)) return 2;
   }
-  if (threadIdx.x != 0) return;
-  BiScal& S = *sc;
+  return 0;
+}
+
+// MATLAB bicgstab's scalar logic of one stage, on the reduced sums tot[]
+__device__ void bi_logic(int stage, const c128* tot, BiScal& S) {
   switch (stage) {
-    case BI_RHO:
-      if (S.flag) return;
-      S.rho = tot[0];
-      if (cz(S.rho) || cinf(S.rho)) { S.flag = 4; S.brk = 1; return; }
-      if (!S.first) {
-        S.beta = cdiv(S.rho, S.rho1) * cdiv(S.alpha, S.omega);
-        if (cz(S.beta) || !cfin(S.beta)) { S.flag = 4; S.brk = 2; }
-      }
+    case BI_INIT:
+      S.rho_n = tot[0];
       return;
     case BI_ALPHA: {
       if (S.flag) return;
+      // the start of the iteration: rho, beta and their breakdown checks
+      c128 rho, beta;
+      const int brk = bi_rho_beta(S, rho, beta);
+      S.rho = rho;
+      if (brk) { S.flag = 4; S.brk = brk; return; }
+      S.beta = beta;
       const c128 rtv = tot[0];
       S.nph = sqrt(tot[1].x);
       S.nx = sqrt(tot[2].x);
@@ -473,57 +462,135 @@ __global__ __launch_bounds__(RED_THREADS) void k_bi_scalar(const c128* partial, 
       if (!S.flag) {
         S.rho1 = S.rho;
         S.first = 0;
+        S.rho_n = tot[1];
       }
       return;
   }
 }
 
-// p = r (first iteration) or p = r + beta (p - omega v)
-template <typename T>
-__global__ __launch_bounds__(RED_THREADS) void k_bi_p(const cplx<T>* r, const cplx<T>* v, cplx<T>* p, int64_t n,
-                                                      const BiScal* sc) {
-  if (sc->flag) return;
-  const bool first = sc->first != 0;
-  const cplx<T> be = ccast<T>(sc->beta), om = ccast<T>(sc->omega);
-  for (int64_t i = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * RED_THREADS)
-    p[i] = first ? r[i] : r[i] + be * (p[i] - om * v[i]);
+// One wavefront per reduced quantity (K of `partial`, plus ||s|| of the half step when `part_half` is
+// given): lane l sums blocks l, l + 64, ... in order, then a fixed shuffle tree; thread 0 then applies
+// the scalar logic (the half step's first when it is folded in here).
+__global__ __launch_bounds__(320) void k_bi_scalar(const c128* partial, int nblk, int K, const c128* part_half,
+                                                   int stage, BiScal* sc) {
+  __shared__ c128 tot[5];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const bool half = wave == K;
+  if (wave < K || (half && part_half)) {
+    const c128* pp = half ? part_half : partial;
+    const int kk = half ? 0 : wave, KK = half ? 1 : K;
+    double a = 0, b = 0;
+    for (int i = lane; i < nblk; i += 64) {
+      const c128 v = pp[i * KK + kk];
+      a += v.x;
+      b += v.y;
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+      a += __shfl_xor(a, off);
+      b += __shfl_xor(b, off);
+    }
+    if (lane == 0) tot[wave] = mk<double>(a, b);
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  if (part_half) bi_logic(BI_HALF, tot + K, *sc);
+  bi_logic(stage, tot, *sc);
 }
 
-// y = x0 + c x1 ; z = x2 - c x3 ; partial ||z||^2, with c = alpha (half step) or omega (full step)
+// K inner products; partial[blk * K + k]
+template <typename T, int K>
+__global__ __launch_bounds__(RED_THREADS) void k_bi_dots(DotArgs<T> args, int64_t n, c128* partial) {
+  double re[K], im[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) re[k] = im[k] = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * RED_THREADS) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const cplx<T> a = args.a[k][i], b = args.b[k][i];
+      re[k] += (double)a.x * b.x + (double)a.y * b.y;
+      im[k] += (double)a.x * b.y - (double)a.y * b.x;
+    }
+  }
+  c128 tot[K];
+  block_sum<K>(re, im, tot);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < K; ++k) partial[blockIdx.x * K + k] = tot[k];
+}
+
+// p = r (first iteration) or p = r + beta (p0 - omega v0); beta and its breakdown checks are evaluated by
+// every workgroup from the same scalars (the alpha stage records them)
+template <typename T>
+__global__ __launch_bounds__(RED_THREADS) void k_bi_p(const cplx<T>* r, const cplx<T>* v0, const cplx<T>* p0,
+                                                      cplx<T>* p, int64_t n, const BiScal* sc) {
+  const BiScal& S = *sc;
+  if (S.flag) return;
+  c128 rho, beta;
+  if (bi_rho_beta(S, rho, beta)) return;
+  const bool first = S.first != 0;
+  const cplx<T> be = ccast<T>(beta), om = ccast<T>(S.omega);
+  for (int64_t i = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * RED_THREADS)
+    p[i] = first ? r[i] : r[i] + be * (p0[i] - om * v0[i]);
+}
+
+// y = x0 + c x1 ; z = x2 - c x3 with c = alpha (half step) or omega (full step); partials of ||z||^2 and,
+// for the full step, of conj(rt) . z (the next iteration's rho): partial[blk * K + k], K = 1 + full
 template <typename T>
 __global__ __launch_bounds__(RED_THREADS) void k_bi_axpy2(const cplx<T>* x0, const cplx<T>* x1, const cplx<T>* x2,
-                                                          const cplx<T>* x3, cplx<T>* y, cplx<T>* z, int64_t n,
-                                                          const BiScal* sc, int use_omega, c128* partial) {
-  double re[1] = {0.0}, im[1] = {0.0};
+                                                          const cplx<T>* x3, cplx<T>* y, cplx<T>* z, const cplx<T>* rt,
+                                                          int64_t n, const BiScal* sc, int full, c128* partial) {
+  double re[2] = {0.0, 0.0}, im[2] = {0.0, 0.0};
   if (!sc->flag) {
-    const cplx<T> c = ccast<T>(use_omega ? sc->omega : sc->alpha);
+    const cplx<T> c = ccast<T>(full ? sc->omega : sc->alpha);
     for (int64_t i = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * RED_THREADS) {
       y[i] = x0[i] + c * x1[i];
       const cplx<T> o = x2[i] - c * x3[i];
       z[i] = o;
       re[0] += (double)o.x * o.x + (double)o.y * o.y;
+      if (full) {
+        const cplx<T> a = rt[i];
+        re[1] += (double)a.x * o.x + (double)a.y * o.y;
+        im[1] += (double)a.x * o.y - (double)a.y * o.x;
+      }
     }
   }
-  c128 tot[1];
-  block_sum<1>(re, im, tot);
-  if (threadIdx.x == 0) partial[blockIdx.x] = tot[0];
+  c128 tot[2];
+  block_sum<2>(re, im, tot);
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x * (1 + full)] = tot[0];
+    if (full) partial[blockIdx.x * 2 + 1] = tot[1];
+  }
+}
+
+// partial buffers of the stages: the half step's ||s|| partials live apart so that they can be reduced
+// together with the omega stage's
+static c128* part_main(vs_ctx* ctx) { return ctx->red.template as<c128>(); }
+static c128* part_half(vs_ctx* ctx) { return ctx->red.template as<c128>() + 4 * 1024; }
+
+static void launch_scalar(vs_ctx* ctx, const c128* part, int nblk, int K, const c128* ph, int stage, BiScal* sc) {
+  hipLaunchKernelGGL(k_bi_scalar, dim3(1), dim3(64 * (K + (ph ? 1 : 0))), 0, ctx->stream, part, nblk, K, ph, stage,
+                     sc);
 }
 
 template <typename T>
-void bi_stage_rho(Engine<T>& E, const cplx<T>* rt, const cplx<T>* r, const cplx<T>* v, cplx<T>* p, BiScal* sc) {
+void bi_stage_init(Engine<T>& E, const cplx<T>* rt, const cplx<T>* r, BiScal* sc) {
   DotArgs<T> a{};
   a.a[0] = rt;
   a.b[0] = r;
-  c128* part = E.ctx->red.template as<c128>();
-  hipLaunchKernelGGL((k_dots<T, 1>), dim3(E.grid), dim3(RED_THREADS), 0, E.ctx->stream, a, E.n, part);
-  hipLaunchKernelGGL(k_bi_scalar, dim3(1), dim3(RED_THREADS), 0, E.ctx->stream, part, E.grid, 1, (int)BI_RHO, sc);
-  hipLaunchKernelGGL((k_bi_p<T>), dim3(E.grid), dim3(RED_THREADS), 0, E.ctx->stream, r, v, p, E.n, sc);
+  hipLaunchKernelGGL((k_bi_dots<T, 1>), dim3(E.grid), dim3(RED_THREADS), 0, E.ctx->stream, a, E.n, part_main(E.ctx));
+  launch_scalar(E.ctx, part_main(E.ctx), E.grid, 1, nullptr, (int)BI_INIT, sc);
+  VS_HIP(hipGetLastError());
+}
+
+template <typename T>
+void bi_stage_rho(Engine<T>& E, const cplx<T>* r, const cplx<T>* v0, const cplx<T>* p0, cplx<T>* p, BiScal* sc) {
+  hipLaunchKernelGGL((k_bi_p<T>), dim3(E.grid), dim3(RED_THREADS), 0, E.ctx->stream, r, v0, p0, p, E.n, sc);
   VS_HIP(hipGetLastError());
 }
 
 template <typename T>
 void bi_stage_alpha_half(Engine<T>& E, const cplx<T>* rt, const cplx<T>* v, const cplx<T>* ph, const cplx<T>* x,
-                         const cplx<T>* r, cplx<T>* xh, cplx<T>* s, BiScal* sc) {
+                         const cplx<T>* r, cplx<T>* xh, cplx<T>* s, BiScal* sc, bool finish_half) {
   DotArgs<T> a{};
   a.a[0] = rt;
   a.b[0] = v;
@@ -531,18 +598,18 @@ void bi_stage_alpha_half(Engine<T>& E, const cplx<T>* rt, const cplx<T>* v, cons
   a.b[1] = ph;
   a.a[2] = x;
   a.b[2] = x;
-  c128* part = E.ctx->red.template as<c128>();
-  hipLaunchKernelGGL((k_dots<T, 3>), dim3(E.grid), dim3(RED_THREADS), 0, E.ctx->stream, a, E.n, part);
-  hipLaunchKernelGGL(k_bi_scalar, dim3(1), dim3(RED_THREADS), 0, E.ctx->stream, part, E.grid, 3, (int)BI_ALPHA, sc);
-  hipLaunchKernelGGL((k_bi_axpy2<T>), dim3(E.grid), dim3(RED_THREADS), 0, E.ctx->stream, x, ph, r, v, xh, s, E.n, sc,
-                     0, part);
-  hipLaunchKernelGGL(k_bi_scalar, dim3(1), dim3(RED_THREADS), 0, E.ctx->stream, part, E.grid, 1, (int)BI_HALF, sc);
+  vs_ctx* ctx = E.ctx;
+  hipLaunchKernelGGL((k_bi_dots<T, 3>), dim3(E.grid), dim3(RED_THREADS), 0, ctx->stream, a, E.n, part_main(ctx));
+  launch_scalar(ctx, part_main(ctx), E.grid, 3, nullptr, (int)BI_ALPHA, sc);
+  hipLaunchKernelGGL((k_bi_axpy2<T>), dim3(E.grid), dim3(RED_THREADS), 0, ctx->stream, x, ph, r, v, xh, s,
+                     (const cplx<T>*)nullptr, E.n, sc, 0, part_half(ctx));
+  if (finish_half) launch_scalar(ctx, part_half(ctx), E.grid, 1, nullptr, (int)BI_HALF, sc);
   VS_HIP(hipGetLastError());
 }
 
 template <typename T>
 void bi_stage_omega_full(Engine<T>& E, const cplx<T>* t, const cplx<T>* s, const cplx<T>* sh, const cplx<T>* xh,
-                         cplx<T>* x, cplx<T>* r, BiScal* sc) {
+                         const cplx<T>* rt, cplx<T>* x, cplx<T>* r, BiScal* sc, bool fold_half) {
   DotArgs<T> a{};
   a.a[0] = t;
   a.b[0] = t;
@@ -552,21 +619,22 @@ void bi_stage_omega_full(Engine<T>& E, const cplx<T>* t, const cplx<T>* s, const
   a.b[2] = sh;
   a.a[3] = xh;
   a.b[3] = xh;
-  c128* part = E.ctx->red.template as<c128>();
-  hipLaunchKernelGGL((k_dots<T, 4>), dim3(E.grid), dim3(RED_THREADS), 0, E.ctx->stream, a, E.n, part);
-  hipLaunchKernelGGL(k_bi_scalar, dim3(1), dim3(RED_THREADS), 0, E.ctx->stream, part, E.grid, 4, (int)BI_OMEGA, sc);
-  hipLaunchKernelGGL((k_bi_axpy2<T>), dim3(E.grid), dim3(RED_THREADS), 0, E.ctx->stream, xh, sh, s, t, x, r, E.n, sc,
-                     1, part);
-  hipLaunchKernelGGL(k_bi_scalar, dim3(1), dim3(RED_THREADS), 0, E.ctx->stream, part, E.grid, 1, (int)BI_FULL, sc);
+  vs_ctx* ctx = E.ctx;
+  hipLaunchKernelGGL((k_bi_dots<T, 4>), dim3(E.grid), dim3(RED_THREADS), 0, ctx->stream, a, E.n, part_main(ctx));
+  launch_scalar(ctx, part_main(ctx), E.grid, 4, fold_half ? part_half(ctx) : nullptr, (int)BI_OMEGA, sc);
+  hipLaunchKernelGGL((k_bi_axpy2<T>), dim3(E.grid), dim3(RED_THREADS), 0, ctx->stream, xh, sh, s, t, x, r, rt, E.n,
+                     sc, 1, part_main(ctx));
+  launch_scalar(ctx, part_main(ctx), E.grid, 2, nullptr, (int)BI_FULL, sc);
   VS_HIP(hipGetLastError());
 }
 
 #define VS_BI_INST(T)                                                                                             \
-  template void bi_stage_rho<T>(Engine<T>&, const cplx<T>*, const cplx<T>*, const cplx<T>*, cplx<T>*, BiScal*);  \
+  template void bi_stage_init<T>(Engine<T>&, const cplx<T>*, const cplx<T>*, BiScal*);                            \
+  template void bi_stage_rho<T>(Engine<T>&, const cplx<T>*, const cplx<T>*, const cplx<T>*, cplx<T>*, BiScal*);   \
   template void bi_stage_alpha_half<T>(Engine<T>&, const cplx<T>*, const cplx<T>*, const cplx<T>*, const cplx<T>*, \
-                                       const cplx<T>*, cplx<T>*, cplx<T>*, BiScal*);                             \
+                                       const cplx<T>*, cplx<T>*, cplx<T>*, BiScal*, bool);                       \
   template void bi_stage_omega_full<T>(Engine<T>&, const cplx<T>*, const cplx<T>*, const cplx<T>*, const cplx<T>*, \
-                                       cplx<T>*, cplx<T>*, BiScal*);
+                                       const cplx<T>*, cplx<T>*, cplx<T>*, BiScal*, bool);
 VS_BI_INST(float)
 VS_BI_INST(double)
 #undef VS_BI_INST

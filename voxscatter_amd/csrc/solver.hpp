@@ -37,6 +37,7 @@ struct Engine {
 // device-resident scalars of one BiCGStab iteration (read back once per iteration)
 struct BiScal {
   c128 rho, rho1, alpha, omega, beta;
+  c128 rho_n;                 // conj(rt) . r of the current r (the next iteration's rho)
   double ns, nr;              // ||s|| after the half step, ||r|| after the full step
   double nph, nx, nsh, nxh;   // norms of the stagnation tests
   int first;                  // 1 until the first full step (p = r)
@@ -46,14 +47,19 @@ struct BiScal {
   int active;                 // batched solves: 0 once this right-hand side is finished
 };
 
+// p = r + beta (p0 - omega v0) (p = r in the first iteration), out of place
 template <typename T>
-void bi_stage_rho(Engine<T>& E, const cplx<T>* rt, const cplx<T>* r, const cplx<T>* v, cplx<T>* p, BiScal* sc);
+void bi_stage_rho(Engine<T>& E, const cplx<T>* r, const cplx<T>* v0, const cplx<T>* p0, cplx<T>* p, BiScal* sc);
+// finish_half: reduce ||s|| now (otherwise the omega stage folds it in: fold_half)
 template <typename T>
 void bi_stage_alpha_half(Engine<T>& E, const cplx<T>* rt, const cplx<T>* v, const cplx<T>* ph, const cplx<T>* x,
-                         const cplx<T>* r, cplx<T>* xh, cplx<T>* s, BiScal* sc);
+                         const cplx<T>* r, cplx<T>* xh, cplx<T>* s, BiScal* sc, bool finish_half);
 template <typename T>
 void bi_stage_omega_full(Engine<T>& E, const cplx<T>* t, const cplx<T>* s, const cplx<T>* sh, const cplx<T>* xh,
-                         cplx<T>* x, cplx<T>* r, BiScal* sc);
+                         const cplx<T>* rt, cplx<T>* x, cplx<T>* r, BiScal* sc, bool fold_half);
+// rho_n = conj(rt) . r for an r that did not come from the full step (start, replaced residual)
+template <typename T>
+void bi_stage_init(Engine<T>& E, const cplx<T>* rt, const cplx<T>* r, BiScal* sc);
 
 // batched forms (nrhs systems, vectors stored consecutively, one BiScal per system)
 template <typename T>

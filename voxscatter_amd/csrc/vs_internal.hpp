@@ -161,6 +161,14 @@ struct vs_ctx {
   vs::DevBuf red_out;    // reduced scalars
   void* pinned = nullptr;     // host staging for reduced scalars (device -> host)
   void* pinned_up = nullptr;  // host staging for small coefficient uploads (host -> device)
+  // Reusable device workspaces of the solvers.  Solves on one context run one after another, so each
+  // slot keeps its memory between calls instead of allocating per solve (hipFree also synchronises).
+  enum WsSlot : int { WS_KRYLOV = 0, WS_KRYLOV_AUX, WS_REFINE, WS_REFINE_AUX, WS_NSLOTS };
+  vs::DevBuf ws[WS_NSLOTS];
+  void* workspace(int slot, size_t bytes) {
+    ws[slot].ensure(bytes);
+    return ws[slot].p;
+  }
   const vs::FftPlanHost& plan(int n);
 };
 
