@@ -3,13 +3,16 @@
 Follows assembly_nop_const.m:1-174 (far 4-point, medium 10-point, near 20-point Gauss rules per face
 dimension, the near 2x2x2 box corrected by nearby_correct.m:1-145), cube_faces.m:1-39,
 vector_projection.m:1-25, gauss_2d.m:1-19, mexcubatures/cubature_nop_const.cpp:8-124 and
-cubatures/cubature_nop_sym_near.m:1-88.  The five singular face-pair integrals are an input (the
+cubatures/cubature_nop_sym_near.m:1-88; the total field follows E_field_Nop_const.m:62-84 and
+mv_L_const.m:19-85.  The five singular face-pair integrals are an input (the
 reference obtains them from DIRECTFN through singular_*_const.m); ``reference_singular`` evaluates them
 with the reference's own C++ when oracle/_ref has been built (oracle/build_ref.sh).
 
 Parity: the face-face cubature is pinned against the reference's compiled cubature_nop_const.cpp; the
 assembly (box selection, symmetry fill, nearby correction, projection) is pinned by the GPU/oracle
-comparison only through this restatement and is otherwise parity unpinned against MATLAB.
+comparison only through this restatement and is otherwise parity unpinned against MATLAB.  The total
+field is pinned by the constitutive relation it must satisfy inside the scatterer for an exact solution
+of the JVIE, E = J / (j omega eps0 (eps - 1)).
 """
 from __future__ import annotations
 
@@ -17,6 +20,8 @@ import ctypes as C
 import os
 
 import numpy as np
+
+ETA0 = 3.767303134617706e2  # free-space impedance: omega eps0 = ko / ETA0
 
 # faces of the unit cube (cube_faces.m): corner, edge a (vertex 2 - vertex 1), edge b (vertex 4 - vertex 1)
 FACES = np.array([
@@ -224,3 +229,30 @@ def plane_wave_excitation(r, kvec, Eo):
             fac = fac * dx
     E = np.stack([Eo[i] * fac for i in range(3)], axis=-1)
     return E / dx ** 3
+
+
+# ------------------------------------------------------------------------------------------------
+# total field (E_field_Nop_const.m)
+# ------------------------------------------------------------------------------------------------
+_COMP = ((0, 1, 2), (1, 3, 4), (2, 4, 5))  # symmetric 3 x 3 of the six tensor components
+
+
+def mv_L_const(J, fN, Gram):
+    """mv_L_const.m:19-85 over the whole box: N J - Gram J for J (L x M x N x 3), fN the FFT of the
+    circulant embedding (2L x 2M x 2N x 6)."""
+    L, M, N, _ = J.shape
+    shp = fN.shape[:3]
+    fJ = [np.fft.fftn(J[..., c], s=shp) for c in range(3)]
+    out = np.empty(J.shape, dtype=complex)
+    for a in range(3):
+        acc = fN[..., _COMP[a][0]] * fJ[0] + fN[..., _COMP[a][1]] * fJ[1] + fN[..., _COMP[a][2]] * fJ[2]
+        out[..., a] = np.fft.ifftn(acc)[:L, :M, :N] - Gram * J[..., a]
+    return out
+
+
+def E_field_Nop_const(J, fN, Gram, ko, Einc):
+    """E_field_Nop_const.m:62-84: Einc + (N J - Gram J) / (Gram j omega eps0), omega eps0 = ko / ETA0."""
+    J = np.asarray(J, dtype=complex)
+    if not np.any(J):
+        return np.asarray(Einc, dtype=complex)
+    return Einc + mv_L_const(J, fN, Gram) / (Gram * 1j * ko / ETA0)

@@ -4,7 +4,9 @@
   against the reference's DIRECTFN, which converges to it as its order grows (fixtures);
 * Toeplitz tensor with the reference's DIRECTFN singular values against the oracle assembly, 1e-12;
 * preconditioned GMRES on an inhomogeneous cube (form 2, averaged-contrast 2-level preconditioner)
-  against the oracle: same iteration count, solution within 1e-10."""
+  against the oracle: same iteration count, solution within 1e-10;
+* total field (E_field_Nop_const.m) of that solution against the oracle, 1e-10, and the constitutive
+  relation E = J / (j omega eps0 (eps - 1)) in the scatterer to the solve's tolerance."""
 import json
 import math
 import os
@@ -113,3 +115,16 @@ def test_galerkin_gmres_matches_oracle(ctx):
     print(f"  gpu: flag={flag} iter={it}  cpu: flag={flagr} iter={itr}")
     assert flag == flagr == 0 and tuple(it) == tuple(itr)
     assert _rel(xg, xr) <= 1e-10
+    # total field on the whole box
+    nD = L * M * N
+    E = sysg.total_field(xg)
+    Jf = np.zeros(3 * nD, dtype=complex)
+    Jf[sysg.idx3] = xg
+    Je = Jf.reshape((L, M, N, 3), order="F")
+    Eref = OG.E_field_Nop_const(Je, fN, sysg.Gram, ko, sysg.Einc)
+    assert _rel(E, Eref) <= 1e-10
+    inside = np.abs(eps - 1) > 1e-12
+    expect = Je / (1j * (ko / OG.ETA0) * (eps - 1)[..., None])
+    assert _rel(E[inside], expect[inside]) <= 1e-5
+    # the operator's domain is restored
+    assert _rel(sysg.op.mvp(x), afun(x)) <= 1e-12

@@ -1,6 +1,7 @@
 """Oracle pins for the Galerkin N operator: the face-face cubature against the reference's own C++
 (cubature_nop_const.cpp compiled in oracle/_ref), the DIRECTFN fixtures against the same build, the
-projection table and the voxel-averaged plane wave against closed forms."""
+projection table and the voxel-averaged plane wave against closed forms, the total field against the
+constitutive relation of an exact solution."""
 import json
 import math
 import os
@@ -67,3 +68,51 @@ def test_static_self_term_closed_form_in_fixture_scale():
     rec = [r for r in fx if r["ko"] < 1e-6][0]
     assert abs(rec["directfn_64"][0][0] - exact) / exact < 1e-6
     assert abs(rec["directfn_20"][0][0] - exact) / exact < 5e-5
+
+
+def test_total_field_constitutive_relation():
+    """E_field_Nop_const of an exact JVIE solution equals J / (j omega eps0 (eps - 1)) in the scatterer,
+    and mv_L_const agrees with MVP.m on the whole box with Gram -> -Gram, Mc = -1."""
+    from oracle import mvp as OM
+    from oracle import operator as O
+    L, M, N = 3, 3, 2
+    ko, dx = 2 * math.pi, 1 / 35
+    fx = json.load(open(os.path.join(HERE, "golden", "galerkin_singular.json")))
+    sing = np.array([complex(*v) for v in fx[0]["directfn_20"]])
+    T = OG.assembly_nop_const(L, M, N, dx, ko, sing)
+    fN = O.fft_operator(O.circulant_nop_const(T))
+    rng = np.random.default_rng(5)
+    eps = (2.0 + rng.uniform(size=(L, M, N))) * (1 + 0.01j)
+    eps[0, 0, 0] = 1.0  # one air voxel
+    nD = L * M * N
+    Mc = eps - 1
+    idx = np.flatnonzero(np.abs(Mc.reshape(-1, order="F")) > 1e-12)
+    idx3 = np.concatenate([idx, nD + idx, 2 * nD + idx])
+    Gram = dx ** 3
+    xs = [(np.arange(n) - (n - 1) / 2) * dx for n in (L, M, N)]
+    r = np.stack(np.meshgrid(*xs, indexing="ij"), axis=-1)
+    Einc = OG.plane_wave_excitation(r, ko * np.array([1.0, 0, 0]), np.array([0, 0, 1.0]))
+    we0 = ko / OG.ETA0
+    b = (Gram * 1j * we0 * np.concatenate([Mc.reshape(-1, order="F")] * 3) * Einc.reshape(-1, order="F"))[idx3]
+    er3 = np.concatenate([eps.reshape(-1, order="F")] * 3)[idx3]
+    ones = np.ones((L, M, N))
+
+    def afun(v):  # form 2: Gram J - (eps - 1)/eps N J = b / eps
+        return OM.MVP(v, fN, ones, (Mc / eps), Gram, "notransp", idx3)
+
+    A = np.stack([afun(e) for e in np.eye(idx3.size)], axis=1)
+    J = np.linalg.solve(A, b / er3)
+    Jf = np.zeros(3 * nD, dtype=complex)
+    Jf[idx3] = J
+    Je = Jf.reshape((L, M, N, 3), order="F")
+    E = OG.E_field_Nop_const(Je, fN, Gram, ko, Einc)
+    inside = (np.abs(Mc) > 1e-12)
+    expect = Je / (1j * we0 * Mc[..., None])
+    assert np.max(np.abs(E[inside] - expect[inside])) <= 1e-10 * np.max(np.abs(expect[inside]))
+    # the air voxel carries the incident plus the scattered field, which is not small here
+    assert np.linalg.norm(E[0, 0, 0] - Einc[0, 0, 0]) > 1e-3 * np.linalg.norm(Einc[0, 0, 0])
+    mvl = OG.mv_L_const(Je, fN, Gram).reshape(-1, order="F")
+    alt = OM.MVP(Jf, fN, ones, -np.ones((L, M, N)), -Gram, "notransp", np.arange(3 * nD))
+    assert np.allclose(mvl, alt, rtol=0, atol=1e-12 * np.abs(alt).max())
+    # zero current: the incident field unchanged
+    assert np.array_equal(OG.E_field_Nop_const(0 * Je, fN, Gram, ko, Einc), Einc)

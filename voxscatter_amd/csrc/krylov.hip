@@ -948,8 +948,10 @@ SolveResult solve_refine(Engine<double>& E64, Engine<float>& E32, const LinOp<do
 namespace vs {
 
 // Batched BiCGStab: nrhs independent systems advance together, every matvec / preconditioner launch
-// covers the whole batch, the per-system scalars live on the device and one captured graph replays a
-// full iteration.  A system stops when its recursive residual meets tol at a half or full step (x =
+// covers the whole batch and the per-system scalars live on the device, so an iteration needs one
+// read-back.  With VS_GRAPH=1 the iteration is captured once and replayed as a graph (on this runtime a
+// graph launch costs more host time than the eager launches it replaces, so eager is the default).
+// A system stops when its recursive residual meets tol at a half or full step (x =
 // xhalf at a half step); breakdowns stop that system alone.  At the end the true residuals are formed;
 // flag 0 requires the true relative residual <= tol, flag 1 otherwise (or when maxit is reached), 4 on
 // breakdown.
@@ -1012,7 +1014,7 @@ BatchResult solve_bicgstab_batch(Engine<T>& E, const LinOp<T>& op, int k, const 
     VS_HIP(hipMemcpyAsync(hs, sc, sizeof(BiScal) * k, hipMemcpyDeviceToHost, ctx->stream));
   };
   const char* ge = getenv("VS_GRAPH");
-  bool use_graph = !(ge && ge[0] == '0');
+  bool use_graph = ge && ge[0] == '1';
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
   int nact = 0;

@@ -1,7 +1,9 @@
-"""Fields, far field and cross sections of a DDA solution.
+"""Fields of a DDA or Galerkin solution, far field and cross sections of a DDA solution.
 
 * ``total_field_dda``   main_DDA.m:136-150: E = Einc - MVP_all(P) + P/chi, evaluated with the device
                         matvec over every voxel of the box (idxAll, Mc = dV, Gram = 1/alpha).
+* ``total_field_galerkin`` E_field_Nop_const.m:62-84 (mv_L_const.m): Einc + (N J - Gram J)/(Gram j w eps0),
+                        the same full-box device matvec with Gram -> -Gram, Mc = -1.
 * ``farfield`` / ``cross_sections``: device kernels (no reference counterpart, README.md:35).
 * ``sphere_quadrature`` Gauss-Legendre in cos(theta) x trapezoid in phi, for C_sca = int |F|^2 dOmega.
 """
@@ -31,6 +33,28 @@ def total_field_dda(op: Operator, P_sol, Einc, refInd, alpha, dV, idx3):
     E = np.asarray(Einc).reshape(-1, order="F") - temp + Eout
     if saved is not None:
         op.set_domain(*saved)
+    return E.reshape((L, M, N, 3), order="F")
+
+
+def total_field_galerkin(op: Operator, J, Einc, Gram, ko, idx3):
+    """E_field_Nop_const.m:62-84 with mv_L_const.m: Einc + (N J - Gram J) / (Gram j omega eps0) on the whole
+    L x M x N x 3 box, omega eps0 = ko / eta0, for J given on the unknowns idx3.  N J - Gram J is one
+    full-grid matvec with the operator's diagonal set to -Gram and its contrast to -1; the operator's
+    domain is restored afterwards."""
+    from .galerkin import ETA0
+    L, M, N = op.shape
+    nD = L * M * N
+    Jf = np.zeros(3 * nD, dtype=np.complex128)
+    Jf[idx3] = np.asarray(J, dtype=np.complex128).ravel()
+    Ei = np.asarray(Einc, dtype=np.complex128).reshape(-1, order="F")
+    if not np.any(Jf):
+        return Ei.reshape((L, M, N, 3), order="F")
+    saved = getattr(op, "_domain_args", None)
+    op.set_domain(np.arange(3 * nD), -Gram, 1.0, -1.0)
+    y = op.mvp(Jf)
+    if saved is not None:
+        op.set_domain(*saved)
+    E = Ei + y / (Gram * 1j * ko / ETA0)
     return E.reshape((L, M, N, 3), order="F")
 
 

@@ -6,6 +6,7 @@ side Gram j omega eps0 (eps - 1)/eps E_inc with the voxel-averaged plane wave of
 PlaneWave_Excitation.m), with the N operator of getOPERATORS.m assembled on the GPU.
 ``GalerkinSystem.preconditioner`` builds the 2-level Chan preconditioner of scatteringHumanHead.m:
 155-193: inv(Gram I - diag(face3_2) C) with the contrast averaged over x and y.
+``GalerkinSystem.total_field`` is E_field_Nop_const.m (scatteringHumanHead.m:116-120).
 """
 from __future__ import annotations
 
@@ -40,6 +41,7 @@ class GalerkinSystem:
     idx3: np.ndarray
     b: np.ndarray
     Gram: float
+    Einc: np.ndarray  # voxel-averaged incident field, L x M x N x 3
 
     @classmethod
     def build(cls, eps, dx: float, ko: float, kdir=(1.0, 0.0, 0.0), Eo=(0.0, 0.0, 1.0), singular=None,
@@ -64,7 +66,7 @@ class GalerkinSystem:
         er3 = np.concatenate([eps.reshape(-1, order="F")] * 3)
         b = rhs[idx3] / er3[idx3]
         op.set_domain(idx3, Gram, 1.0, (Mc / eps).reshape(-1, order="F"))
-        return cls(op, eps, dx, ko, idx3, b, Gram)
+        return cls(op, eps, dx, ko, idx3, b, Gram, Einc)
 
     def preconditioner(self, levels: int = 2, precisions: int = VS_FP64) -> Preconditioner:
         """inv(Gram I - diag(face3_2) C) with the x/y-averaged contrast (scatteringHumanHead.m:178-184)."""
@@ -72,3 +74,8 @@ class GalerkinSystem:
             raise ValueError("the averaged-contrast Galerkin preconditioner is 2-level")
         w = averaged_contrast(self.eps)
         return Preconditioner.from_operator(self.op, 2, self.Gram, w=w, precisions=precisions)
+
+    def total_field(self, J) -> np.ndarray:
+        """Total field on the whole box (E_field_Nop_const.m) for the solution J on the non-air unknowns."""
+        from .fields import total_field_galerkin
+        return total_field_galerkin(self.op, J, self.Einc, self.Gram, self.ko, self.idx3)
