@@ -164,6 +164,25 @@ int vs_pgmresdr(vs_op* op, vs_pc* pc, int side, const double* b, double* x, int 
                 int maxit, int ritz, const double* x0, int* flag, double* relres, int iter[2],
                 double* resvec, int* nres);
 
+/* Krylov-basis checkpoint of GMRES-DR (no counterpart in the reference: pgmresDR.m keeps its deflation
+ * space only between the restarts of one solve).  A vs_recycle holds, in device memory, the deflation
+ * space (ritz vectors of the preconditioned operator) that a solve ended with, for the next solve with
+ * the same operator, preconditioner and side -- e.g. the adjoint solve of a DDA gradient (A^T = A) after
+ * the forward solve.  Memory: unknowns * ritz complex doubles. */
+typedef struct vs_recycle vs_recycle;
+int vs_recycle_create(vs_ctx* ctx, vs_recycle** out);
+void vs_recycle_destroy(vs_recycle* rec);
+int vs_recycle_size(const vs_recycle* rec); /* vectors held; 0 when empty */
+int vs_recycle_clear(vs_recycle* rec);
+/* vs_pgmresdr with a checkpoint (rec may be NULL: then identical to vs_pgmresdr).  A non-empty rec, made
+ * with the same unknown count, ritz and side, replaces the first plain cycle by the projection onto it
+ * (GCRO-DR: [C,R] = qr(A U), U = U R^-1, x = U C^H r, r = r - C C^H r; one extra resvec entry); on
+ * return rec holds the final deflation space (unchanged when no deflated cycle ran).  resvec holds up to
+ * restart*(maxit+1)+2 values. */
+int vs_pgmresdr_rec(vs_op* op, vs_pc* pc, int side, const double* b, double* x, int restart, double tol,
+                    int maxit, int ritz, const double* x0, vs_recycle* rec, int* flag, double* relres,
+                    int iter[2], double* resvec, int* nres);
+
 /* Device-pointer forms: b, x are FP64 device vectors of vs_op_nunknowns() values. */
 int vs_bicgstab_dev(vs_op* op, vs_pc* pc, const void* b, void* x, double tol, int maxit, int* flag,
                     double* relres, double* iter, int* nres);

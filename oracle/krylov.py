@@ -406,9 +406,16 @@ def getHarmVecs2(m, k, H2, V, U, C):
     return vecs[:, order[m - k:]]
 
 
-def pgmresDR(afun, b, restart=50, tol=1e-3, maxit=10, ritz=None, L1=None, R1=None, x0=None):
+def pgmresDR(afun, b, restart=50, tol=1e-3, maxit=10, ritz=None, L1=None, R1=None, x0=None, U0=None,
+             return_U=False):
     """``[x,flag,relres,iter,resvec] = pgmresDR(A,b,restart,tol,maxit,ritz,L1,[],R1,[],x0)``
-    (pgmresDR.m:73-280; one left factor L1 or right factor R1)."""
+    (pgmresDR.m:73-280; one left factor L1 or right factor R1).
+
+    Recycling across solves (no counterpart in pgmresDR.m; GCRO-DR of Parks et al., SIAM J. Sci. Comput.
+    28 (2006), Alg. 2 lines 2-6): with ``U0`` (n x ritz, a deflation space kept from an earlier solve with
+    the same operator and preconditioner) the first plain cycle is replaced by [C, R] = qr(A U0),
+    U = U0 R^-1, x = U C^H r, r = r - C C^H r (one more resvec entry), and the deflated cycles follow.
+    ``return_U``: also return the deflation space at the end (None when no deflated cycle ran)."""
     b = np.asarray(b, dtype=np.complex128)
     ritz = min(10, int(math.floor(restart / 2 + 0.5))) if ritz is None else min(ritz, restart - 1)
     x0 = np.zeros_like(b) if x0 is None else np.array(x0, dtype=np.complex128)
@@ -419,19 +426,32 @@ def pgmresDR(afun, b, restart=50, tol=1e-3, maxit=10, ritz=None, L1=None, R1=Non
         bp = L1(bp)
     bnorm = _norm(bp)
     resvec = [_norm(r) / bnorm]
-    V, H, _, p, rin = pgmresDR_iter(afun, r, restart, L1, R1, None, tol * bnorm)
-    resvec += list(rin / bnorm)
-    y = np.linalg.lstsq(H, V.conj().T @ r, rcond=None)[0]
-    x = V[:, :p] @ y
-    r = r - V @ (H @ y)
-    U = C = None
-    if resvec[-1] > tol:
-        k = p - 1 if ritz > p - 1 else ritz
-        P = getHarmVecs1(p, k, H)
-        U = V[:, :p] @ P
-        C, R = np.linalg.qr(H @ P)
-        C = V @ C
-        U = np.linalg.solve(R.T, U.T).T
+    p = 0
+    if U0 is not None:
+        U0 = np.asarray(U0, dtype=np.complex128)
+        if U0.shape != (b.size, ritz):
+            raise ValueError("the recycled space must hold ritz vectors of the system size")
+        W = np.stack([_apply_side(afun, U0[:, i], L1, R1) for i in range(ritz)], axis=1)
+        C, R = np.linalg.qr(W)
+        U = np.linalg.solve(R.T, U0.T).T
+        cr = C.conj().T @ r
+        x = U @ cr
+        r = r - C @ cr
+        resvec.append(_norm(r) / bnorm)
+    else:
+        V, H, _, p, rin = pgmresDR_iter(afun, r, restart, L1, R1, None, tol * bnorm)
+        resvec += list(rin / bnorm)
+        y = np.linalg.lstsq(H, V.conj().T @ r, rcond=None)[0]
+        x = V[:, :p] @ y
+        r = r - V @ (H @ y)
+        U = C = None
+        if resvec[-1] > tol:
+            k = p - 1 if ritz > p - 1 else ritz
+            P = getHarmVecs1(p, k, H)
+            U = V[:, :p] @ P
+            C, R = np.linalg.qr(H @ P)
+            C = V @ C
+            U = np.linalg.solve(R.T, U.T).T
     outit = 0
     while resvec[-1] > tol and outit < maxit:
         outit += 1
@@ -456,4 +476,6 @@ def pgmresDR(afun, b, restart=50, tol=1e-3, maxit=10, ritz=None, L1=None, R1=Non
     x = x0 + x
     relres = _norm(b - afun(x)) / _norm(b)
     flag = 0 if relres <= tol else 1
+    if return_U:
+        return x, flag, relres, (p, outit), np.array(resvec), (U if outit > 0 or U0 is not None else None)
     return x, flag, relres, (p, outit), np.array(resvec)

@@ -177,3 +177,32 @@ def test_pgmresdr_parity(c1, side):
     assert tuple(it) == tuple(itr) and len(resvec) == len(resvecr)
     assert np.allclose(resvec, resvecr, rtol=1e-7, atol=1e-14)
     assert _rel(x, xr) <= 1e-9
+
+
+@pytest.mark.parametrize("side", ["L1", "R1"])
+def test_pgmresdr_recycling_parity(c1, ctx, side):
+    """Krylov-basis checkpoint: the deflation space of one solve starts the next (GCRO-DR projection);
+    iteration counts, residual history and solution equal the oracle's recycling restatement, and the
+    solution equals a cold solve's."""
+    print(f"[pgmresDR recycle] side={side}", flush=True)
+    sysd = c1["sys"]
+    pc = sysd.preconditioner(2)
+    mfun = _oracle_pc(c1, 2)
+    rng = np.random.default_rng(8)
+    b2 = rng.normal(size=sysd.b.size) + 1j * rng.normal(size=sysd.b.size)
+    rec = api.Recycle(ctx)
+    assert rec.size == 0
+    x1, f1, _, it1, rv1 = api.pgmresDR(sysd.op, sysd.b, 8, 1e-8, 20, 3, recycle=rec, **{side: pc})
+    x1r, f1r, _, it1r, rv1r, Ur = OK.pgmresDR(c1["afun"], sysd.b, 8, 1e-8, 20, 3, return_U=True, **{side: mfun})
+    assert f1 == f1r == 0 and tuple(it1) == tuple(it1r) and rec.size == 3
+    x2, f2, _, it2, rv2 = api.pgmresDR(sysd.op, b2, 8, 1e-8, 20, 3, recycle=rec, **{side: pc})
+    x2r, f2r, _, it2r, rv2r = OK.pgmresDR(c1["afun"], b2, 8, 1e-8, 20, 3, U0=Ur, **{side: mfun})
+    print(f"  gpu: iter={it2} nres={len(rv2)}  cpu: iter={it2r} nres={len(rv2r)}")
+    assert f2 == f2r == 0 and tuple(it2) == tuple(it2r) and len(rv2) == len(rv2r)
+    assert np.allclose(rv2, rv2r, rtol=1e-6, atol=1e-13)
+    assert _rel(x2, x2r) <= 1e-9
+    x3, f3, _, it3, rv3 = api.pgmresDR(sysd.op, b2, 8, 1e-8, 20, 3, **{side: pc})
+    assert f3 == 0 and _rel(x2, x3) <= 1e-7
+    # a space of another size is refused
+    with pytest.raises(api.VoxScatterError):
+        api.pgmresDR(sysd.op, b2, 8, 1e-8, 20, 4, recycle=rec, **{side: pc})

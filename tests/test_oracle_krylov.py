@@ -123,3 +123,35 @@ def test_pgmresdr_preconditioned_sides():
     for kw in ({"L1": mf}, {"R1": mf}):
         x, flag, relres, it, rv = K.pgmresDR(af, b, 15, 1e-10, 20, 4, **kw)
         assert flag == 0 and np.linalg.norm(x - xs) / np.linalg.norm(xs) < 1e-8
+
+
+def test_pgmresdr_recycling():
+    """Recycling the deflation space between solves (GCRO-DR): a space containing the solution solves
+    the system by the projection alone; a kept space from one right-hand side cuts the iterations of
+    the next; left and right preconditioning give the dense solution."""
+    A, b = _outlier_system()
+    af = lambda v: A @ v  # noqa: E731
+    n = b.size
+    rng = np.random.default_rng(4)
+    xs = np.linalg.solve(A, b)
+    U0 = np.column_stack([xs] + [rng.normal(size=n) + 1j * rng.normal(size=n) for _ in range(5)])
+    x, flag, relres, it, rv, U = K.pgmresDR(af, b, 20, 1e-10, 30, 6, U0=U0, return_U=True)
+    assert flag == 0 and tuple(it) == (0, 0) and len(rv) == 2 and rv[1] < 1e-12
+    assert np.linalg.norm(x - xs) / np.linalg.norm(xs) < 1e-12
+    # a second right-hand side with the first solve's space
+    b2 = rng.normal(size=n) + 1j * rng.normal(size=n)
+    x1, f1, r1, i1, rv1, Ukeep = K.pgmresDR(af, b, 20, 1e-10, 30, 6, return_U=True)
+    assert f1 == 0 and Ukeep.shape == (n, 6)
+    x2, f2, r2, i2, rv2 = K.pgmresDR(af, b2, 20, 1e-10, 30, 6)
+    x3, f3, r3, i3, rv3 = K.pgmresDR(af, b2, 20, 1e-10, 30, 6, U0=Ukeep)
+    xs2 = np.linalg.solve(A, b2)
+    assert f2 == f3 == 0 and len(rv3) < len(rv2)
+    assert np.linalg.norm(x3 - xs2) / np.linalg.norm(xs2) < 1e-9
+    Minv = np.linalg.inv(np.diag(np.diag(A)))
+    mf = lambda v: Minv @ v  # noqa: E731
+    for kw in ({"L1": mf}, {"R1": mf}):
+        _, _, _, _, _, Uk = K.pgmresDR(af, b, 15, 1e-10, 20, 4, return_U=True, **kw)
+        x4, f4, _, _, _ = K.pgmresDR(af, b2, 15, 1e-10, 20, 4, U0=Uk, **kw)
+        assert f4 == 0 and np.linalg.norm(x4 - xs2) / np.linalg.norm(xs2) < 1e-8
+    with pytest.raises(ValueError):
+        K.pgmresDR(af, b, 20, 1e-10, 30, 6, U0=U0[:, :3])

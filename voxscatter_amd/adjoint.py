@@ -4,6 +4,7 @@ The DDA system with per-voxel polarisability is A(eps) P = Einc with A = diag(1/
 symmetric point-dyadic interaction, so A^T = A: the adjoint solve A lam = w reuses the forward operator
 (its 'notransp' matvec) and preconditioner.  For a holomorphic objective z = w^T P,
     dz/deps_j = g_j = -(d(1/alpha)/deps)_j sum_c lam_{c,j} P_{c,j}      (device kernel vs_dda_grad_eps)
+(the forward solve's GMRES-DR deflation space can start the adjoint solve: a Krylov-basis checkpoint),
 and for real objectives built on it:
     C_ext = Im(c^T P), c = 4 pi k dV conj(Einc)/|E0|^2:   dC/dRe eps = Im g,   dC/dIm eps = Re g
     I_fwd = sum_c |b_c^T P|^2 (forward far field):         w = sum_c conj(b_c^T P) b_c,
@@ -84,11 +85,25 @@ class PerVoxelDDA:
         Einc = PlaneWavePointWise(scat.Eo, kvec, r).reshape(-1, order="F")[idx3]
         return cls(scat, op, pc, eps_vox, ainv, dainv, dV, idx3, Einc, kvec)
 
-    def solve(self, rhs, tol=1e-10, maxit=2000):
-        x, flag, relres, it, resvec = api.bicgstab(self.op, rhs, tol, maxit, self.pc)
+    def solve(self, rhs, tol=1e-10, maxit=2000, solver="bicgstab", recycle: api.Recycle | None = None,
+              restart=30, ritz=10):
+        """A x = rhs by BiCGStab (main_DDA.m) or by GMRES-DR with the preconditioner on the right; ``recycle``
+        (GMRES-DR only) is a Krylov-basis checkpoint that this solve starts from when it holds a space and
+        leaves its own deflation space in.  The residual-history length is kept in ``last_nres``."""
+        if solver == "bicgstab":
+            x, flag, relres, it, resvec = api.bicgstab(self.op, rhs, tol, maxit, self.pc)
+        elif solver == "pgmresdr":
+            x, flag, relres, it, resvec = api.pgmresDR(self.op, rhs, restart, tol, max(1, maxit // restart), ritz,
+                                                       R1=self.pc, recycle=recycle)
+        else:
+            raise ValueError(f"unknown solver {solver!r}")
+        self.last_nres = len(resvec)
         if flag != 0:
-            raise RuntimeError(f"bicgstab did not converge (flag {flag}, relres {relres:.2e})")
+            raise RuntimeError(f"{solver} did not converge (flag {flag}, relres {relres:.2e})")
         return x
+
+    def _checkpoint(self, solver, checkpoint):
+        return api.Recycle(self.op.ctx) if solver == "pgmresdr" and checkpoint else None
 
     def grad_holomorphic(self, P, lam):
         ctx = get_context()
@@ -105,11 +120,14 @@ class PerVoxelDDA:
     def cext(self, P, E0sq=1.0):
         return float(np.imag(self.cext_vector(E0sq) @ P))
 
-    def cext_and_grad(self, tol=1e-10):
-        """C_ext and (dC/dRe eps, dC/dIm eps) at every occupied voxel (one forward + one adjoint solve)."""
-        P = self.solve(self.Einc, tol)
+    def cext_and_grad(self, tol=1e-10, solver="bicgstab", checkpoint=True, restart=30, ritz=10):
+        """C_ext and (dC/dRe eps, dC/dIm eps) at every occupied voxel (one forward + one adjoint solve).
+        With ``solver="pgmresdr"`` and ``checkpoint`` the forward solve's deflation space starts the
+        adjoint solve (same operator: A^T = A)."""
+        rec = self._checkpoint(solver, checkpoint)
+        P = self.solve(self.Einc, tol, solver=solver, recycle=rec, restart=restart, ritz=ritz)
         c = self.cext_vector()
-        lam = self.solve(c, tol)
+        lam = self.solve(c, tol, solver=solver, recycle=rec, restart=restart, ritz=ritz)
         g = self.grad_holomorphic(P, lam)
         return self.cext(P), g.imag.copy(), g.real.copy()
 
@@ -127,12 +145,14 @@ class PerVoxelDDA:
                 Bm[c, cc * nv:(cc + 1) * nv] = k * k * self.dV * proj[c, cc] * ph
         return Bm
 
-    def fwd_intensity_and_grad(self, tol=1e-10):
-        """|F(k_hat)|^2 and its gradients (dI/dRe eps, dI/dIm eps)."""
-        P = self.solve(self.Einc, tol)
+    def fwd_intensity_and_grad(self, tol=1e-10, solver="bicgstab", checkpoint=True, restart=30, ritz=10):
+        """|F(k_hat)|^2 and its gradients (dI/dRe eps, dI/dIm eps); ``solver`` / ``checkpoint`` as in
+        ``cext_and_grad``."""
+        rec = self._checkpoint(solver, checkpoint)
+        P = self.solve(self.Einc, tol, solver=solver, recycle=rec, restart=restart, ritz=ritz)
         Bm = self.forward_rows()
         z = Bm @ P
         w = np.conj(z) @ Bm
-        lam = self.solve(w, tol)
+        lam = self.solve(w, tol, solver=solver, recycle=rec, restart=restart, ritz=ritz)
         g = self.grad_holomorphic(P, lam)
         return float(np.sum(np.abs(z) ** 2)), 2 * g.real, -2 * g.imag

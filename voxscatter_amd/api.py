@@ -548,10 +548,39 @@ def pgmres(A, b, restart=50, tol=1e-3, maxit=10, L1=None, L2=None, R1=None, R2=N
     return x, flag.value, relres.value, (it[0], it[1]), resvec[:nres.value].copy()
 
 
-def pgmresDR(A, b, restart=50, tol=1e-3, maxit=10, ritz=None, L1=None, L2=None, R1=None, R2=None, x0=None):
+class Recycle:
+    """Krylov-basis checkpoint of GMRES-DR: the deflation space a ``pgmresDR`` solve ended with, kept in
+    device memory for the next solve with the same operator, preconditioner and side (for example the
+    adjoint solve of a DDA gradient after the forward solve, A^T = A).  No counterpart in the reference."""
+
+    def __init__(self, ctx: Context | None = None):
+        self.ctx = ctx or get_context()
+        h = C.c_void_p()
+        check(self.ctx.lib.vs_recycle_create(self.ctx.h, C.byref(h)))
+        self.h = h
+
+    @property
+    def size(self) -> int:
+        """Vectors held (0 when empty)."""
+        return int(self.ctx.lib.vs_recycle_size(self.h))
+
+    def clear(self):
+        check(self.ctx.lib.vs_recycle_clear(self.h))
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value:
+            self.ctx.lib.vs_recycle_destroy(h)
+            self.h = None
+
+
+def pgmresDR(A, b, restart=50, tol=1e-3, maxit=10, ritz=None, L1=None, L2=None, R1=None, R2=None, x0=None,
+             recycle: Recycle | None = None):
     """``[x, flag, relres, iter, resvec] = pgmresDR(A, b, restart, tol, maxit, ritz, L1, L2, R1, R2, x0)``:
     GMRES with deflated restarting (ritz harmonic Ritz vectors kept between cycles).  One preconditioner
-    factor is supported, either as L1 (left) or R1 (right)."""
+    factor is supported, either as L1 (left) or R1 (right).  ``recycle``: a ``Recycle`` checkpoint; when it
+    holds a space from an earlier solve, the first cycle is replaced by the projection onto it, and on
+    return it holds this solve's final deflation space."""
     A = _check_A(A)
     if L2 is not None or R2 is not None or (L1 is not None and R1 is not None):
         raise VoxScatterError(capi.VS_ERR_UNSUPPORTED, "only one of L1 / R1 is supported")
@@ -564,10 +593,16 @@ def pgmresDR(A, b, restart=50, tol=1e-3, maxit=10, ritz=None, L1=None, L2=None, 
     relres = C.c_double()
     it = (C.c_int * 2)()
     nres = C.c_int()
-    resvec = np.zeros(int(restart) * (int(maxit) + 1) + 1)
-    check(A.ctx.lib.vs_pgmresdr(A.h, None if pc is None else pc.h, side, dptr(b), dptr(x), int(restart), float(tol),
-                                int(maxit), 0 if ritz is None else int(ritz), dptr(x0a), C.byref(flag),
-                                C.byref(relres), it, dptr(resvec), C.byref(nres)))
+    resvec = np.zeros(int(restart) * (int(maxit) + 1) + 2)
+    if recycle is None:
+        check(A.ctx.lib.vs_pgmresdr(A.h, None if pc is None else pc.h, side, dptr(b), dptr(x), int(restart),
+                                    float(tol), int(maxit), 0 if ritz is None else int(ritz), dptr(x0a),
+                                    C.byref(flag), C.byref(relres), it, dptr(resvec), C.byref(nres)))
+    else:
+        check(A.ctx.lib.vs_pgmresdr_rec(A.h, None if pc is None else pc.h, side, dptr(b), dptr(x), int(restart),
+                                        float(tol), int(maxit), 0 if ritz is None else int(ritz), dptr(x0a),
+                                        recycle.h, C.byref(flag), C.byref(relres), it, dptr(resvec),
+                                        C.byref(nres)))
     return x, flag.value, relres.value, (it[0], it[1]), resvec[:nres.value].copy()
 
 

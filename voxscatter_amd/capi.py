@@ -94,6 +94,12 @@ SIGNATURES = {
                         C.POINTER(C.c_int)]),
     "vs_pgmresdr": (INT, [P, P, INT, D, D, INT, C.c_double, INT, INT, D, C.POINTER(C.c_int), D,
                           C.POINTER(C.c_int), D, C.POINTER(C.c_int)]),
+    "vs_recycle_create": (INT, [P, C.POINTER(P)]),
+    "vs_recycle_destroy": (None, [P]),
+    "vs_recycle_size": (INT, [P]),
+    "vs_recycle_clear": (INT, [P]),
+    "vs_pgmresdr_rec": (INT, [P, P, INT, D, D, INT, C.c_double, INT, INT, D, P, C.POINTER(C.c_int), D,
+                              C.POINTER(C.c_int), D, C.POINTER(C.c_int)]),
 }
 
 _lib = None

@@ -728,6 +728,64 @@ int vs_pgmresdr(vs_op* op, vs_pc* pc, int side, const double* b, double* x, int 
   });
 }
 
+struct vs_recycle {
+  vs_ctx* ctx = nullptr;
+  vs::Recycle r;
+};
+
+int vs_recycle_create(vs_ctx* ctx, vs_recycle** out) {
+  return guarded([&] {
+    VS_CHECK(ctx && out, VS_ERR_ARG, "null pointer");
+    auto rec = std::make_unique<vs_recycle>();
+    rec->ctx = ctx;
+    *out = rec.release();
+  });
+}
+
+void vs_recycle_destroy(vs_recycle* rec) {
+  if (!rec) return;
+  DeviceGuard g(rec->ctx->device);
+  (void)hipStreamSynchronize(rec->ctx->stream);
+  delete rec;
+}
+
+int vs_recycle_size(const vs_recycle* rec) { return rec ? rec->r.k : 0; }
+
+int vs_recycle_clear(vs_recycle* rec) {
+  return guarded([&] {
+    VS_CHECK(rec, VS_ERR_ARG, "null pointer");
+    rec->r.k = 0;
+  });
+}
+
+int vs_pgmresdr_rec(vs_op* op, vs_pc* pc, int side, const double* b, double* x, int restart, double tol, int maxit,
+                    int ritz, const double* x0, vs_recycle* rec, int* flag, double* relres, int iter[2],
+                    double* resvec, int* nres) {
+  return guarded([&] {
+    VS_CHECK(op && b && x && flag && relres && iter && resvec && nres, VS_ERR_ARG, "null pointer");
+    VS_CHECK(maxit >= 0 && restart >= 2, VS_ERR_ARG, "maxit must be >= 0 and restart >= 2");
+    VS_CHECK(side == 0 || side == 1, VS_ERR_ARG, "side must be 0 (L1) or 1 (R1)");
+    VS_CHECK(!rec || rec->ctx == op->ctx, VS_ERR_ARG, "recycled space belongs to another context");
+    check_solver_args(op, pc, VS_FP64);
+    vs_ctx* ctx = op->ctx;
+    DeviceGuard g(ctx->device);
+    const int64_t n = op->nunk;
+    DevBuf db(sizeof(c128) * n), dx(sizeof(c128) * n);
+    h2d(ctx, db.p, b, sizeof(c128) * n);
+    if (x0) h2d(ctx, dx.p, x0, sizeof(c128) * n);
+    Engine<double> E(ctx, n);
+    SolveResult R = solve_pgmresdr<double>(E, make_linop<double>(op, pc), side, db.as<c128>(), dx.as<c128>(),
+                                           x0 != nullptr, restart, tol, maxit, ritz, rec ? &rec->r : nullptr);
+    d2h(ctx, x, dx.p, sizeof(c128) * n);
+    *flag = R.flag;
+    *relres = R.relres;
+    iter[0] = R.inner;
+    iter[1] = R.outer;
+    *nres = (int)R.resvec.size();
+    std::copy(R.resvec.begin(), R.resvec.end(), resvec);
+  });
+}
+
 int vs_gmres_dev(vs_op* op, vs_pc* pc, const void* b, void* x, int restart, double tol, int maxit, int* flag,
                  double* relres, int iter[2], int* nres) {
   return guarded([&] {

@@ -650,7 +650,7 @@ Mat harm_vecs2(int m, int k, const Mat& H2, const Mat& CU, const Mat& VU) {
 
 template <typename T>
 SolveResult solve_pgmresdr(Engine<T>& E, const LinOp<T>& op, int side, const cplx<T>* b, cplx<T>* x, bool has_x0,
-                           int restart, double tol, int maxit, int ritz) {
+                           int restart, double tol, int maxit, int ritz, Recycle* rec) {
   using cv = cplx<T>;
   const int64_t n = E.n;
   SolveResult R;
@@ -748,10 +748,54 @@ SolveResult solve_pgmresdr(Engine<T>& E, const LinOp<T>& op, int side, const cpl
     return p;
   };
 
-  // first cycle (pgmresDR.m:152-185)
+  const int mode = left ? 0 : (right ? 1 : 2);
+  const bool recycled = rec && rec->k > 0;
   Mat H, Bm;
-  int p = arnoldi(H, Bm);
-  {
+  int p = 0;
+  if (recycled) {
+    // recycled deflation space (GCRO-DR, Parks et al. 2006, Alg. 2 lines 2-6): [C, R] = qr(A U0) by two
+    // Cholesky-QR passes, U = U0 R^-1, x = U C^H r, r = r - C C^H r
+    VS_CHECK(rec->n == n && rec->k == ritz && rec->mode == mode && rec->elem == (int)sizeof(cv), VS_ERR_ARG,
+             "recycled space does not match this solve (unknowns, ritz, preconditioner side or precision)");
+    const int k = rec->k;
+    const cv* U0 = rec->U.as<cv>();
+    for (int i = 0; i < k; ++i) {
+      const cv* in = U0 + (int64_t)i * n;
+      cv* out = Cp + (int64_t)i * n;
+      if (right) {
+        op.M(in, tmp);
+        op.A(tmp, out);
+      } else if (left) {
+        op.A(in, tmp);
+        op.M(tmp, out);
+      } else {
+        op.A(in, out);
+      }
+    }
+    Mat Rt(k, k), I(k, k);
+    for (int i = 0; i < k; ++i) Rt(i, i) = I(i, i) = 1.0;
+    for (int pass = 0; pass < 2; ++pass) {
+      Mat G(k, k);
+      std::vector<c128> g(k);
+      for (int j = 0; j < k; ++j) {
+        E.mdot(Cp, n, k, Cp + (int64_t)j * n, g.data());
+        for (int i = 0; i < k; ++i) G(i, j) = Z(g[i]);
+      }
+      const Mat Rk = dense::adjoint(dense::cholesky(G));
+      E.combine(Cp, n, k, to_c(dense::right_div_upper(I, Rk)).data(), k, Cn, n, false);
+      std::swap(Cp, Cn);
+      Rt = dense::matmul(Rk, Rt);
+    }
+    E.combine(U0, n, k, to_c(dense::right_div_upper(I, Rt)).data(), k, Up, n, false);
+    std::vector<c128> cr(k);
+    E.mdot(Cp, n, k, r, cr.data());
+    E.gemv_acc(Up, n, k, cr.data(), 1.0, xa);
+    E.gemv_acc(Cp, n, k, cr.data(), -1.0, r);
+    resvec.push_back(E.norm(r) / bnorm);
+    kc = k;
+  } else {
+    // first cycle (pgmresDR.m:152-185)
+    p = arnoldi(H, Bm);
     std::vector<c128> vr(p + 1);
     E.mdot(Vb[0], n, p + 1, r, vr.data());
     std::vector<zd> rhs(p + 1);
@@ -844,6 +888,15 @@ SolveResult solve_pgmresdr(Engine<T>& E, const LinOp<T>& op, int side, const cpl
       kc = ritz;
     }
   }
+  // keep the final deflation space for the next solve
+  if (rec && (outit > 0 || recycled) && kc == ritz) {
+    rec->U.ensure(sizeof(cv) * (size_t)n * kc);
+    VS_HIP(hipMemcpyAsync(rec->U.p, Up, sizeof(cv) * (size_t)n * kc, hipMemcpyDeviceToDevice, E.ctx->stream));
+    rec->n = n;
+    rec->k = kc;
+    rec->mode = mode;
+    rec->elem = (int)sizeof(cv);
+  }
   // x = x0 + R1 x (pgmresDR.m:257-271)
   if (right) {
     op.M(xa, tmp);
@@ -863,9 +916,9 @@ SolveResult solve_pgmresdr(Engine<T>& E, const LinOp<T>& op, int side, const cpl
 }
 
 template SolveResult solve_pgmresdr<double>(Engine<double>&, const LinOp<double>&, int, const c128*, c128*, bool, int,
-                                            double, int, int);
+                                            double, int, int, Recycle*);
 template SolveResult solve_pgmresdr<float>(Engine<float>&, const LinOp<float>&, int, const c64*, c64*, bool, int,
-                                           double, int, int);
+                                           double, int, int, Recycle*);
 
 template SolveResult solve_bicgstab<double>(Engine<double>&, const LinOp<double>&, const c128*, c128*, bool, double, int);
 template SolveResult solve_bicgstab<float>(Engine<float>&, const LinOp<float>&, const c64*, c64*, bool, double, int);

@@ -119,9 +119,21 @@ template <typename T>
 SolveResult solve_pgmres(Engine<T>& E, const LinOp<T>& op, int side, const cplx<T>* b, cplx<T>* x, bool has_x0,
                          int restart, double tol, int maxit);
 
+// deflation space kept between GMRES-DR solves (device, n x k column-major, of the preconditioned
+// operator of one side: 0 left, 1 right, 2 none)
+struct Recycle {
+  DevBuf U;
+  int64_t n = 0;
+  int k = 0;
+  int mode = -1;
+  int elem = 0;  // bytes per complex value
+};
+
+// rec (nullable): a non-empty space replaces the first plain cycle by a projection onto it; on return it
+// holds the solve's final deflation space (unchanged when no deflated cycle ran)
 template <typename T>
 SolveResult solve_pgmresdr(Engine<T>& E, const LinOp<T>& op, int side, const cplx<T>* b, cplx<T>* x, bool has_x0,
-                           int restart, double tol, int maxit, int ritz);
+                           int restart, double tol, int maxit, int ritz, Recycle* rec = nullptr);
 
 SolveResult solve_refine(Engine<double>& E64, Engine<float>& E32, const LinOp<double>& A64, const LinOp<float>& A32,
                          const c128* b, c128* x, bool has_x0, double tol, int max_outer, int inner_solver,
