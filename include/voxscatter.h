@@ -100,8 +100,9 @@ int vs_op_set_domain(vs_op* op, const int64_t* idx, int64_t nidx, const double* 
                      const double* Mr, int64_t mr_len, const double* Mc, int64_t mc_len);
 int64_t vs_op_nunknowns(const vs_op* op);
 /* The matvec variant used in a precision (VS_FP64 / VS_FP32): 0 five FFT passes, 1 fused padded-slab
- * kernel, 2 fused parity-split slab kernel.  Picked by timing at the operator's first domain (no
- * reference counterpart: the reference has one FFT path). */
+ * kernel, 2 fused parity-split slab kernel, 3 parity-split kernel with the two ky parities in separate
+ * workgroups.  Picked by timing at the operator's first domain (no reference counterpart: the reference
+ * has one FFT path). */
 int vs_op_matvec_variant(const vs_op* op, int precision, int* variant);
 
 /* ---------------------------------------------------------------------------------------------- */

@@ -127,7 +127,8 @@ def test_mvp_full_size_linearity_and_adjoint(ctx):
 @pytest.mark.parametrize("transp", ["notransp", "transp"])
 def test_fused_slab_mvp_matches_oracle_and_unfused(ctx, shape, transp, monkeypatch):
     """Grids whose shape has fused slab kernels: every matvec variant (five passes, padded slab, parity
-    split; VS_MATVEC pins one, an unavailable one falls back to the autotuned choice) against the oracle
+    split, parity split over two workgroups per slab; VS_MATVEC pins one, an unavailable one falls back
+    to the autotuned choice) against the oracle
     in FP64, and in FP32 against the oracle and the five-pass FP32 result."""
     print(f"[fused mvp] shape={shape} {transp}", flush=True)
     k = 2 * math.pi
@@ -143,14 +144,14 @@ def test_fused_slab_mvp_matches_oracle_and_unfused(ctx, shape, transp, monkeypat
     op = api.Operator.dda(r, k, precisions=capi.VS_FP64 | capi.VS_FP32, ctx=ctx)
     op.set_domain(idx3, 1 / alpha, 1.0, dV * chi.reshape(-1, order="F"))
     out = {}
-    for variant in ("pass5", "slab", "split"):
+    for variant in ("pass5", "slab", "split", "split2"):
         monkeypatch.setenv("VS_MATVEC", variant)
         out[("fp64", variant)] = op.mvp(x, transp=transp == "transp")
         xt = api.DeviceArray.from_numpy(x[None], np.complex64, ctx)
         yt = api.DeviceArray((1, x.size), np.complex64, ctx)
         op.mvp_dev(xt.ptr, yt.ptr, transp == "transp", capi.VS_FP32, 1)
         out[("fp32", variant)] = yt.numpy()[0].astype(np.complex128)
-    for variant in ("pass5", "slab", "split"):
+    for variant in ("pass5", "slab", "split", "split2"):
         assert _rel(out[("fp64", variant)], yref) <= 1e-12, variant
         assert _rel(out[("fp32", variant)], yref) <= 2e-5, variant
         assert _rel(out[("fp32", variant)], out[("fp32", "pass5")]) <= 2e-5, variant
@@ -170,7 +171,7 @@ def test_fused_slab_mvp_koch_full_size_fp32(ctx, monkeypatch):
     xt = api.DeviceArray.from_numpy(X, np.complex64, ctx)
     yt = api.DeviceArray(X.shape, np.complex64, ctx)
     res = {}
-    for variant in ("pass5", "slab", "split"):
+    for variant in ("pass5", "slab", "split", "split2"):
         monkeypatch.setenv("VS_MATVEC", variant)
         op.mvp_dev(xt.ptr, yt.ptr, False, capi.VS_FP32, 2)
         res[variant] = yt.numpy().astype(np.complex128)

@@ -261,7 +261,7 @@ class Operator:
         """Matvec variant in use for a precision: 'pass5', 'slab' or 'split' (autotuned at the first domain)."""
         v = C.c_int()
         check(self.ctx.lib.vs_op_matvec_variant(self.h, int(precision), C.byref(v)))
-        return ("pass5", "slab", "split")[v.value]
+        return ("pass5", "slab", "split", "split2")[v.value]
 
     def mvp_dev(self, x_ptr: int, y_ptr: int, transp: bool = False, precision: int = VS_FP64, nrhs: int = 1):
         """Device matvec on raw device pointers (e.g. torch ``data_ptr()``), asynchronous on the context stream."""

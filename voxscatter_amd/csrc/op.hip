@@ -255,6 +255,24 @@ struct IoMvIxSlab {
   __device__ __forceinline__ void store(int64_t g, int xp, int rhs, cplx<T> v) const { out.store(g, xp, rhs, v); }
 };
 
+// As IoMvIxSlab, loading the sum of two slab volumes (the two ky-parity terms of the split slab pass)
+template <typename T>
+struct IoMvIxSlab2 {
+  IoMvIx<T> out;
+  int64_t SL;
+  const cplx<T>* S2;
+  using Pre = typename IoMvIx<T>::Pre;
+  __device__ __forceinline__ cplx<T> load(int64_t g, int kx, int rhs) const {
+    const int64_t o = rhs * out.S_rhs + kx * SL + g;
+    return out.S[o] + S2[o];
+  }
+  __device__ __forceinline__ Pre pre(int64_t g, int xp, int rhs) const { return out.pre(g, xp, rhs); }
+  __device__ __forceinline__ void store(int64_t g, int xp, int rhs, cplx<T> v, const Pre& p) const {
+    out.store(g, xp, rhs, v, p);
+  }
+  __device__ __forceinline__ void store(int64_t g, int xp, int rhs, cplx<T> v) const { out.store(g, xp, rhs, v); }
+};
+
 // Compressed tensor: the octant kx <= L, ky <= M, kz <= N of fN.  Mirrored points pick up the parity
 // sign of each component: xy is odd in x and y, xz in x and z, yz in y and z (gperiodic_coeff_nop.m).
 template <typename T>
@@ -929,6 +947,10 @@ __global__ __launch_bounds__(1024) void k_yz(const cplx<T>* __restrict__ twy, co
 // ky parities of its kx slab one after the other with an unpadded spectrum of M x 3N values (half the
 // padded one), keeping the first half's result in registers; the z stage is one thread per (ky, c) with
 // both kz parities in registers (length-N register butterflies).
+// HALF: the two ky parities of a slab run in two workgroups (twice the workgroups, two resident per CU
+// where one slab's workgroup alone left the CU half idle); each writes its term of the cropped inverse
+// (IDFT_M(X_even), resp. conj(w_2M^y) IDFT_M(X_odd)) to a buffer of its own, and the inverse x pass adds
+// the two when it loads.
 #define VS_YZQ_SHAPES(X) X(128, 16) X(64, 16) X(128, 8) X(64, 8) X(32, 8) X(32, 16) X(16, 8) X(16, 4)
 
 template <typename T, int M, int N>
@@ -940,42 +962,95 @@ struct YzqCfg {
   static constexpr int IS = (NL * M + TH - 1) / TH;  // slab values per thread
   static constexpr int IZ = (M * N + TH - 1) / TH;   // tensor-product points per thread
   static constexpr int SP = M * NLP > NL * (M + 1) ? M * NLP : NL * (M + 1);
-  static constexpr size_t BYTES = sizeof(cplx<T>) * (size_t)SP;
+  static constexpr int TW = M + 2 * M + 2 * N;       // twiddles of M, 2M, 2N points, after the spectrum
+  static constexpr size_t BYTES = sizeof(cplx<T>) * (size_t)(SP + TW);
   static_assert(3 * M <= TH, "one thread per (ky, component) in the z stage");
 };
 
-template <typename T, int M, int N>
-__global__ __launch_bounds__(512) void k_yzq(const cplx<T>* __restrict__ twm, const cplx<T>* __restrict__ tw2m,
+// Latency structure: every global load of a phase is issued before the first of them is used, and the
+// twiddle tables live in LDS; in FP32 the tensor of a kz parity is requested before the z transforms it
+// waits for (the next parity's right after the product), so a parity costs three global round trips (FP64
+// loads the tensor at the product: its registers do not allow the prefetch).
+template <typename T, int M, int N, bool HALF>
+__global__ __launch_bounds__(512, HALF && sizeof(T) == 4 ? 4 : 2) void k_yzq(const cplx<T>* __restrict__ twm, const cplx<T>* __restrict__ tw2m,
                                              const cplx<T>* __restrict__ tw2n, cplx<T>* S, TensorQ<T> fN,
-                                             int64_t S_rhs) {
+                                             int64_t S_rhs, cplx<T>* __restrict__ Se, cplx<T>* __restrict__ So) {
   using C = YzqCfg<T, M, N>;
-  constexpr int NL = C::NL, NLP = C::NLP, RY1 = C::RY1, RY2 = C::RY2, TH = C::TH;
+  constexpr int NL = C::NL, NLP = C::NLP, RY1 = C::RY1, RY2 = C::RY2, TH = C::TH, IZ = C::IZ;
   cplx<T>* sp = reinterpret_cast<cplx<T>*>(vs_smem);
+  cplx<T>* lm = sp + C::SP;  // DFT_M twiddles
+  cplx<T>* l2m = lm + M;     // w_2M^y
+  cplx<T>* l2n = l2m + 2 * M;  // w_2N^z
   const int t = threadIdx.x;
   const int L = fN.L, L2 = 2 * L;
-  // mirror slabs kx, 2L - kx on consecutive logical ids, laid out XCD by XCD (as in k_yz)
-  const int per_xcd = (L2 + 7) / 8;
-  const int q = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
-  if (q >= L2) return;
+  // mirror slabs kx, 2L - kx on consecutive logical ids (HALF: both parities of each), laid out XCD by XCD
+  // (as in k_yz)
+  const int nlog = HALF ? 2 * L2 : L2;
+  const int per_xcd = (nlog + 7) / 8;
+  const int lid = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+  if (lid >= nlog) return;
+  const int q = HALF ? lid >> 1 : lid;
   const int kx = q == 0 ? 0 : (q == 1 ? L : ((q & 1) == 0 ? 1 + (q - 2) / 2 : L2 - 1 - (q - 3) / 2));
   cplx<T>* g = S + (int64_t)blockIdx.y * S_rhs + (int64_t)kx * NL * M;
-  cplx<T> acc[C::IS];  // result of the even-ky half, element t + i TH of the slab
+  cplx<T> acc[HALF ? 1 : C::IS];  // result of the even-ky half, element t + i TH of the slab
+  const int py0 = HALF ? (lid & 1) : 0, py1 = HALF ? py0 + 1 : 2;
+  constexpr bool PREF = sizeof(T) == 4 && !HALF;
+  for (int e = t; e < C::TW; e += TH) lm[e] = e < M ? twm[e] : (e < 3 * M ? tw2m[e - M] : tw2n[e - 3 * M]);
+  __syncthreads();
 
-  for (int py = 0; py < 2; ++py) {
+  // tensor of this thread's product points for kz parity pz
+  auto tload = [&](int py, int pz, cplx<T> (&f)[IZ][6]) {
+#pragma unroll
+    for (int i = 0; i < IZ; ++i) {
+      const int e = min(t + i * TH, M * N - 1);
+      const int jj = e % M, m = e / M;
+      fN.load6(kx, 2 * jj + py, 2 * m + pz, f[i]);
+    }
+  };
+  // load-and-apply point by point (FP64)
+  auto tmul = [&](int py, int pz) {
+#pragma unroll
+    for (int i = 0; i < IZ; ++i) {
+      const int e = t + i * TH;
+      if (e < M * N) {
+        const int jj = e % M, m = e / M;
+        cplx<T> f[6];
+        fN.load6(kx, 2 * jj + py, 2 * m + pz, f);
+        cplx<T>* s3 = sp + jj * NLP + m;
+        const cplx<T> jx = s3[0], jy = s3[N], jz = s3[2 * N];
+        s3[0] = f[0] * jx + f[1] * jy + f[2] * jz;
+        s3[N] = f[1] * jx + f[3] * jy + f[4] * jz;
+        s3[2 * N] = f[2] * jx + f[4] * jy + f[5] * jz;
+      }
+    }
+  };
+  auto tapply = [&](const cplx<T> (&f)[IZ][6]) {
+#pragma unroll
+    for (int i = 0; i < IZ; ++i) {
+      const int e = t + i * TH;
+      if (e < M * N) {
+        const int jj = e % M, m = e / M;
+        cplx<T>* s3 = sp + jj * NLP + m;
+        const cplx<T> jx = s3[0], jy = s3[N], jz = s3[2 * N];
+        s3[0] = f[i][0] * jx + f[i][1] * jy + f[i][2] * jz;
+        s3[N] = f[i][1] * jx + f[i][3] * jy + f[i][4] * jz;
+        s3[2 * N] = f[i][2] * jx + f[i][4] * jy + f[i][5] * jz;
+      }
+    }
+  };
+
+  for (int py = py0; py < py1; ++py) {
     // 0. slab -> LDS [line][y] (pitch M + 1), twisted by w_2M^y for the odd half
     {
       cplx<T> st[C::IS];
 #pragma unroll
-      for (int i = 0; i < C::IS; ++i) {
-        const int e = t + i * TH;
-        st[i] = g[e < NL * M ? e : 0];
-      }
+      for (int i = 0; i < C::IS; ++i) st[i] = g[min(t + i * TH, NL * M - 1)];
 #pragma unroll
       for (int i = 0; i < C::IS; ++i) {
         const int e = t + i * TH;
         if (e < NL * M) {
           const int line = e / M, y = e - line * M;
-          sp[line * (M + 1) + y] = py ? st[i] * tw2m[y] : st[i];
+          sp[line * (M + 1) + y] = py ? st[i] * l2m[y] : st[i];
         }
       }
     }
@@ -989,7 +1064,7 @@ __global__ __launch_bounds__(512) void k_yzq(const cplx<T>* __restrict__ twm, co
         const int line = it % NL, n1 = it / NL;
 #pragma unroll
         for (int n2 = 0; n2 < RY2; ++n2) v[i][n2] = sp[line * (M + 1) + n1 + RY1 * n2];
-        fft2_step1<T, M, false>(v[i], n1, twm);
+        fft2_step1<T, M, false>(v[i], n1, lm);
       }
     }
     __syncthreads();
@@ -1024,52 +1099,50 @@ __global__ __launch_bounds__(512) void k_yzq(const cplx<T>* __restrict__ twm, co
       const int j = t % M, c = t / M;
       cplx<T>* col = sp + j * NLP + c * N;
       cplx<T> a[N], o[N];
-      if (zt) {
+      {
+        cplx<T> f[PREF ? IZ : 1][6];
+        if constexpr (PREF) tload(py, 0, f);
+        if (zt) {
 #pragma unroll
-        for (int z = 0; z < N; ++z) {
-          a[z] = col[z];
-          o[z] = a[z] * tw2n[z];
-        }
-        Bfly<T, N>::run(a, false);
-        Bfly<T, N>::run(o, false);
-#pragma unroll
-        for (int m = 0; m < N; ++m) col[m] = a[m];  // even kz
-      }
-      auto zmul = [&](int pz) {
-#pragma unroll
-        for (int i = 0; i < C::IZ; ++i) {
-          const int e = t + i * TH;
-          if (e < M * N) {
-            const int jj = e % M, m = e / M;
-            cplx<T> f[6];
-            fN.load6(kx, 2 * jj + py, 2 * m + pz, f);
-            cplx<T>* s3 = sp + jj * NLP + m;
-            const cplx<T> jx = s3[0], jy = s3[N], jz = s3[2 * N];
-            s3[0] = f[0] * jx + f[1] * jy + f[2] * jz;
-            s3[N] = f[1] * jx + f[3] * jy + f[4] * jz;
-            s3[2 * N] = f[2] * jx + f[4] * jy + f[5] * jz;
+          for (int z = 0; z < N; ++z) {
+            a[z] = col[z];
+            o[z] = a[z] * l2n[z];
           }
+          Bfly<T, N>::run(a, false);
+          Bfly<T, N>::run(o, false);
+#pragma unroll
+          for (int m = 0; m < N; ++m) col[m] = a[m];  // even kz
         }
-      };
-      __syncthreads();
-      zmul(0);
-      __syncthreads();
-      if (zt) {
-#pragma unroll
-        for (int m = 0; m < N; ++m) a[m] = col[m];
-        Bfly<T, N>::run(a, true);
-#pragma unroll
-        for (int m = 0; m < N; ++m) col[m] = o[m];  // odd kz
+        __syncthreads();
+        if constexpr (PREF)
+          tapply(f);
+        else
+          tmul(py, 0);
       }
-      __syncthreads();
-      zmul(1);
+      {
+        cplx<T> f[PREF ? IZ : 1][6];
+        if constexpr (PREF) tload(py, 1, f);
+        __syncthreads();
+        if (zt) {
+#pragma unroll
+          for (int m = 0; m < N; ++m) a[m] = col[m];
+          Bfly<T, N>::run(a, true);
+#pragma unroll
+          for (int m = 0; m < N; ++m) col[m] = o[m];  // odd kz
+        }
+        __syncthreads();
+        if constexpr (PREF)
+          tapply(f);
+        else
+          tmul(py, 1);
+      }
       __syncthreads();
       if (zt) {
 #pragma unroll
         for (int m = 0; m < N; ++m) o[m] = col[m];
         Bfly<T, N>::run(o, true);
 #pragma unroll
-        for (int z = 0; z < N; ++z) col[z] = a[z] + cconj(tw2n[z]) * o[z];
+        for (int z = 0; z < N; ++z) col[z] = a[z] + cconj(l2n[z]) * o[z];
       }
       __syncthreads();
     }
@@ -1081,7 +1154,7 @@ __global__ __launch_bounds__(512) void k_yzq(const cplx<T>* __restrict__ twm, co
         const int line = it % NL, n1 = it / NL;
 #pragma unroll
         for (int n2 = 0; n2 < RY2; ++n2) v[i][n2] = sp[(n1 + RY1 * n2) * NLP + line];
-        fft2_step1<T, M, true>(v[i], n1, twm);
+        fft2_step1<T, M, true>(v[i], n1, lm);
       }
     }
     __syncthreads();
@@ -1110,17 +1183,30 @@ __global__ __launch_bounds__(512) void k_yzq(const cplx<T>* __restrict__ twm, co
       }
     }
     __syncthreads();
-    // 6. even half: keep in registers; odd half: combine and write the slab
+    // 6. even half: keep in registers; odd half: combine and write the slab.  HALF: write this parity's term
+    if (HALF) {
+      cplx<T>* o = (py ? So : Se) + (int64_t)blockIdx.y * S_rhs + (int64_t)kx * NL * M;
 #pragma unroll
-    for (int i = 0; i < C::IS; ++i) {
-      const int e = t + i * TH;
-      if (e < NL * M) {
-        const int line = e / M, y = e - line * M;
-        const cplx<T> r = sp[y * NLP + line];
-        if (py == 0)
-          acc[i] = r;
-        else
-          g[e] = acc[i] + cconj(tw2m[y]) * r;
+      for (int i = 0; i < C::IS; ++i) {
+        const int e = t + i * TH;
+        if (e < NL * M) {
+          const int line = e / M, y = e - line * M;
+          const cplx<T> r = sp[y * NLP + line];
+          o[e] = py ? cconj(l2m[y]) * r : r;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < C::IS; ++i) {
+        const int e = t + i * TH;
+        if (e < NL * M) {
+          const int line = e / M, y = e - line * M;
+          const cplx<T> r = sp[y * NLP + line];
+          if (py == 0)
+            acc[i] = r;
+          else
+            g[e] = acc[i] + cconj(l2m[y]) * r;
+        }
       }
     }
     __syncthreads();
@@ -1426,9 +1512,10 @@ bool yz_fused_available(int L, int M, int N) {
 template bool yz_fused_available<float>(int, int, int);
 template bool yz_fused_available<double>(int, int, int);
 
-// matvec variants: 0 = five passes, 1 = padded-slab fused kernel, 2 = parity-split fused kernel.
-// VS_MATVEC=pass5|slab|split forces one (when the operator holds its tensor layout); VS_FUSED=0 is
-// pass5.  Otherwise the variant timed fastest for the operator at its first domain is used.
+// matvec variants: 0 = five passes, 1 = padded-slab fused kernel, 2 = parity-split fused kernel, 3 = the
+// parity-split kernel with the two ky parities in separate workgroups.  VS_MATVEC=pass5|slab|split|split2
+// forces one (when the operator holds its tensor layout); VS_FUSED=0 is pass5.  Otherwise the variant
+// timed fastest for the operator at its first domain is used.
 static int forced_variant() {
   const char* f = std::getenv("VS_FUSED");
   if (f && f[0] == '0') return 0;
@@ -1437,6 +1524,7 @@ static int forced_variant() {
   if (!std::strcmp(e, "pass5")) return 0;
   if (!std::strcmp(e, "slab")) return 1;
   if (!std::strcmp(e, "split")) return 2;
+  if (!std::strcmp(e, "split2")) return 3;
   return -1;
 }
 
@@ -1453,7 +1541,7 @@ static int choose_variant(const vs_op* op) {
   if (f >= 0 && variant_available<T>(op, f)) return f;
   const int tuned = op->mv_variant[sizeof(T) == 8 ? 0 : 1];
   if (tuned >= 0 && variant_available<T>(op, tuned)) return tuned;
-  for (int v = 2; v >= 1; --v)
+  for (int v = 3; v >= 1; --v)
     if (variant_available<T>(op, v)) return v;
   return 0;
 }
@@ -1479,7 +1567,9 @@ static void yzq_set_attr(int M, int N) {
   switch (M * 4096 + N) {
 #define VS_YZQ_ATTR(A, B)                                                                                    \
   case A * 4096 + B:                                                                                         \
-    VS_HIP(hipFuncSetAttribute((const void*)k_yzq<T, A, B>, hipFuncAttributeMaxDynamicSharedMemorySize,      \
+    VS_HIP(hipFuncSetAttribute((const void*)k_yzq<T, A, B, false>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                               (int)YzqCfg<T, A, B>::BYTES));                                                \
+    VS_HIP(hipFuncSetAttribute((const void*)k_yzq<T, A, B, true>, hipFuncAttributeMaxDynamicSharedMemorySize,  \
                                (int)YzqCfg<T, A, B>::BYTES));                                                \
     return;
     VS_YZQ_SHAPES(VS_YZQ_ATTR)
@@ -1488,19 +1578,27 @@ static void yzq_set_attr(int M, int N) {
   }
 }
 
+// Se == nullptr: one workgroup per slab, result in place; otherwise one per (slab, ky parity), the two
+// terms to Se and So (same layout and right-hand-side stride as S)
 template <typename T>
-static void launch_yzq(vs_ctx* ctx, cplx<T>* S, const TensorQ<T>& fN, int L, int M, int N, int nrhs, int64_t S_rhs) {
+static void launch_yzq(vs_ctx* ctx, cplx<T>* S, const TensorQ<T>& fN, int L, int M, int N, int nrhs, int64_t S_rhs,
+                       cplx<T>* Se = nullptr, cplx<T>* So = nullptr) {
   const cplx<T>* twm = ctx->plan(M).dev<T>().tw;
   const cplx<T>* tw2m = ctx->plan(2 * M).dev<T>().tw;
   const cplx<T>* tw2n = ctx->plan(2 * N).dev<T>().tw;
-  const int per_xcd = (2 * L + 7) / 8;
+  const bool half = Se != nullptr;
+  const int per_xcd = ((half ? 4 : 2) * L + 7) / 8;
   dim3 grid((unsigned)(per_xcd * 8), (unsigned)nrhs);
   switch (M * 4096 + N) {
 #define VS_YZQ_CASE(A, B)                                                                                  \
   case A * 4096 + B: {                                                                                     \
     using Cf = YzqCfg<T, A, B>;                                                                            \
-    hipLaunchKernelGGL((k_yzq<T, A, B>), grid, dim3(Cf::TH), Cf::BYTES, ctx->stream, twm, tw2m, tw2n, S,   \
-                       fN, S_rhs);                                                                         \
+    if (half)                                                                                              \
+      hipLaunchKernelGGL((k_yzq<T, A, B, true>), grid, dim3(Cf::TH), Cf::BYTES, ctx->stream, twm, tw2m,    \
+                         tw2n, S, fN, S_rhs, Se, So);                                                      \
+    else                                                                                                   \
+      hipLaunchKernelGGL((k_yzq<T, A, B, false>), grid, dim3(Cf::TH), Cf::BYTES, ctx->stream, twm, tw2m,   \
+                         tw2n, S, fN, S_rhs, Se, So);                                                      \
     VS_HIP(hipGetLastError());                                                                             \
     return;                                                                                                \
   }
@@ -1556,8 +1654,23 @@ static void mvp_impl(vs_op* op, const cplx<T>* x, cplx<T>* y, int transp, int nr
   if (variant > 0) {
     // slab-major volume 2L x (3MN): x pass, fused y-z slab pass, inverse x pass
     const int64_t SL = lines, Sr = (int64_t)L2 * SL;
+    if (variant == 3) {
+      // input slabs, then the even-ky and odd-ky terms, nrhs slab volumes each
+      op->S.ensure(sizeof(cplx<T>) * 3 * Sr * nrhs);
+      S = op->S.as<cplx<T>>();
+    }
     IoMvFx<T> fx0{x, map3, op->nunk, op->V, L, M, transp, coef_dev<T>(op->b), S, Sr};
     run_rows<T, false, IoMvFxSlab<T>, false, true>(ctx, IoMvFxSlab<T>{fx0, SL}, L2, lines, L, L2, nrhs);
+    if (variant == 3) {
+      TensorQ<T> tq{fq, L, M, N, M / 2 + 1, N / 2 + 1};
+      cplx<T>* Se = S + Sr * nrhs;
+      cplx<T>* So = S + 2 * Sr * nrhs;
+      launch_yzq<T>(ctx, S, tq, L, M, N, nrhs, Sr, Se, So);
+      IoMvIx<T> ix0{x, y, map3, op->nunk, op->V, L, M, transp, coef_dev<T>(op->a), coef_dev<T>(op->at),
+                    coef_dev<T>(op->b), Se, Sr, (T)(1.0 / (double)op->P)};
+      run_rows<T, true, IoMvIxSlab2<T>, true, false>(ctx, IoMvIxSlab2<T>{ix0, SL, So}, L2, lines, L2, L, nrhs);
+      return;
+    }
     if (variant == 2) {
       TensorQ<T> tq{fq, L, M, N, M / 2 + 1, N / 2 + 1};
       launch_yzq<T>(ctx, S, tq, L, M, N, nrhs, Sr);
@@ -1631,7 +1744,7 @@ static void autotune_prec(vs_op* op) {
   VS_HIP(hipEventCreate(&e1));
   float best = 1e30f;
   int best_v = 0;
-  for (int v = 0; v <= 2; ++v) {
+  for (int v = 0; v <= 3; ++v) {
     if (!variant_available<T>(op, v)) continue;
     op->mv_variant[slot] = v;
     mvp_impl<T>(op, x.as<cplx<T>>(), y.as<cplx<T>>(), 0, 1);
