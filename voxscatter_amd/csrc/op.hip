@@ -557,6 +557,139 @@ __global__ __launch_bounds__(256) void k_row2(const cplx<T>* __restrict__ tw, IO
   }
 }
 
+
+// Zero-padded forward / cropped inverse row transform of length 2n as two length-n transforms split by the
+// parity of the frequency index: X(2j) = DFT_n(x)(j), X(2j+1) = DFT_n(x w_2n^x)(j), and the inverse cropped
+// to x < n, y(x) = IDFT_n(X_even)(x) + conj(w_2n^x) IDFT_n(X_odd)(x).  One thread group per (line, parity):
+// twice the threads of the length-2n kernel for the same lines, and no work on the padding.  Forward: the n
+// inputs through io.load, the 2n outputs staged in LDS and stored line-contiguous (slab-major volumes);
+// inverse: the 2n inputs staged line-contiguous, the n outputs combined and stored through io.store.
+template <typename T, int NH>
+struct RowPCfg {
+  static constexpr int R1 = Split<NH>::R1, R2 = Split<NH>::R2, TPL = cmax(R1, R2);
+  static constexpr int LPW = sizeof(T) == 8 ? 8 : 16;  // lines per workgroup
+  static constexpr int HL = 2 * LPW;                   // (line, parity) groups
+  static constexpr int THREADS = TPL * HL;
+  static constexpr int LP = LPW + 1, HLP = HL + 1;
+  static constexpr int LDS = cmax(2 * NH * LP, NH * HLP);  // staging [k][line] or exchange [n][group]
+};
+
+template <typename T, int NH, bool INV, class IO>
+__global__ __launch_bounds__(512) void k_row2p(const cplx<T>* __restrict__ tw, const cplx<T>* __restrict__ tw2, IO io,
+                                               int64_t nlines) {
+  using C = RowPCfg<T, NH>;
+  constexpr int R1 = C::R1, R2 = C::R2, TPL = C::TPL, LPW = C::LPW, LP = C::LP, HLP = C::HLP, TH = C::THREADS;
+  constexpr int N2 = 2 * NH;
+  constexpr int PB = sizeof(T) == 8 ? 4 : 8;
+  __shared__ cplx<T> lds[C::LDS];
+  const int t = threadIdx.x;
+  const int hl = t / TPL, s = t - hl * TPL;
+  const int l = hl >> 1, par = hl & 1;
+  const int64_t g0 = (int64_t)blockIdx.x * LPW;
+  const int rhs = blockIdx.y;
+  cplx<T> v[R2];
+  if (INV) {
+    constexpr int NI = (N2 * LPW + TH - 1) / TH;
+    cplx<T> st[NI];
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+      const int e = t + i * TH;
+      const int ll = e % LPW, k = e / LPW;
+      const bool ok = e < N2 * LPW && g0 + ll < nlines;
+      st[i] = io.load(ok ? g0 + ll : 0, ok ? k : 0, rhs);
+      if (!ok) st[i] = mk<T>(0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+      const int e = t + i * TH;
+      if (e < N2 * LPW) lds[(e / LPW) * LP + e % LPW] = st[i];
+    }
+    __syncthreads();
+    if (s < R1) {
+#pragma unroll
+      for (int n2 = 0; n2 < R2; ++n2) v[n2] = lds[(2 * (s + R1 * n2) + par) * LP + l];
+      fft2_step1<T, NH, true>(v, s, tw);
+    }
+    __syncthreads();
+  } else if (s < R1) {
+    const bool live = g0 + l < nlines;
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) {
+      const int n = s + R1 * n2;
+      cplx<T> x = io.load(live ? g0 + l : 0, live ? n : 0, rhs);
+      if (!live) x = mk<T>(0, 0);
+      v[n2] = par ? x * tw2[n] : x;
+    }
+    fft2_step1<T, NH, false>(v, s, tw);
+  }
+  // exchange [n1 R2 + k2][group]
+  if (s < R1) {
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) lds[(s * R2 + k2) * HLP + hl] = v[k2];
+  }
+  __syncthreads();
+  cplx<T> u[R1];
+  if (s < R2) {
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1) u[n1] = lds[(n1 * R2 + s) * HLP + hl];
+    fft2_step2<T, NH, INV>(u);
+  }
+  __syncthreads();  // the exchange buffer is reused below
+  if (!INV) {
+    // X(2 j + par), j = s + R2 k1, staged at [kx][line]
+    if (s < R2) {
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) lds[(2 * (s + R2 * k1) + par) * LP + l] = u[k1];
+    }
+    __syncthreads();
+    constexpr int NO = (N2 * LPW + TH - 1) / TH;
+#pragma unroll
+    for (int i0 = 0; i0 < NO; i0 += PB) {
+      typename IO::Pre pr[PB];
+#pragma unroll
+      for (int j = 0; j < PB; ++j) {
+        const int e = t + (i0 + j) * TH;
+        const int ll = e % LPW, k = e / LPW;
+        const bool ok = i0 + j < NO && e < N2 * LPW && g0 + ll < nlines;
+        pr[j] = io.pre(ok ? g0 + ll : 0, ok ? k : 0, rhs);
+      }
+#pragma unroll
+      for (int j = 0; j < PB; ++j) {
+        const int e = t + (i0 + j) * TH;
+        const int ll = e % LPW, k = e / LPW;
+        if (i0 + j < NO && e < N2 * LPW && g0 + ll < nlines) io.store(g0 + ll, k, rhs, lds[k * LP + ll], pr[j]);
+      }
+    }
+  } else {
+    // y_par(x), x = s + R2 k1, at [x][group]; then y = y_even + conj(w_2n^x) y_odd, x fastest over threads
+    if (s < R2) {
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) lds[(s + R2 * k1) * HLP + hl] = u[k1];
+    }
+    __syncthreads();
+    constexpr int NO = (NH * LPW + TH - 1) / TH;
+#pragma unroll
+    for (int i0 = 0; i0 < NO; i0 += PB) {
+      typename IO::Pre pr[PB];
+#pragma unroll
+      for (int j = 0; j < PB; ++j) {
+        const int e = t + (i0 + j) * TH;
+        const int x = e % NH, ll = e / NH;
+        const bool ok = i0 + j < NO && e < NH * LPW && g0 + ll < nlines;
+        pr[j] = io.pre(ok ? g0 + ll : 0, ok ? x : 0, rhs);
+      }
+#pragma unroll
+      for (int j = 0; j < PB; ++j) {
+        const int e = t + (i0 + j) * TH;
+        const int x = e % NH, ll = e / NH;
+        if (i0 + j < NO && e < NH * LPW && g0 + ll < nlines) {
+          const cplx<T> y = lds[x * HLP + 2 * ll] + cconj(tw2[x]) * lds[x * HLP + 2 * ll + 1];
+          io.store(g0 + ll, x, rhs, y, pr[j]);
+        }
+      }
+    }
+  }
+}
 template <typename T, int N>
 struct ColCfg {
   static constexpr int R1 = Split<N>::R1, R2 = Split<N>::R2, TPL = cmax(R1, R2);
@@ -971,6 +1104,13 @@ struct YzqCfg {
 // twiddle tables live in LDS; in FP32 the tensor of a kz parity is requested before the z transforms it
 // waits for (the next parity's right after the product), so a parity costs three global round trips (FP64
 // loads the tensor at the product: its registers do not allow the prefetch).
+// an index the compiler must treat as unknown, so that address arithmetic derived from it is redone where
+// it is used instead of being hoisted and held in registers across the whole kernel
+__device__ __forceinline__ int opaque(int v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
 template <typename T, int M, int N, bool HALF>
 __global__ __launch_bounds__(512, HALF && sizeof(T) == 4 ? 4 : 2) void k_yzq(const cplx<T>* __restrict__ twm, const cplx<T>* __restrict__ tw2m,
                                              const cplx<T>* __restrict__ tw2n, cplx<T>* S, TensorQ<T> fN,
@@ -1002,7 +1142,7 @@ __global__ __launch_bounds__(512, HALF && sizeof(T) == 4 ? 4 : 2) void k_yzq(con
   auto tload = [&](int py, int pz, cplx<T> (&f)[IZ][6]) {
 #pragma unroll
     for (int i = 0; i < IZ; ++i) {
-      const int e = min(t + i * TH, M * N - 1);
+      const int e = min(opaque(t + i * TH), M * N - 1);
       const int jj = e % M, m = e / M;
       fN.load6(kx, 2 * jj + py, 2 * m + pz, f[i]);
     }
@@ -1011,7 +1151,7 @@ __global__ __launch_bounds__(512, HALF && sizeof(T) == 4 ? 4 : 2) void k_yzq(con
   auto tmul = [&](int py, int pz) {
 #pragma unroll
     for (int i = 0; i < IZ; ++i) {
-      const int e = t + i * TH;
+      const int e = opaque(t + i * TH);
       if (e < M * N) {
         const int jj = e % M, m = e / M;
         cplx<T> f[6];
@@ -1027,7 +1167,7 @@ __global__ __launch_bounds__(512, HALF && sizeof(T) == 4 ? 4 : 2) void k_yzq(con
   auto tapply = [&](const cplx<T> (&f)[IZ][6]) {
 #pragma unroll
     for (int i = 0; i < IZ; ++i) {
-      const int e = t + i * TH;
+      const int e = opaque(t + i * TH);
       if (e < M * N) {
         const int jj = e % M, m = e / M;
         cplx<T>* s3 = sp + jj * NLP + m;
@@ -1044,10 +1184,10 @@ __global__ __launch_bounds__(512, HALF && sizeof(T) == 4 ? 4 : 2) void k_yzq(con
     {
       cplx<T> st[C::IS];
 #pragma unroll
-      for (int i = 0; i < C::IS; ++i) st[i] = g[min(t + i * TH, NL * M - 1)];
+      for (int i = 0; i < C::IS; ++i) st[i] = g[min(opaque(t + i * TH), NL * M - 1)];
 #pragma unroll
       for (int i = 0; i < C::IS; ++i) {
-        const int e = t + i * TH;
+        const int e = opaque(t + i * TH);
         if (e < NL * M) {
           const int line = e / M, y = e - line * M;
           sp[line * (M + 1) + y] = py ? st[i] * l2m[y] : st[i];
@@ -1188,7 +1328,7 @@ __global__ __launch_bounds__(512, HALF && sizeof(T) == 4 ? 4 : 2) void k_yzq(con
       cplx<T>* o = (py ? So : Se) + (int64_t)blockIdx.y * S_rhs + (int64_t)kx * NL * M;
 #pragma unroll
       for (int i = 0; i < C::IS; ++i) {
-        const int e = t + i * TH;
+        const int e = opaque(t + i * TH);
         if (e < NL * M) {
           const int line = e / M, y = e - line * M;
           const cplx<T> r = sp[y * NLP + line];
@@ -1198,7 +1338,7 @@ __global__ __launch_bounds__(512, HALF && sizeof(T) == 4 ? 4 : 2) void k_yzq(con
     } else {
 #pragma unroll
       for (int i = 0; i < C::IS; ++i) {
-        const int e = t + i * TH;
+        const int e = opaque(t + i * TH);
         if (e < NL * M) {
           const int line = e / M, y = e - line * M;
           const cplx<T> r = sp[y * NLP + line];
@@ -1354,6 +1494,36 @@ static void run_rows(vs_ctx* ctx, const IO& io, int n, int64_t nlines, int n_in,
   hipLaunchKernelGGL((k_row_fft<T, IO>), grid, dim3(FFT_THREADS), lds, ctx->stream, pd, io, nlines, nl, n_in,
                      n_out, INV ? 1 : 0);
   VS_HIP(hipGetLastError());
+}
+
+// the parity-split row transform of length 2 nh (k_row2p) when nh has a compiled split; false otherwise
+// (VS_ROWP=0 turns it off)
+template <typename T, bool INV, class IO>
+static bool run_rows_p(vs_ctx* ctx, const IO& io, int nh, int64_t nlines, int nrhs) {
+  static const bool off = [] {
+    const char* e = std::getenv("VS_ROWP");
+    return e && e[0] == '0';
+  }();
+  if (off || nh < 8) return false;
+  const cplx<T>* tw = ctx->plan(nh).dev<T>().tw;
+  const cplx<T>* tw2 = ctx->plan(2 * nh).dev<T>().tw;
+  switch (nh) {
+#define VS_ROWP_CASE(NN)                                                                                 \
+  case NN: {                                                                                             \
+    using Cf = RowPCfg<T, NN>;                                                                           \
+    if constexpr (Cf::LDS * sizeof(cplx<T>) <= 65536 && Cf::THREADS <= 512) {                            \
+      dim3 grid((unsigned)((nlines + Cf::LPW - 1) / Cf::LPW), (unsigned)nrhs);                          \
+      hipLaunchKernelGGL((k_row2p<T, NN, INV, IO>), grid, dim3(Cf::THREADS), 0, ctx->stream, tw, tw2, io, \
+                         nlines);                                                                        \
+      VS_HIP(hipGetLastError());                                                                         \
+      return true;                                                                                       \
+    }                                                                                                    \
+    return false;                                                                                        \
+  }
+    VS_FFT2_SIZES(VS_ROWP_CASE)
+#undef VS_ROWP_CASE
+    default: return false;
+  }
 }
 
 static bool fft2_length(int n) {
@@ -1660,7 +1830,8 @@ static void mvp_impl(vs_op* op, const cplx<T>* x, cplx<T>* y, int transp, int nr
       S = op->S.as<cplx<T>>();
     }
     IoMvFx<T> fx0{x, map3, op->nunk, op->V, L, M, transp, coef_dev<T>(op->b), S, Sr};
-    run_rows<T, false, IoMvFxSlab<T>, false, true>(ctx, IoMvFxSlab<T>{fx0, SL}, L2, lines, L, L2, nrhs);
+    if (!run_rows_p<T, false>(ctx, IoMvFxSlab<T>{fx0, SL}, L, lines, nrhs))
+      run_rows<T, false, IoMvFxSlab<T>, false, true>(ctx, IoMvFxSlab<T>{fx0, SL}, L2, lines, L, L2, nrhs);
     if (variant == 3) {
       TensorQ<T> tq{fq, L, M, N, M / 2 + 1, N / 2 + 1};
       cplx<T>* Se = S + Sr * nrhs;
@@ -1668,7 +1839,8 @@ static void mvp_impl(vs_op* op, const cplx<T>* x, cplx<T>* y, int transp, int nr
       launch_yzq<T>(ctx, S, tq, L, M, N, nrhs, Sr, Se, So);
       IoMvIx<T> ix0{x, y, map3, op->nunk, op->V, L, M, transp, coef_dev<T>(op->a), coef_dev<T>(op->at),
                     coef_dev<T>(op->b), Se, Sr, (T)(1.0 / (double)op->P)};
-      run_rows<T, true, IoMvIxSlab2<T>, true, false>(ctx, IoMvIxSlab2<T>{ix0, SL, So}, L2, lines, L2, L, nrhs);
+      if (!run_rows_p<T, true>(ctx, IoMvIxSlab2<T>{ix0, SL, So}, L, lines, nrhs))
+        run_rows<T, true, IoMvIxSlab2<T>, true, false>(ctx, IoMvIxSlab2<T>{ix0, SL, So}, L2, lines, L2, L, nrhs);
       return;
     }
     if (variant == 2) {
@@ -1680,7 +1852,8 @@ static void mvp_impl(vs_op* op, const cplx<T>* x, cplx<T>* y, int transp, int nr
     }
     IoMvIx<T> ix0{x, y, map3, op->nunk, op->V, L, M, transp, coef_dev<T>(op->a), coef_dev<T>(op->at),
                   coef_dev<T>(op->b), S, Sr, (T)(1.0 / (double)op->P)};
-    run_rows<T, true, IoMvIxSlab<T>, true, false>(ctx, IoMvIxSlab<T>{ix0, SL}, L2, lines, L2, L, nrhs);
+    if (!run_rows_p<T, true>(ctx, IoMvIxSlab<T>{ix0, SL}, L, lines, nrhs))
+      run_rows<T, true, IoMvIxSlab<T>, true, false>(ctx, IoMvIxSlab<T>{ix0, SL}, L2, lines, L2, L, nrhs);
     return;
   }
   // 1. gather + x FFT
