@@ -44,13 +44,15 @@ TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profile
                             "matvec_traffic_koch128x128x16.txt")
 
 
-def measured_traffic():
-    """HBM bytes per FP32 matvec from the committed hardware-counter summary (FETCH_SIZE + WRITE_SIZE of
-    the matvec kernels, scaled by the same counters on a device copy of known size), or None."""
+def measured_traffic(variant: str):
+    """HBM bytes per FP32 matvec of the given variant from the committed hardware-counter summary
+    (FETCH_SIZE + WRITE_SIZE of the matvec kernels, scaled by the same counters on a device copy of known
+    size), or None when the summary holds no figure for that variant."""
+    key = f"fp32_matvec_hbm_bytes[{variant}]:"
     try:
         with open(TRAFFIC_FILE) as f:
             for line in f:
-                if line.startswith("fp32_matvec_hbm_bytes:"):
+                if line.startswith(key):
                     return int(line.split(":")[1])
     except OSError:
         return None
@@ -220,7 +222,7 @@ def main():
                         pc_gbs=B_pc / (ms_pc * 1e-3) / 1e9)
 
     m32 = mv[VS_FP32]
-    traffic = measured_traffic()
+    traffic = measured_traffic(op.matvec_variant(VS_FP32))
     roof = {"bound": "hbm", "achieved": round(m32["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(m32["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic,
             "kernel": f"FP32 FFT matvec, one vs_mvp_dev call (variant '{op.matvec_variant(VS_FP32)}': "
