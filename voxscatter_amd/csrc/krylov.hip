@@ -110,8 +110,14 @@ SolveResult solve_bicgstab(Engine<T>& E, const LinOp<T>& op, const cplx<T>* b, c
   E.copy(x, xmin);
   double imin = 0;
   const double tolb = tol * n2b;
-  op.A(x, r);
-  double normr = E.update(UPD_SUB_, b, nullptr, nullptr, nullptr, r, nullptr, C(0), C(0), true);
+  double normr;
+  if (has_x0) {
+    op.A(x, r);
+    normr = E.update(UPD_SUB_, b, nullptr, nullptr, nullptr, r, nullptr, C(0), C(0), true);
+  } else {
+    E.copy(b, r);  // x0 = 0: r = b exactly, no matvec
+    normr = n2b;
+  }
   double normr_act = normr;
   if (normr <= tolb) {
     R.flag = 0;
@@ -960,9 +966,15 @@ SolveResult solve_refine(Engine<double>& E64, Engine<float>& E32, const LinOp<do
   R.flag = 1;
   int outer = 0;
   for (outer = 0; outer <= max_outer; ++outer) {
-    A64.A(x, r64.as<c128>());
-    const double nr = E64.update(UPD_SUB_, b, nullptr, nullptr, nullptr, r64.as<c128>(), nullptr, mk<double>(0, 0),
-                                 mk<double>(0, 0), true);
+    double nr;
+    if (outer == 0 && !has_x0) {
+      E64.copy(b, r64.as<c128>());  // x0 = 0: r = b exactly, no matvec
+      nr = n2b;
+    } else {
+      A64.A(x, r64.as<c128>());
+      nr = E64.update(UPD_SUB_, b, nullptr, nullptr, nullptr, r64.as<c128>(), nullptr, mk<double>(0, 0),
+                      mk<double>(0, 0), true);
+    }
     relres = nr / n2b;
     R.resvec.push_back(relres);
     if (!std::isfinite(nr)) {
@@ -1195,10 +1207,15 @@ BatchResult solve_refine_batch(Engine<double>& E64, Engine<float>& E32, const Li
   for (int i = 0; i < k; ++i) nb[i] = std::sqrt(h[i].x);
   int tot = 0, outer = 0;
   for (outer = 0; outer <= max_outer; ++outer) {
-    A64.Ab(X, R64.as<c128>(), k);
-    batch_sub<double>(E64, k, B, R64.as<c128>());
-    batch_norm2<double>(E64, k, R64.as<c128>(), part.as<c128>(), nrm.as<c128>());
-    E64.read(nrm.as<c128>(), k, h.data());
+    if (outer == 0) {
+      // X = 0: R = B exactly, no matvec
+      VS_HIP(hipMemcpyAsync(R64.p, B, sizeof(c128) * n * k, hipMemcpyDeviceToDevice, ctx->stream));
+    } else {
+      A64.Ab(X, R64.as<c128>(), k);
+      batch_sub<double>(E64, k, B, R64.as<c128>());
+      batch_norm2<double>(E64, k, R64.as<c128>(), part.as<c128>(), nrm.as<c128>());
+      E64.read(nrm.as<c128>(), k, h.data());
+    }
     bool all = true;
     std::vector<double> sc(k), back(k);
     for (int i = 0; i < k; ++i) {
