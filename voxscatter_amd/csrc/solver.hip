@@ -83,28 +83,26 @@ __global__ __launch_bounds__(RED_THREADS) void k_dots(DotArgs<T> args, int64_t n
 }
 
 // out[k] = sum over nblk partials, fixed order
+// out[k] = sum over nblk partials, fixed order: one wavefront per k (lane l sums blocks l, l + 64, ..., then a
+// shuffle tree); workgroup b covers k = 4 b ... 4 b + 3
 __global__ __launch_bounds__(RED_THREADS) void k_reduce_partials(const c128* partial, int nblk, int K, c128* out) {
-  __shared__ double sre[RED_THREADS], sim[RED_THREADS];
-  for (int k = 0; k < K; ++k) {
-    double a = 0, b = 0;
-    for (int i = threadIdx.x; i < nblk; i += RED_THREADS) {
-      a += partial[i * K + k].x;
-      b += partial[i * K + k].y;
-    }
-    sre[threadIdx.x] = a;
-    sim[threadIdx.x] = b;
-    __syncthreads();
-    for (int s = RED_THREADS / 2; s > 0; s >>= 1) {
-      if (threadIdx.x < s) {
-        sre[threadIdx.x] += sre[threadIdx.x + s];
-        sim[threadIdx.x] += sim[threadIdx.x + s];
-      }
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) out[k] = mk<double>(sre[0], sim[0]);
-    __syncthreads();
+  const int k = blockIdx.x * (RED_THREADS / 64) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (k >= K) return;
+  double a = 0, b = 0;
+  for (int i = lane; i < nblk; i += 64) {
+    const c128 v = partial[(int64_t)i * K + k];
+    a += v.x;
+    b += v.y;
   }
+  for (int off = 32; off >= 1; off >>= 1) {
+    a += __shfl_xor(a, off);
+    b += __shfl_xor(b, off);
+  }
+  if (lane == 0) out[k] = mk<double>(a, b);
 }
+
+// workgroups of k_reduce_partials for K sums
+static unsigned reduce_grid(int K) { return (unsigned)((K + RED_THREADS / 64 - 1) / (RED_THREADS / 64)); }
 
 // element-wise updates, each optionally accumulating ||result||^2 partials
 enum UpdOp : int {
@@ -158,32 +156,47 @@ __global__ __launch_bounds__(RED_THREADS) void k_update(int op, UpdArgs<T> a, in
 template <typename T>
 __global__ __launch_bounds__(RED_THREADS) void k_mdot(const cplx<T>* V, int64_t ldv, int k, const cplx<T>* w, int64_t n,
                                                       c128* partial) {
-  __shared__ double sre[RED_THREADS / 64], sim[RED_THREADS / 64];
+  // four basis vectors per pass: their loads are in flight together and w is read once per pass
+  constexpr int JB = 4;
+  __shared__ double sre[JB][RED_THREADS / 64], sim[JB][RED_THREADS / 64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int j = 0; j < k; ++j) {
-    double re = 0, im = 0;
-    const cplx<T>* v = V + (int64_t)j * ldv;
+  for (int j0 = 0; j0 < k; j0 += JB) {
+    double re[JB], im[JB];
+    const cplx<T>* v[JB];
+#pragma unroll
+    for (int q = 0; q < JB; ++q) {
+      re[q] = im[q] = 0.0;
+      v[q] = V + (int64_t)min(j0 + q, k - 1) * ldv;
+    }
     for (int64_t i = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * RED_THREADS) {
-      const cplx<T> a = v[i], b = w[i];
-      re += (double)a.x * b.x + (double)a.y * b.y;
-      im += (double)a.x * b.y - (double)a.y * b.x;
+      const cplx<T> b = w[i];
+#pragma unroll
+      for (int q = 0; q < JB; ++q) {
+        const cplx<T> a = v[q][i];
+        re[q] += (double)a.x * b.x + (double)a.y * b.y;
+        im[q] += (double)a.x * b.y - (double)a.y * b.x;
+      }
     }
-    for (int off = 32; off >= 1; off >>= 1) {
-      re += __shfl_xor(re, off);
-      im += __shfl_xor(im, off);
-    }
-    if (lane == 0) {
-      sre[wave] = re;
-      sim[wave] = im;
+#pragma unroll
+    for (int q = 0; q < JB; ++q) {
+      for (int off = 32; off >= 1; off >>= 1) {
+        re[q] += __shfl_xor(re[q], off);
+        im[q] += __shfl_xor(im[q], off);
+      }
+      if (lane == 0) {
+        sre[q][wave] = re[q];
+        sim[q][wave] = im[q];
+      }
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < JB && j0 + (int)threadIdx.x < k) {
+      const int q = threadIdx.x;
       double a = 0, b = 0;
-      for (int q = 0; q < RED_THREADS / 64; ++q) {
-        a += sre[q];
-        b += sim[q];
+      for (int r = 0; r < RED_THREADS / 64; ++r) {
+        a += sre[q][r];
+        b += sim[q][r];
       }
-      partial[(int64_t)blockIdx.x * k + j] = mk<double>(a, b);
+      partial[(int64_t)blockIdx.x * k + j0 + q] = mk<double>(a, b);
     }
     __syncthreads();
   }
@@ -232,8 +245,8 @@ Engine<T>::Engine(vs_ctx* c, int64_t n_) : ctx(c), n(n_), grid(red_grid(n_)) {
 
 template <typename T>
 void Engine<T>::fetch(int K, c128* out) {
-  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(RED_THREADS), 0, ctx->stream, ctx->red.as<c128>(), grid, K,
-                     ctx->red_out.as<c128>());
+  hipLaunchKernelGGL(k_reduce_partials, dim3(reduce_grid(K)), dim3(RED_THREADS), 0, ctx->stream, ctx->red.as<c128>(),
+                     grid, K, ctx->red_out.as<c128>());
   VS_HIP(hipGetLastError());
   VS_HIP(hipMemcpyAsync(ctx->pinned, ctx->red_out.p, sizeof(c128) * K, hipMemcpyDeviceToHost, ctx->stream));
   VS_HIP(hipStreamSynchronize(ctx->stream));
@@ -297,8 +310,8 @@ template <typename T>
 void Engine<T>::mdot_dev(const cplx<T>* V, int64_t ldv, int k, const cplx<T>* w, c128* out_dev) {
   VS_CHECK(k <= 512, VS_ERR_UNSUPPORTED, "Krylov basis larger than 512 vectors");
   hipLaunchKernelGGL((k_mdot<T>), dim3(grid), dim3(RED_THREADS), 0, ctx->stream, V, ldv, k, w, n, ctx->red.as<c128>());
-  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(RED_THREADS), 0, ctx->stream, ctx->red.as<c128>(), grid, k,
-                     out_dev);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(reduce_grid(k)), dim3(RED_THREADS), 0, ctx->stream, ctx->red.as<c128>(),
+                     grid, k, out_dev);
   VS_HIP(hipGetLastError());
 }
 
@@ -314,8 +327,8 @@ void Engine<T>::norm2_dev(const cplx<T>* x, c128* out_dev) {
   a.a[0] = x;
   a.b[0] = x;
   hipLaunchKernelGGL((k_dots<T, 1>), dim3(grid), dim3(RED_THREADS), 0, ctx->stream, a, n, ctx->red.as<c128>());
-  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(RED_THREADS), 0, ctx->stream, ctx->red.as<c128>(), grid, 1,
-                     out_dev);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(reduce_grid(1)), dim3(RED_THREADS), 0, ctx->stream, ctx->red.as<c128>(),
+                     grid, 1, out_dev);
   VS_HIP(hipGetLastError());
 }
 
