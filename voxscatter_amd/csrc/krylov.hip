@@ -83,7 +83,7 @@ static double eps_of() {
 // read back on its own.
 template <typename T>
 SolveResult solve_bicgstab(Engine<T>& E, const LinOp<T>& op, const cplx<T>* b, cplx<T>* x, bool has_x0, double tol,
-                           int maxit) {
+                           int maxit, bool verify) {
   using cv = cplx<T>;
   const int64_t n = E.n;
   vs_ctx* ctx = E.ctx;
@@ -222,6 +222,14 @@ SolveResult solve_bicgstab(Engine<T>& E, const LinOp<T>& op, const cplx<T>* b, c
     normr = h.ns;
     normr_act = normr;
     resvec[2 * ii - 1] = normr;
+    if (!verify && normr <= tolb && moresteps == 0) {
+      cancel_spec();
+      E.copy(XH(ii), x);
+      flag = 0;
+      iter = ii - 0.5;
+      nres = 2 * ii;
+      break;
+    }
     if (normr <= tolb || stag >= maxstagsteps || moresteps) {
       cancel_spec();
       op.A(XH(ii), S(ii));
@@ -270,6 +278,13 @@ SolveResult solve_bicgstab(Engine<T>& E, const LinOp<T>& op, const cplx<T>* b, c
     normr = h.nr;
     normr_act = normr;
     resvec[2 * ii] = normr;
+    if (!verify && normr <= tolb && moresteps == 0) {
+      cancel_spec();
+      flag = 0;
+      iter = ii;
+      nres = 2 * ii + 1;
+      break;
+    }
     if (normr <= tolb || stag >= maxstagsteps || moresteps) {
       cancel_spec();
       op.A(x, r);
@@ -926,8 +941,10 @@ template SolveResult solve_pgmresdr<double>(Engine<double>&, const LinOp<double>
 template SolveResult solve_pgmresdr<float>(Engine<float>&, const LinOp<float>&, int, const c64*, c64*, bool, int,
                                            double, int, int, Recycle*);
 
-template SolveResult solve_bicgstab<double>(Engine<double>&, const LinOp<double>&, const c128*, c128*, bool, double, int);
-template SolveResult solve_bicgstab<float>(Engine<float>&, const LinOp<float>&, const c64*, c64*, bool, double, int);
+template SolveResult solve_bicgstab<double>(Engine<double>&, const LinOp<double>&, const c128*, c128*, bool, double, int,
+                                            bool);
+template SolveResult solve_bicgstab<float>(Engine<float>&, const LinOp<float>&, const c64*, c64*, bool, double, int,
+                                           bool);
 template SolveResult solve_gmres<double>(Engine<double>&, const LinOp<double>&, const c128*, c128*, bool, int, double,
                                          int);
 template SolveResult solve_gmres<float>(Engine<float>&, const LinOp<float>&, const c64*, c64*, bool, int, double, int);
@@ -992,7 +1009,7 @@ SolveResult solve_refine(Engine<double>& E64, Engine<float>& E32, const LinOp<do
     convert_dev(ctx, r64.p, VS_FP64, r32.p, VS_FP32, n);
     SolveResult in;
     if (inner_solver == 0)
-      in = solve_bicgstab<float>(E32, A32, r32.as<c64>(), d32.as<c64>(), false, inner_tol, inner_maxit);
+      in = solve_bicgstab<float>(E32, A32, r32.as<c64>(), d32.as<c64>(), false, inner_tol, inner_maxit, false);
     else
       in = solve_gmres<float>(E32, A32, r32.as<c64>(), d32.as<c64>(), false, restart, inner_tol, inner_maxit);
     tot += inner_solver == 0 ? (int)(in.resvec.size() - 1) : (int)(in.resvec.size() - 1);

@@ -110,8 +110,10 @@ struct SolveResult {
 };
 
 template <typename T>
+// verify = false (inner solves of the refinement, whose result the FP64 residual checks): a recursive
+// residual norm <= tol * ||b|| ends the solve without MATLAB's true-residual check
 SolveResult solve_bicgstab(Engine<T>& E, const LinOp<T>& op, const cplx<T>* b, cplx<T>* x, bool has_x0, double tol,
-                           int maxit);
+                           int maxit, bool verify = true);
 template <typename T>
 SolveResult solve_gmres(Engine<T>& E, const LinOp<T>& op, const cplx<T>* b, cplx<T>* x, bool has_x0, int restart,
                         double tol, int maxit);
