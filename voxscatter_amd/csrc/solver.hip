@@ -412,7 +412,7 @@ void Engine<T>::combine(const cplx<T>* X, int64_t ldx, int q, const c128* P_host
 
 // ------------------------------------------------------------------------------------------------
 // device-resident BiCGStab scalars: every scalar of an iteration is formed on the device, so a whole
-// iteration runs without host round trips (and can be replayed as one captured graph)
+// iteration runs without host round trips (the host reads a copy once per iteration, behind the next one)
 // ------------------------------------------------------------------------------------------------
 __device__ __forceinline__ bool cz(c128 a) { return a.x == 0.0 && a.y == 0.0; }
 __device__ __forceinline__ bool cinf(c128 a) { return isinf(a.x) || isinf(a.y); }
