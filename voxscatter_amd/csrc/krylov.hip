@@ -255,7 +255,7 @@ SolveResult solve_bicgstab(Engine<T>& E, const LinOp<T>& op, const cplx<T>* b, c
     }
     if (normr_act < normrmin) {
       normrmin = normr_act;
-      E.copy(XH(ii), xmin);
+      if (verify) E.copy(XH(ii), xmin);
       imin = ii - 0.5;
     }
     if (!second_done) {
@@ -308,7 +308,7 @@ SolveResult solve_bicgstab(Engine<T>& E, const LinOp<T>& op, const cplx<T>* b, c
     }
     if (normr_act < normrmin) {
       normrmin = normr_act;
-      E.copy(x, xmin);
+      if (verify) E.copy(x, xmin);
       imin = ii;
     }
     if (stag >= maxstagsteps) {
@@ -322,6 +322,10 @@ SolveResult solve_bicgstab(Engine<T>& E, const LinOp<T>& op, const cplx<T>* b, c
   if (nres < 0) nres = 2 * (int64_t)maxit + 1;
   double relres;
   if (flag == 0) {
+    relres = normr_act / n2b;
+  } else if (!verify) {
+    // unverified (inner) solves keep the last iterate: their caller checks the result itself
+    iter = ii;
     relres = normr_act / n2b;
   } else {
     op.A(xmin, r);

@@ -111,7 +111,8 @@ struct SolveResult {
 
 template <typename T>
 // verify = false (inner solves of the refinement, whose result the FP64 residual checks): a recursive
-// residual norm <= tol * ||b|| ends the solve without MATLAB's true-residual check
+// residual norm <= tol * ||b|| ends the solve without MATLAB's true-residual check, and an unconverged
+// solve returns its last iterate (no minimum-residual iterate is kept)
 SolveResult solve_bicgstab(Engine<T>& E, const LinOp<T>& op, const cplx<T>* b, cplx<T>* x, bool has_x0, double tol,
                            int maxit, bool verify = true);
 template <typename T>
