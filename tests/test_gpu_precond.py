@@ -85,12 +85,13 @@ def test_apply_matches_oracle(ctx, levels):
     assert _rel(pc.apply(x), yref) <= 1e-11
 
 
-@pytest.mark.parametrize("shape", [(10, 8, 4), (9, 7, 3), (6, 5, 2), (2, 3, 2)])
+@pytest.mark.parametrize("shape", [(10, 8, 4), (9, 7, 3), (6, 5, 2), (2, 3, 2), (4, 3, 30), (6, 5, 36)])
 @pytest.mark.parametrize("weights", ["scalar", "vector"])
 def test_mirrored_two_level_apply(ctx, shape, weights):
     """The device 2-level preconditioner keeps only the blocks kx <= L/2, ky <= M/2 and applies the others
     through the sign symmetry; its apply equals the oracle's apply with every block inverted, for even and
-    odd grids, scalar and per-row weights (packed and full storage), one and five right-hand sides."""
+    odd grids, scalar and per-row weights (packed and full storage), one and five right-hand sides; the
+    3N = 90 and 108 blocks take more than 64 KB of LDS per workgroup for four right-hand sides at once."""
     print(f"[mirror] shape={shape} weights={weights}", flush=True)
     L, M, N = shape
     k, d = 2 * math.pi, 0.07

@@ -702,6 +702,9 @@ void pc_store_inverse(vs_pc* pc, const c128* inv) {
   if (pc->precisions & VS_FP32) go(float(0), pc->inv32);
 }
 
+// LDS of one CU: the mirrored block product's budget for large blocks with several right-hand sides
+constexpr size_t kBmmLdsMax = 160 * 1024;
+
 // forced row split of the mirrored block product (a tuning knob; 0 = automatic)
 static int bmm_rows_split() {
   static const int v = [] {
@@ -729,17 +732,22 @@ static void pc_apply_impl(vs_pc* pc, const cplx<T>* x, cplx<T>* y, int nrhs) {
     const int n = pc->nb;
     if (pc->mirror) {
       // right-hand sides per block pass (times the four mirror points): the largest of 4, 2, 1 that keeps
-      // a tile of at least 8 stored blocks within 64 KB of LDS
+      // a tile of at least 8 stored blocks within 64 KB of LDS, or, for large blocks (3N above ~40 in FP64,
+      // ~80 in FP32) with several right-hand sides, a tile of 4 within the 160 KB of LDS of a CU, so that
+      // one pass over the blocks still serves up to 4 of them (the blocks are read nrhs / RC times)
       const int Lb = pc->Lb, Mb = pc->Mb;
-      auto fits = [&](int rc, int w) { return (size_t)4 * rc * n * w * sizeof(cplx<T>) <= 65536; };
+      size_t budget = 65536;
+      auto fits = [&](int rc, int w) { return (size_t)4 * rc * n * w * sizeof(cplx<T>) <= budget; };
       int RC = nrhs >= 4 ? 4 : (nrhs >= 2 ? 2 : 1);
-      while (RC > 1 && !fits(RC, std::min(8, Lb))) RC /= 2;
+      const bool big = RC > 1 && !fits(RC, std::min(8, Lb));
+      if (big) budget = kBmmLdsMax;
+      while (RC > 1 && !fits(RC, std::min(big ? 4 : 8, Lb))) RC /= 2;
       VS_CHECK(fits(RC, 1), VS_ERR_UNSUPPORTED, "2-level block too large for LDS");
       // workgroups cover W columns of a Wt-wide layout tile (W | Wt, both powers of two); rows are split over
       // RS sibling workgroups of about 256 lanes (measured best on 48-row blocks in both precisions)
       const int Wt = mirror_tile<T>();
       int lw = 0;
-      while ((2 << lw) <= Wt && fits(RC, 2 << lw)) ++lw;
+      while ((2 << lw) <= Wt && (2 << lw) <= (big ? 4 : Wt) && fits(RC, 2 << lw)) ++lw;
       const int W = 1 << lw;
       const int tiles = (Lb + Wt - 1) / Wt, subs = Wt / W;
       const int chunks = (nrhs + RC - 1) / RC;
@@ -754,8 +762,13 @@ static void pc_apply_impl(vs_pc* pc, const cplx<T>* x, cplx<T>* y, int nrhs) {
       pc->U2.ensure(sizeof(cplx<T>) * V3 * nrhs);
       cplx<T>* U2 = pc->U2.as<cplx<T>>();
 #define VS_BMM(SYMV, RCV)                                                                                 \
-  hipLaunchKernelGGL((k_blockmv2m<T, SYMV, RCV>), grid, dim3(threads), lds, ctx->stream, B, U, U2, L, M, Lb, Mb, n, \
-                     Wt, lw, RS, V3, nrhs)
+  do {                                                                                                    \
+    if (lds > 65536)                                                                                      \
+      VS_HIP(hipFuncSetAttribute((const void*)k_blockmv2m<T, SYMV, RCV>,                                  \
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));                  \
+    hipLaunchKernelGGL((k_blockmv2m<T, SYMV, RCV>), grid, dim3(threads), lds, ctx->stream, B, U, U2, L, M, Lb, \
+                       Mb, n, Wt, lw, RS, V3, nrhs);                                                      \
+  } while (0)
       if (pc->sym) {
         if (RC == 4) VS_BMM(true, 4); else if (RC == 2) VS_BMM(true, 2); else VS_BMM(true, 1);
       } else {
