@@ -574,6 +574,120 @@ __global__ __launch_bounds__(512) void k_blockmv2m(const cplx<T>* __restrict__ B
   }
 }
 
+// Mirrored 2-level apply for large blocks (3N >= 96): the product of k_blockmv2m organised like a
+// GEMM.  A workgroup of 256 lanes covers a whole layout tile of W = 128 B / sizeof(complex) stored blocks
+// (one full 128-byte line per block entry) and RL x RPT rows (RL = 256 / W row lanes, RPT rows per lane).
+// The vectors are staged through LDS in chunks of SC columns, and each value read from LDS serves RPT
+// rows, so a lane holds RPT x 4 RC accumulators and the gathered vectors are read n / (RL RPT) times.
+template <typename T, bool SYM, int RC, int RPT>
+__global__ __launch_bounds__(256) void k_blockmv2k(const cplx<T>* __restrict__ B, const cplx<T>* __restrict__ U,
+                                                   cplx<T>* __restrict__ U2, int L, int M, int Lb, int Mb, int n,
+                                                   int RS, int64_t rhs_stride, int nrhs) {
+  constexpr int NV = 4 * RC;
+  constexpr int W = 128 / (int)sizeof(cplx<T>);
+  constexpr int RL = 256 / W;
+  constexpr int SC = 32;
+  constexpr int ROWS = RL * RPT;
+  cplx<T>* u = reinterpret_cast<cplx<T>*>(vs_smem);  // [NV][SC][W]
+  const int tiles = (Lb + W - 1) / W;
+  const int64_t G = (int64_t)tiles * Mb * RS;
+  const int64_t per_xcd = (G + 7) / 8;
+  const int64_t lid = (int64_t)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+  if (lid >= G) return;
+  const int rsi = (int)(lid % RS);
+  const int64_t rest = lid / RS;
+  const int tile = (int)(rest % tiles), kyb = (int)(rest / tiles);
+  const int kx0 = tile * W;
+  const int wv = max(0, min(W, Lb - kx0));
+  const int r0 = blockIdx.y * RC;
+  const int nr = min(RC, nrhs - r0);
+  const int N = n / 3;
+  const int64_t sstride = (int64_t)L * M;
+  const bool ymir = kyb != 0 && 2 * kyb != M;
+  const cplx<T>* Ui = U + (int64_t)r0 * rhs_stride;
+  cplx<T>* Uo = U2 + (int64_t)r0 * rhs_stride;
+  const int NT = SYM ? n * (n + 1) / 2 : n * n;
+  const int nz0 = 2 * N;
+  const cplx<T>* Bk = B + ((int64_t)tile * Mb + kyb) * NT * W;
+  const int w = threadIdx.x & (W - 1);
+  const int kx = kx0 + w;
+  const bool wl = w < wv;
+  const bool xl = kx != 0 && 2 * kx != L;
+  // rows of this lane (clamped to n - 1 past the end; their results are dropped)
+  int rr[RPT], t0[RPT];
+  bool rz[RPT];
+#pragma unroll
+  for (int i = 0; i < RPT; ++i) {
+    const int r = min(rsi * ROWS + (int)(threadIdx.x / W) + RL * i, n - 1);
+    rr[i] = r;
+    t0[i] = SYM ? r * (2 * n - r + 1) / 2 : r * n;
+    rz[i] = r >= nz0;
+  }
+  cplx<T> acc[RPT][NV];
+#pragma unroll
+  for (int i = 0; i < RPT; ++i)
+#pragma unroll
+    for (int c = 0; c < NV; ++c) acc[i][c] = mk<T>(0, 0);
+  for (int s0 = 0; s0 < n; s0 += SC) {
+    // stage columns s0 .. s0 + SC - 1: u[(v SC + sl) W + w] is variant v of column s0 + sl (zero past n)
+#pragma unroll 8
+    for (int e = threadIdx.x; e < NV * SC * W; e += 256) {
+      const int q = e / W;
+      const int sl = q % SC, v = q / SC;
+      const int s = s0 + sl;
+      const int rc = v >> 2, xm = v & 1, ym = (v >> 1) & 1;
+      const bool live = s < n && wl && rc < nr && (!xm || xl) && (!ym || ymir);
+      const int comp = (s >= N) + (s >= 2 * N);
+      const int gx = live ? (xm ? L - kx : kx) : 0, gy = live ? (ym ? M - kyb : kyb) : 0;
+      const cplx<T> val = Ui[live ? rc * rhs_stride + (int64_t)s * sstride + gx + (int64_t)L * gy : 0];
+      u[e] = (live ? (((xm && comp == 0) || (ym && comp == 1)) ? (T)-1 : (T)1) : (T)0) * val;
+    }
+    __syncthreads();
+    const int sn = min(SC, n - s0);
+#pragma unroll 4
+    for (int sl = 0; sl < sn; ++sl) {
+      const int s = s0 + sl;
+      cplx<T> uv[NV];
+#pragma unroll
+      for (int c = 0; c < NV; ++c) uv[c] = u[(c * SC + sl) * W + w];
+      const int ts = SYM ? s * (2 * n - s + 1) / 2 : 0;
+#pragma unroll
+      for (int i = 0; i < RPT; ++i) {
+        int t;
+        T g = (T)1;
+        if (SYM) {
+          // s < r: transposed entry (s, r) with the z sign pattern; s >= r: row r of the upper triangle
+          const bool lo = s < rr[i];
+          t = lo ? ts + (rr[i] - s) : t0[i] + (s - rr[i]);
+          if (lo && rz[i] != (s >= nz0)) g = (T)-1;
+        } else {
+          t = t0[i] + s;
+        }
+        const cplx<T> b = g * Bk[(int64_t)t * W + w];
+#pragma unroll
+        for (int c = 0; c < NV; ++c) acc[i][c] = acc[i][c] + b * uv[c];
+      }
+    }
+    __syncthreads();
+  }
+  if (!wl) return;
+#pragma unroll
+  for (int i = 0; i < RPT; ++i) {
+    const int r = rsi * ROWS + (int)(threadIdx.x / W) + RL * i;
+    if (r >= n) continue;
+    const int comp = r / N;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      const int rc = v >> 2, xm = v & 1, ym = (v >> 1) & 1;
+      const bool live = rc < nr && (!xm || xl) && (!ym || ymir);
+      if (!live) continue;
+      const int gx = xm ? L - kx : kx, gy = ym ? M - kyb : kyb;
+      const bool neg = (xm && comp == 0) || (ym && comp == 1);
+      Uo[rc * rhs_stride + r * sstride + gx + (int64_t)L * gy] = neg ? -acc[i][v] : acc[i][v];
+    }
+  }
+}
+
 // per kx: U2(:, kx) = Binv(kx) U(:, kx), one wavefront per row
 template <typename T>
 __global__ __launch_bounds__(256) void k_blockmv1(const cplx<T>* __restrict__ B, const cplx<T>* __restrict__ U,
@@ -736,6 +850,35 @@ static void pc_apply_impl(vs_pc* pc, const cplx<T>* x, cplx<T>* y, int nrhs) {
       // ~80 in FP32) with several right-hand sides, a tile of 4 within the 160 KB of LDS of a CU, so that
       // one pass over the blocks still serves up to 4 of them (the blocks are read nrhs / RC times)
       const int Lb = pc->Lb, Mb = pc->Mb;
+      if (n >= 96 && nrhs >= 2) {
+        // large blocks, several right-hand sides: the tiled product, 2 or 4 right-hand sides per pass over
+        // the blocks (one right-hand side: the row-split product below is faster)
+        constexpr int Wk = 128 / (int)sizeof(cplx<T>);
+        const int RCk = nrhs >= 4 ? 4 : 2;
+        const int tiles = (Lb + Wk - 1) / Wk;
+        const int chunks = (nrhs + RCk - 1) / RCk;
+        const size_t lds = sizeof(cplx<T>) * (size_t)4 * RCk * 32 * Wk;
+        pc->U2.ensure(sizeof(cplx<T>) * V3 * nrhs);
+        cplx<T>* U2 = pc->U2.as<cplx<T>>();
+#define VS_BMK(SYMV, RCV, RPTV)                                                                            \
+  do {                                                                                                     \
+    const int RSk = (n + (256 / Wk) * RPTV - 1) / ((256 / Wk) * RPTV);                                    \
+    const int64_t Gk = (int64_t)tiles * Mb * RSk;                                                          \
+    hipLaunchKernelGGL((k_blockmv2k<T, SYMV, RCV, RPTV>), dim3((unsigned)(((Gk + 7) / 8) * 8), (unsigned)chunks), \
+                       dim3(256), lds, ctx->stream, B, U, U2, L, M, Lb, Mb, n, RSk, V3, nrhs);              \
+  } while (0)
+        // rows per lane: the accumulators (RPT x 4 RC complex values) take at most 64 VGPRs in FP32 and 128
+        // in FP64
+        constexpr int R4 = sizeof(T) == 4 ? 2 : 1, R2 = sizeof(T) == 4 ? 4 : 2;
+        if (pc->sym) {
+          if (RCk == 4) VS_BMK(true, 4, R4); else VS_BMK(true, 2, R2);
+        } else {
+          if (RCk == 4) VS_BMK(false, 4, R4); else VS_BMK(false, 2, R2);
+        }
+#undef VS_BMK
+        VS_HIP(hipGetLastError());
+        U = U2;
+      } else {
       size_t budget = 65536;
       auto fits = [&](int rc, int w) { return (size_t)4 * rc * n * w * sizeof(cplx<T>) <= budget; };
       int RC = nrhs >= 4 ? 4 : (nrhs >= 2 ? 2 : 1);
@@ -777,6 +920,7 @@ static void pc_apply_impl(vs_pc* pc, const cplx<T>* x, cplx<T>* y, int nrhs) {
 #undef VS_BMM
       VS_HIP(hipGetLastError());
       U = U2;
+      }
     } else {
     // right-hand sides per block-matrix pass: 1, 4 or 8
     int RC = nrhs >= 8 ? 8 : (nrhs >= 4 ? 4 : 1);
