@@ -319,16 +319,23 @@ def polarisations(khat) -> tuple[np.ndarray, np.ndarray]:
 
 
 def plane_wave_batch(scat: Scatterer, ndir: int = 32):
-    """Right-hand sides [Ex; Ey; Ez](idx) for ndir Fibonacci directions x 2 polarisations: (2 ndir, 3 nvox)."""
-    from .excitation import PlaneWavePointWise
-    rows = []
+    """Right-hand sides [Ex; Ey; Ez](idx) for ndir Fibonacci directions x 2 polarisations: (2 ndir, 3 nvox).
+
+    PlaneWavePointWise.m:7-19 (E0 exp(i k.r) at the voxel centres) evaluated at the occupied voxels only,
+    one phase per direction shared by its two polarisations."""
+    rv = np.asarray(scat.r, dtype=np.float64).reshape(-1, 3, order="F")[np.asarray(scat.idx)]
+    dirs = fibonacci_sphere(ndir)
+    phase = np.exp(1j * (rv @ (scat.k0 * dirs).T))          # (nvox, ndir)
+    nv = rv.shape[0]
+    out = np.empty((2 * ndir, 3 * nv), dtype=np.complex128)
     meta = []
-    for khat in fibonacci_sphere(ndir):
-        for e in polarisations(khat):
-            E = PlaneWavePointWise(e, scat.k0 * khat, scat.r)
-            rows.append(np.concatenate([E[..., c].reshape(-1, order="F")[scat.idx] for c in range(3)]))
+    for d, khat in enumerate(dirs):
+        for p, e in enumerate(polarisations(khat)):
+            row = out[2 * d + p]
+            for c in range(3):
+                row[c * nv:(c + 1) * nv] = e[c] * phase[:, d]
             meta.append((khat, e))
-    return np.array(rows), meta
+    return out, meta
 
 
 def _rotation(rng) -> np.ndarray:
