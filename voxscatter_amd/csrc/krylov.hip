@@ -1034,8 +1034,8 @@ SolveResult solve_refine(Engine<double>& E64, Engine<float>& E32, const LinOp<do
 namespace vs {
 
 // Batched BiCGStab: nrhs independent systems advance together, every matvec / preconditioner launch
-// covers the whole batch and the per-system scalars live on the device, so an iteration needs one
-// read-back.  With VS_GRAPH=1 the iteration is captured once and replayed as a graph (on this runtime a
+// covers the running systems of the batch and the per-system scalars live on the device, so an
+// iteration needs one read-back.  With VS_GRAPH=1 the iteration is captured once and replayed as a graph (on this runtime a
 // graph launch costs more host time than the eager launches it replaces, so eager is the default).
 // A system stops when its recursive residual meets tol at a half or full step (x =
 // xhalf at a half step); breakdowns stop that system alone.  At the end the true residuals are formed;
@@ -1053,8 +1053,12 @@ BatchResult solve_bicgstab_batch(Engine<T>& E, const LinOp<T>& op, int k, const 
   R.flag.assign(k, 1);
   R.relres.assign(k, 0.0);
   R.iter.assign(k, 0.0);
-  Vecs<T> W(ctx, vs_ctx::WS_KRYLOV, n * k, 9);
+  // the iterates live in xw (slot j holds system sys[j]); a finished system's solution is copied to X.
+  // Once a quarter of the slots hold finished systems, the running ones are moved to the front and the
+  // batch shrinks (the state carried between iterations is r, rt, p, v, xw and the scalars).
+  Vecs<T> W(ctx, vs_ctx::WS_KRYLOV, n * k, 10);
   cv *r = W[0], *rt = W[1], *p = W[2], *v = W[3], *ph = W[4], *s = W[5], *sh = W[6], *t = W[7], *xh = W[8];
+  cv* xw = W[9];
   const cv* phv = op.Mb ? ph : p;
   const cv* shv = op.Mb ? sh : s;
   Carve ws;
@@ -1064,16 +1068,18 @@ BatchResult solve_bicgstab_batch(Engine<T>& E, const LinOp<T>& op, int k, const 
   const Span part = ws.at(ip), nrm = ws.at(in_), dsc = ws.at(isc);
   BiScal* sc = dsc.as<BiScal>();
   VS_HIP(hipMemsetAsync(X, 0, sizeof(cv) * n * k, ctx->stream));
+  VS_HIP(hipMemsetAsync(xw, 0, sizeof(cv) * n * k, ctx->stream));
   batch_norm2<T>(E, k, B, part.as<c128>(), nrm.as<c128>());
   std::vector<c128> nb(k);
   E.read(nrm.as<c128>(), k, nb.data());
   std::vector<double> tolb(k);
-  std::vector<int> active(k);
+  std::vector<int> active(k), sys(k);
   BiScal* up = reinterpret_cast<BiScal*>(ctx->pinned_up);
   for (int i = 0; i < k; ++i) {
     const double nbi = std::sqrt(nb[i].x);
     tolb[i] = tol * nbi;
     active[i] = nbi > 0 ? 1 : 0;
+    sys[i] = i;
     if (!active[i]) R.flag[i] = 0;
     BiScal h{};
     h.rho = C(1);
@@ -1089,15 +1095,19 @@ BatchResult solve_bicgstab_batch(Engine<T>& E, const LinOp<T>& op, int k, const 
   VS_HIP(hipMemcpyAsync(rt, B, sizeof(cv) * n * k, hipMemcpyDeviceToDevice, ctx->stream));
   BiScal* hs = reinterpret_cast<BiScal*>(ctx->pinned);
   c128* pt = part.as<c128>();
+  int kc = k;  // slots in use
   auto iteration = [&] {
-    bib_stage_rho<T>(E, k, rt, r, v, p, sc, pt);
-    if (op.Mb) op.Mb(p, ph, k);
-    op.Ab(phv, v, k);
-    bib_stage_alpha_half<T>(E, k, rt, v, phv, X, r, xh, s, sc, pt);
-    if (op.Mb) op.Mb(s, sh, k);
-    op.Ab(shv, t, k);
-    bib_stage_omega_full<T>(E, k, t, s, shv, xh, X, r, sc, pt);
-    VS_HIP(hipMemcpyAsync(hs, sc, sizeof(BiScal) * k, hipMemcpyDeviceToHost, ctx->stream));
+    bib_stage_rho<T>(E, kc, rt, r, v, p, sc, pt);
+    if (op.Mb) op.Mb(p, ph, kc);
+    op.Ab(phv, v, kc);
+    bib_stage_alpha_half<T>(E, kc, rt, v, phv, xw, r, xh, s, sc, pt);
+    if (op.Mb) op.Mb(s, sh, kc);
+    op.Ab(shv, t, kc);
+    bib_stage_omega_full<T>(E, kc, t, s, shv, xh, xw, r, sc, pt);
+    VS_HIP(hipMemcpyAsync(hs, sc, sizeof(BiScal) * kc, hipMemcpyDeviceToHost, ctx->stream));
+  };
+  auto copy_vec = [&](const cv* src, cv* dst) {
+    VS_HIP(hipMemcpyAsync(dst, src, sizeof(cv) * n, hipMemcpyDeviceToDevice, ctx->stream));
   };
   const char* ge = getenv("VS_GRAPH");
   bool use_graph = ge && ge[0] == '1';
@@ -1136,47 +1146,68 @@ BatchResult solve_bicgstab_batch(Engine<T>& E, const LinOp<T>& op, int k, const 
       iteration();
     VS_HIP(hipStreamSynchronize(ctx->stream));
     bool changed = false;
-    for (int i = 0; i < k; ++i) {
-      if (!active[i]) continue;
-      const BiScal& h = hs[i];
+    for (int j = 0; j < kc; ++j) {
+      if (!active[j]) continue;
+      const int i = sys[j];
+      const BiScal& h = hs[j];
       int stop = 0;
       if (h.flag && h.brk <= 4) {
         R.flag[i] = h.flag;
         R.iter[i] = ii - 1;
         stop = 1;
-      } else if (h.ns <= tolb[i]) {
-        VS_HIP(hipMemcpyAsync(X + (int64_t)i * n, xh + (int64_t)i * n, sizeof(cv) * n, hipMemcpyDeviceToDevice,
-                              ctx->stream));
+      } else if (h.ns <= tolb[j]) {
+        copy_vec(xh + (int64_t)j * n, X + (int64_t)i * n);
         R.flag[i] = 0;
         R.iter[i] = ii - 0.5;
-        stop = 1;
+        stop = 2;
       } else if (h.flag) {
         R.flag[i] = h.flag;
         R.iter[i] = ii - 1;
         stop = 1;
-      } else if (h.nr <= tolb[i]) {
+      } else if (h.nr <= tolb[j]) {
         R.flag[i] = 0;
         R.iter[i] = ii;
         stop = 1;
       }
       if (stop) {
-        active[i] = 0;
+        if (stop == 1) copy_vec(xw + (int64_t)j * n, X + (int64_t)i * n);
+        active[j] = 0;
         --nact;
         changed = true;
-        *reinterpret_cast<int*>(reinterpret_cast<char*>(ctx->pinned_up) + sizeof(int) * i) = 0;
+        *reinterpret_cast<int*>(reinterpret_cast<char*>(ctx->pinned_up) + sizeof(int) * j) = 0;
       } else {
         R.iter[i] = ii;
       }
     }
-    if (changed) {
+    if (changed && !gexec && nact > 0 && 4 * nact <= 3 * kc) {
+      // compact: running slots to the front (a source slot is never below its destination, and never a
+      // later destination, so the copies run in increasing order)
+      int jn = 0;
+      for (int j = 0; j < kc; ++j) {
+        if (!active[j]) continue;
+        if (j != jn) {
+          for (cv* vec : {r, rt, p, v, xw}) copy_vec(vec + (int64_t)j * n, vec + (int64_t)jn * n);
+          VS_HIP(hipMemcpyAsync(sc + jn, sc + j, sizeof(BiScal), hipMemcpyDeviceToDevice, ctx->stream));
+          sys[jn] = sys[j];
+          tolb[jn] = tolb[j];
+          active[jn] = 1;
+        }
+        ++jn;
+      }
+      kc = jn;
+      VS_HIP(hipStreamSynchronize(ctx->stream));
+    } else if (changed) {
       // deactivate the finished systems on the device (the staging ints are consumed before returning)
-      for (int i = 0; i < k; ++i)
-        if (!active[i])
-          VS_HIP(hipMemcpyAsync(&sc[i].active, reinterpret_cast<char*>(ctx->pinned_up) + sizeof(int) * i, sizeof(int),
+      for (int j = 0; j < kc; ++j)
+        if (!active[j])
+          VS_HIP(hipMemcpyAsync(&sc[j].active, reinterpret_cast<char*>(ctx->pinned_up) + sizeof(int) * j, sizeof(int),
                                 hipMemcpyHostToDevice, ctx->stream));
       VS_HIP(hipStreamSynchronize(ctx->stream));
     }
   }
+  // systems still running at maxit keep their last iterate
+  for (int j = 0; j < kc; ++j)
+    if (active[j]) copy_vec(xw + (int64_t)j * n, X + (int64_t)sys[j] * n);
   R.iterations = ii - 1;
   if (gexec) hipGraphExecDestroy(gexec);
   if (graph) hipGraphDestroy(graph);
@@ -1187,7 +1218,7 @@ BatchResult solve_bicgstab_batch(Engine<T>& E, const LinOp<T>& op, int k, const 
   std::vector<c128> nr(k);
   E.read(nrm.as<c128>(), k, nr.data());
   for (int i = 0; i < k; ++i) {
-    const double nbi = tolb[i] / (tol > 0 ? tol : 1.0);
+    const double nbi = std::sqrt(nb[i].x);
     R.relres[i] = nbi > 0 ? std::sqrt(nr[i].x) / nbi : 0.0;
     if (R.flag[i] == 0 && nbi > 0 && R.relres[i] > tol) R.flag[i] = 1;
   }
