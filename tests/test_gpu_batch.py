@@ -117,3 +117,26 @@ def test_particle_workload_refined_batches(ctx):
         assert flag == 0
         # both solutions are within 1e-6 (relative residual) of the exact one
         assert _rel(r.X[0], x) <= 1e-4
+
+
+def test_batch_edge_cases(ctx):
+    """A zero right-hand side (x = 0, flag 0), a duplicated one (the same result from another slot, also
+    after finished systems have been dropped from the batch), a scaled one, and a batch beyond the
+    per-call limit (an error before any work)."""
+    s, sysd, pc, B = _setup(ctx)
+    n = B.shape[1]
+    Bx = np.stack([B[0], np.zeros(n), B[1], B[0], 2 * B[1], B[2], B[3]])
+    k = Bx.shape[0]
+    Bd = api.DeviceArray.from_numpy(Bx, np.complex128, ctx)
+    Xd = api.DeviceArray((k, n), np.complex128, ctx)
+    flags, relres, iters = api.bicgstab_batch_dev(sysd.op, Bd, Xd, 1e-8, 2000, pc)
+    X = Xd.numpy()
+    print(f"  edge batch: flags={flags} iters={iters}")
+    assert np.all(flags == 0)
+    assert np.all(X[1] == 0) and relres[1] == 0
+    assert _rel(X[3], X[0]) <= 1e-12 and iters[3] == iters[0]
+    assert _rel(X[4], 2 * X[2]) <= 1e-6
+    big_b = api.DeviceArray((200, n), np.complex128, ctx)
+    big_x = api.DeviceArray((200, n), np.complex128, ctx)
+    with pytest.raises(api.VoxScatterError, match="too many right-hand sides"):
+        api.bicgstab_batch_dev(sysd.op, big_b, big_x, 1e-8, 10, pc)
