@@ -34,9 +34,11 @@ def _parse():
                     help="FP32 inner-solve tolerance (default tol/3: one inner solve, the FP64 residual confirms it)")
     ap.add_argument("--inner", default="bicgstab", choices=["bicgstab", "gmres"])
     ap.add_argument("--sweep", action="store_true", help="also time other inner solvers/tolerances (stderr)")
-    ap.add_argument("--workload", default="koch", choices=["koch", "particles"],
+    ap.add_argument("--workload", default="koch", choices=["koch", "particles", "galerkin"],
                     help="koch: the 128x128x16 prism (default); particles: seeded particles of grids 24^3..96^3, "
-                         "2 x ndir plane waves each, batched FP32 + FP64-refinement solves")
+                         "2 x ndir plane waves each, batched FP32 + FP64-refinement solves; galerkin: the "
+                         "high-contrast 64^3 Galerkin cube, FP64 GMRES(100) against the precision form")
+    ap.add_argument("--seed", type=int, default=0, help="galerkin workload: seed of the permittivity field")
     ap.add_argument("--particles", type=int, default=96, help="particles workload: number of particles")
     ap.add_argument("--ndir", type=int, default=32, help="particles workload: incident directions per particle")
     ap.add_argument("--matvec-only", action="store_true",
@@ -144,6 +146,96 @@ def particles(args, api, geometry, ctx):
     print(json.dumps(out), flush=True)
 
 
+def galerkin_eps(L, M, N, seed, sigma=4.0):
+    """m = 1.5 + 2u + 0.01i, u = uniform noise smoothed by a periodic Gaussian of sigma voxels, min-max scaled."""
+    import math
+    rng = np.random.default_rng(seed)
+    u = rng.uniform(size=(L, M, N))
+    k = [np.fft.fftfreq(n) for n in (L, M, N)]
+    KX, KY, KZ = np.meshgrid(*k, indexing="ij")
+    g = np.exp(-2 * (math.pi * sigma) ** 2 * (KX ** 2 + KY ** 2 + KZ ** 2))
+    sm = np.real(np.fft.ifftn(np.fft.fftn(u) * g))
+    sm = (sm - sm.min()) / (sm.max() - sm.min())
+    return (1.5 + 2.0 * sm + 0.01j) ** 2
+
+
+def galerkin(args, api, ctx):
+    """High-contrast Galerkin cube: 64^3, m in [1.5, 3.5] + 0.01i (seeded smooth random field, sigma 4
+    voxels), voxel lambda0 / 70, form 2 with the averaged-contrast 2-level preconditioner.  One step =
+    one solve to relative residual 1e-6 from a device-resident right-hand side: FP64 GMRES(100), and the
+    precision form (FP32 matvec and blocks, inner GMRES(100) to 1e-3, FP64 refinement).  value = the
+    precision form's time; the internal fields of the two solutions are compared."""
+    import math
+    from voxscatter_amd.capi import VS_FP32, VS_FP64
+    from voxscatter_amd.galerkin import GalerkinSystem
+    L = M = N = 64
+    eps = galerkin_eps(L, M, N, args.seed)
+    sysg = GalerkinSystem.build(eps, 1 / 70, 2 * math.pi, precisions=VS_FP64 | VS_FP32, ctx=ctx)
+    pc = sysg.preconditioner(2, precisions=VS_FP64 | VS_FP32)
+    op = sysg.op
+    b = api.DeviceArray.from_numpy(sysg.b, np.complex128, ctx)
+    x64 = api.DeviceArray(op.n, np.complex128, ctx)
+    xr = api.DeviceArray(op.n, np.complex128, ctx)
+
+    def run64():
+        return api.gmres_dev(op, b.ptr, x64.ptr, 100, TOL, 20, pc)
+
+    def runr():
+        return api.refine_dev(op, b.ptr, xr.ptr, pc, tol=TOL, max_outer=10, inner="gmres", inner_tol=1e-3,
+                              restart=100)
+
+    def timed(f):
+        for _ in range(args.warmup):
+            f()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            res = f()
+        ctx.synchronize()
+        return (time.perf_counter() - t0) / args.steps, res
+
+    t64, (f64, rr64, it64, _) = timed(run64)
+    tr, (fr, rrr, outer, inner) = timed(runr)
+    X64, XR = x64.numpy(), xr.numpy()
+    nb = np.linalg.norm(sysg.b)
+    r64 = float(np.linalg.norm(op.mvp(X64) - sysg.b) / nb)
+    rr = float(np.linalg.norm(op.mvp(XR) - sysg.b) / nb)
+    E64, Er = sysg.total_field(X64), sysg.total_field(XR)
+    err = float(np.linalg.norm((Er - E64).ravel()) / np.linalg.norm(E64.ravel()))
+    print(f"[bench] galerkin: FP64 GMRES(100) {t64 * 1e3:.1f} ms iter={it64} flag={f64}; refined {tr * 1e3:.1f} ms "
+          f"outer={outer} inner={inner} flag={fr}; field diff {err:.2e}", file=sys.stderr, flush=True)
+    out = {
+        "metric": "Galerkin time-to-solution @rel-res 1e-6 (FP32+refine vs FP64 GMRES(100)), internal-field error",
+        "value": round(tr * 1e3, 2),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(tr * 1e3, 2),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64 inner / c128 refinement",
+        "data": "synthetic",
+        "config": {"workload": f"Galerkin JVIE 64^3 cube, m in [1.5,3.5]+0.01i (seed {args.seed}, sigma 4 voxels), "
+                               "voxel lambda/70, 2-level averaged-contrast preconditioner, GMRES(100)",
+                   "unknowns": int(op.n)},
+        "fp64_gmres100_ms": round(t64 * 1e3, 2),
+        "fp64_gmres100_iters": list(it64),
+        "fp64_flag": f64,
+        "refine_flag": fr,
+        "refine_outer": outer,
+        "refine_inner_iterations": inner,
+        "relres_fp64": r64,
+        "relres_refined": rr,
+        "internal_field_rel_diff": err,
+        "speedup_vs_fp64": round(t64 / tr, 3),
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def cpu_baseline(scat, sysd, gpu_iters: float, n_iters: int):
     """Oracle (NumPy/SciPy FP64 restatement of the MATLAB path) on the host cores: time a bounded
     number of preconditioned BiCGStab iterations and scale to the GPU FP64 iteration count."""
@@ -199,6 +291,9 @@ def main():
     ctx = api.get_context()
     if args.workload == "particles":
         particles(args, api, geometry, ctx)
+        return
+    if args.workload == "galerkin":
+        galerkin(args, api, ctx)
         return
     scat = geometry.koch_prism(128, 128, 16, levels=3, kR=20.0, m=1.31 + 0.01j)
     L, M, N = scat.shape
