@@ -153,53 +153,72 @@ __global__ __launch_bounds__(RED_THREADS) void k_update(int op, UpdArgs<T> a, in
 }
 
 // GMRES helpers: partial[blk*(k) + j] = sum_i conj(V_j[i]) w[i]  for j < k
+// Each lane keeps up to IPT of its elements of w in registers while the basis streams past four vectors
+// at a time, so w is read once (per chunk of IPT grid strides) instead of once per four basis vectors;
+// with more chunks the per-vector sums accumulate in LDS.
 template <typename T>
 __global__ __launch_bounds__(RED_THREADS) void k_mdot(const cplx<T>* V, int64_t ldv, int k, const cplx<T>* w, int64_t n,
                                                       c128* partial) {
-  // four basis vectors per pass: their loads are in flight together and w is read once per pass
-  constexpr int JB = 4;
+  constexpr int JB = 4, IPT = 8;
   __shared__ double sre[JB][RED_THREADS / 64], sim[JB][RED_THREADS / 64];
+  __shared__ c128 sacc[512];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int j0 = 0; j0 < k; j0 += JB) {
-    double re[JB], im[JB];
-    const cplx<T>* v[JB];
+  const int64_t stride = (int64_t)gridDim.x * RED_THREADS;
+  for (int j = threadIdx.x; j < k; j += RED_THREADS) sacc[j] = mk<double>(0, 0);
+  __syncthreads();
+  for (int64_t i0 = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; i0 - threadIdx.x < n; i0 += IPT * stride) {
+    cplx<T> wv[IPT];
 #pragma unroll
-    for (int q = 0; q < JB; ++q) {
-      re[q] = im[q] = 0.0;
-      v[q] = V + (int64_t)min(j0 + q, k - 1) * ldv;
+    for (int t = 0; t < IPT; ++t) {
+      const int64_t i = i0 + t * stride;
+      wv[t] = i < n ? w[i] : mk<T>(0, 0);
     }
-    for (int64_t i = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * RED_THREADS) {
-      const cplx<T> b = w[i];
+    for (int j0 = 0; j0 < k; j0 += JB) {
+      double re[JB], im[JB];
+      const cplx<T>* v[JB];
 #pragma unroll
       for (int q = 0; q < JB; ++q) {
-        const cplx<T> a = v[q][i];
-        re[q] += (double)a.x * b.x + (double)a.y * b.y;
-        im[q] += (double)a.x * b.y - (double)a.y * b.x;
+        re[q] = im[q] = 0.0;
+        v[q] = V + (int64_t)min(j0 + q, k - 1) * ldv;
       }
-    }
 #pragma unroll
-    for (int q = 0; q < JB; ++q) {
-      for (int off = 32; off >= 1; off >>= 1) {
-        re[q] += __shfl_xor(re[q], off);
-        im[q] += __shfl_xor(im[q], off);
+      for (int t = 0; t < IPT; ++t) {
+        const int64_t i = i0 + t * stride;
+        if (i < n) {
+          const cplx<T> b = wv[t];
+#pragma unroll
+          for (int q = 0; q < JB; ++q) {
+            const cplx<T> a = v[q][i];
+            re[q] += (double)a.x * b.x + (double)a.y * b.y;
+            im[q] += (double)a.x * b.y - (double)a.y * b.x;
+          }
+        }
       }
-      if (lane == 0) {
-        sre[q][wave] = re[q];
-        sim[q][wave] = im[q];
+#pragma unroll
+      for (int q = 0; q < JB; ++q) {
+        for (int off = 32; off >= 1; off >>= 1) {
+          re[q] += __shfl_xor(re[q], off);
+          im[q] += __shfl_xor(im[q], off);
+        }
+        if (lane == 0) {
+          sre[q][wave] = re[q];
+          sim[q][wave] = im[q];
+        }
       }
+      __syncthreads();
+      if (threadIdx.x < JB && j0 + (int)threadIdx.x < k) {
+        const int q = threadIdx.x;
+        double a = 0, b = 0;
+        for (int r = 0; r < RED_THREADS / 64; ++r) {
+          a += sre[q][r];
+          b += sim[q][r];
+        }
+        sacc[j0 + q] = sacc[j0 + q] + mk<double>(a, b);
+      }
+      __syncthreads();
     }
-    __syncthreads();
-    if (threadIdx.x < JB && j0 + (int)threadIdx.x < k) {
-      const int q = threadIdx.x;
-      double a = 0, b = 0;
-      for (int r = 0; r < RED_THREADS / 64; ++r) {
-        a += sre[q][r];
-        b += sim[q][r];
-      }
-      partial[(int64_t)blockIdx.x * k + j0 + q] = mk<double>(a, b);
-    }
-    __syncthreads();
   }
+  for (int j = threadIdx.x; j < k; j += RED_THREADS) partial[(int64_t)blockIdx.x * k + j] = sacc[j];
 }
 
 // w += sgn * sum_j V_j h_j  (h in FP64 on the device)
