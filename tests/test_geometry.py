@@ -47,3 +47,35 @@ def test_workload_geometries():
     s = G.koch_prism()
     assert s.shape == (128, 128, 16)
     assert abs(s.k0 * s.d - 0.3125) < 1e-12
+
+
+def test_particle_set_and_plane_wave_batch():
+    """The seeded orientation-averaging set (96 particles, seed 0) and its plane-wave right-hand sides:
+    grid counts per size, unit directions, orthonormal polarisations, and rows equal to E0 exp(i k.r) at
+    the occupied voxels (PlaneWavePointWise.m:7-19, evaluated directly)."""
+    from collections import Counter
+    parts = G.particle_batch(count=96, seed=0)
+    assert Counter(p.r.shape[0] for p in parts) == {96: 26, 64: 22, 48: 19, 32: 11, 24: 18}
+    assert all(p.idx.size > 0 for p in parts)
+    p = [q for q in parts if q.r.shape[0] == 24][0]
+    B, meta = G.plane_wave_batch(p, ndir=5)
+    assert B.shape == (10, 3 * p.idx.size)
+    rv = p.r.reshape(-1, 3, order="F")[p.idx]
+    for row, (khat, e) in zip(B, meta):
+        assert abs(np.linalg.norm(khat) - 1) < 1e-12 and abs(np.dot(khat, e)) < 1e-12
+        ph = np.exp(1j * p.k0 * (rv @ khat))
+        ref = np.concatenate([e[0] * ph, e[1] * ph, e[2] * ph])
+        assert np.abs(row - ref).max() <= 1e-12
+    e1, e2 = meta[0][1], meta[1][1]
+    assert abs(np.dot(e1, e2)) < 1e-12
+
+
+def test_bench_refuses_several_gpus():
+    """bench.py targets one GPU: --gpus 2 exits with status 2 before any device work."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    res = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                         text=True, timeout=120)
+    assert res.returncode == 2 and "one GPU" in res.stderr
