@@ -223,9 +223,10 @@ int vs_dda_cross_sections(vs_ctx* ctx, int64_t nvox, const double* P, const doub
 int vs_dda_grad_eps(vs_ctx* ctx, int64_t nvox, const double* P, const double* lam, const double* dainv, double* g);
 
 /* Batched solves (no reference counterpart; the batching form of the solve loop): nrhs right-hand
- * sides B (nrhs consecutive device vectors) solved together, every matvec and preconditioner launch
- * covering the whole batch, one captured graph per iteration.  flags/relres/iters receive nrhs values;
- * flag 0 = true relative residual <= tol, 1 = not reached, 4 = breakdown. */
+ * sides B (nrhs consecutive device vectors, nrhs <= 186) solved together, every matvec and
+ * preconditioner launch covering the systems still running (finished ones drop out of the batch).
+ * flags/relres/iters receive nrhs values; flag 0 = true relative residual <= tol, 1 = not reached,
+ * 4 = breakdown. */
 int vs_bicgstab_batch_dev(vs_op* op, vs_pc* pc, int nrhs, const void* B, void* X, int precision, double tol,
                           int maxit, int* flags, double* relres, double* iters);
 /* Batched FP32-inner / FP64-refinement solve; B, X are FP64 device vectors. */
