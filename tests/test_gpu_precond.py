@@ -121,3 +121,32 @@ def test_mirrored_two_level_apply(ctx, shape, weights):
     Yd = api.DeviceArray(X.shape, np.complex128, ctx)
     pc.apply_dev(Xd.ptr, Yd.ptr, nrhs=5)
     assert _rel(Yd.numpy(), Yref) <= 1e-11
+
+
+@pytest.mark.parametrize("which", ["plate32x32x8", "koch128x128x16"])
+def test_fp32_apply_matches_fp64(ctx, which):
+    """FP32 2-level apply of one right-hand side against the FP64 apply (relative difference <= 2e-5,
+    FP32 rounding), and against the FP32 apply of a two-right-hand-side batch (two right-hand sides per
+    block pass) to the same level, on the 32x32x8 plate and the full-size Koch prism."""
+    from voxscatter_amd.capi import VS_FP32, VS_FP64
+    from voxscatter_amd.dda import DDASystem
+    print(f"[fp32 apply] {which}", flush=True)
+    s = G.hex_plate() if which.startswith("plate") else G.koch_prism(128, 128, 16, levels=3, kR=20.0,
+                                                                        m=1.31 + 0.01j)
+    sysd = DDASystem.build(s, precisions=VS_FP64 | VS_FP32, ctx=ctx)
+    pc = sysd.preconditioner(2, precisions=VS_FP64 | VS_FP32)
+    n = sysd.op.n
+    rng = np.random.default_rng(5)
+    X = rng.normal(size=(2, n)) + 1j * rng.normal(size=(2, n))
+    y64 = api.DeviceArray((1, n), np.complex128, ctx)
+    x64 = api.DeviceArray.from_numpy(X[:1], np.complex128, ctx)
+    pc.apply_dev(x64.ptr, y64.ptr, VS_FP64, 1)
+    x32 = api.DeviceArray.from_numpy(X, np.complex64, ctx)
+    y32 = api.DeviceArray((2, n), np.complex64, ctx)
+    pc.apply_dev(x32.ptr, y32.ptr, VS_FP32, 1)
+    one = y32.numpy()[0].astype(np.complex128)
+    ref = y64.numpy()[0]
+    assert _rel(one, ref) <= 2e-5
+    pc.apply_dev(x32.ptr, y32.ptr, VS_FP32, 2)
+    two = y32.numpy()[0].astype(np.complex128)
+    assert _rel(two, ref) <= 2e-5 and _rel(one, two) <= 2e-5
