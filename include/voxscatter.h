@@ -87,6 +87,11 @@ int vs_op_create_galerkin(vs_ctx* ctx, int L, int M, int N, double dx, double ko
  * as 5 interleaved complex doubles (replaces singular_st_const.m, singular_ea_const.m,
  * singular_va_const.m with this library's own semi-analytic method instead of DIRECTFN). */
 int vs_galerkin_singular(vs_ctx* ctx, double ko, double dx, double out[10]);
+/* The face-pair table of the 2 x 2 x 2 near box (replaces the matrices C and P of nearby_correct.m:1-143):
+ * for cell c = mx + 2 my + 4 mz (0-based offsets) and observation / source faces i, j (cube_faces.m order
+ * -x, +x, -y, +y, -z, +z), code[36 c + 6 i + j] is 1 ST, 2 EAc, 3 EAo, 4 VAc, 5 VAo or 0, and keep[...] is
+ * 1 where the cubature is kept.  Host data only (no device needed). */
+int vs_galerkin_near_table(int* code, int* keep);
 
 int vs_op_create_fN(vs_ctx* ctx, int L, int M, int N, const double* fN, const double* Toep,
                     int precisions, vs_op** out);

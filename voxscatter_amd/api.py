@@ -18,14 +18,42 @@ Differences forced by the language boundary: indices are 0-based; an operator "f
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import hashlib
+import weakref
 from dataclasses import dataclass
 
 import numpy as np
 
 from . import capi
 from .capi import VS_FP32, VS_FP64, VoxScatterError, c2, check, cplx_arr, dptr, i64ptr
+
+# --------------------------------------------------------------------------------------------------
+# handle lifetime
+# --------------------------------------------------------------------------------------------------
+# The library keeps a context alive while any handle made on it lives (reference counted), so handles
+# may be released in any order.  Every live handle is also registered here, and an exit hook releases
+# them children first (buffers, checkpoints, preconditioners, operators, then contexts) while the
+# interpreter and the HIP runtime are still whole, instead of leaving the order to finalisers.
+_LIVE = {kind: weakref.WeakSet() for kind in ("array", "recycle", "pc", "op", "ctx")}
+
+
+def _track(kind: str, obj):
+    _LIVE[kind].add(obj)
+
+
+def close_all():
+    """Release every live handle of this process, children first."""
+    for kind in ("array", "recycle", "pc", "op", "ctx"):
+        for obj in list(_LIVE[kind]):
+            try:
+                obj.close()
+            except Exception:
+                pass
+
+
+atexit.register(close_all)
 
 # --------------------------------------------------------------------------------------------------
 # context
@@ -41,6 +69,7 @@ class Context:
         check(self.lib.vs_ctx_create(int(device), C.byref(h)))
         self.h = h
         self.device = device
+        _track("ctx", self)
 
     def synchronize(self):
         check(self.lib.vs_ctx_synchronize(self.h))
@@ -62,7 +91,7 @@ _default_ctx: Context | None = None
 
 def get_context() -> Context:
     global _default_ctx
-    if _default_ctx is None:
+    if _default_ctx is None or not getattr(_default_ctx, "h", None):
         _default_ctx = Context(0)
     return _default_ctx
 
@@ -108,6 +137,8 @@ class DeviceArray:
         p = C.c_void_p()
         check(self.ctx.lib.vs_dev_alloc(self.ctx.h, self.nbytes, C.byref(p)))
         self.ptr = p.value
+        self._ctxh = self.ctx.h  # the buffer keeps its context alive in the library
+        _track("array", self)
 
     @property
     def precision(self) -> int:
@@ -136,7 +167,7 @@ class DeviceArray:
 
     def close(self):
         if getattr(self, "ptr", None):
-            self.ctx.lib.vs_dev_free(self.ctx.h, C.c_void_p(self.ptr))
+            self.ctx.lib.vs_dev_free(self._ctxh, C.c_void_p(self.ptr))
             self.ptr = None
 
     def __del__(self):
@@ -161,6 +192,7 @@ class Operator:
         self.precisions = precisions
         self._domain_key = None
         self.n = 0
+        _track("op", self)
 
     # construction ---------------------------------------------------------------------------------
     @classmethod
@@ -282,6 +314,7 @@ class Preconditioner:
         self.ctx = ctx
         self.levels = levels
         self.n = n
+        _track("pc", self)
 
     @classmethod
     def from_operator(cls, op: Operator, levels: int, a, w=None, wscalar=None, precisions: int = VS_FP64):
@@ -558,6 +591,7 @@ class Recycle:
         h = C.c_void_p()
         check(self.ctx.lib.vs_recycle_create(self.ctx.h, C.byref(h)))
         self.h = h
+        _track("recycle", self)
 
     @property
     def size(self) -> int:
@@ -567,11 +601,17 @@ class Recycle:
     def clear(self):
         check(self.ctx.lib.vs_recycle_clear(self.h))
 
-    def __del__(self):
+    def close(self):
         h = getattr(self, "h", None)
         if h is not None and h.value:
             self.ctx.lib.vs_recycle_destroy(h)
-            self.h = None
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def pgmresDR(A, b, restart=50, tol=1e-3, maxit=10, ritz=None, L1=None, L2=None, R1=None, R2=None, x0=None,

@@ -61,6 +61,7 @@ SIGNATURES = {
     "vs_op_create_galerkin": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, D, C.c_int,
                                         C.POINTER(P)]),
     "vs_galerkin_singular": (C.c_int, [P, C.c_double, C.c_double, D]),
+    "vs_galerkin_near_table": (C.c_int, [C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "vs_op_matvec_variant": (C.c_int, [P, C.c_int, C.POINTER(C.c_int)]),
     "vs_mvp": (INT, [P, D, D, INT]),
     "vs_mvp_dev": (INT, [P, P, P, INT, INT, INT]),

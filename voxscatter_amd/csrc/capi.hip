@@ -27,6 +27,22 @@ void pc_store_inverse(vs_pc* pc, const c128* inv);
 
 using namespace vs;
 
+void vs::ctx_retain(vs_ctx* ctx) { ctx->refs.fetch_add(1, std::memory_order_relaxed); }
+
+void vs::ctx_release(vs_ctx* ctx) {
+  if (ctx->refs.fetch_sub(1, std::memory_order_acq_rel) != 1) return;
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  ctx->plans.clear();
+  for (auto& w : ctx->ws) w.release();
+  ctx->red.release();
+  ctx->red_out.release();
+  if (ctx->pinned) hipHostFree(ctx->pinned);
+  if (ctx->pinned_up) hipHostFree(ctx->pinned_up);
+  hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
 namespace {
 
 void check_dims(int L, int M, int N) {
@@ -137,15 +153,7 @@ int vs_ctx_create(int device, vs_ctx** out) {
 
 void vs_ctx_destroy(vs_ctx* ctx) {
   if (!ctx) return;
-  hipSetDevice(ctx->device);
-  hipStreamSynchronize(ctx->stream);
-  ctx->plans.clear();
-  ctx->red.release();
-  ctx->red_out.release();
-  if (ctx->pinned) hipHostFree(ctx->pinned);
-  if (ctx->pinned_up) hipHostFree(ctx->pinned_up);
-  hipStreamDestroy(ctx->stream);
-  delete ctx;
+  ctx_release(ctx);  // the stream and memory go with the last handle made on this context
 }
 
 int vs_ctx_synchronize(vs_ctx* ctx) {
@@ -166,15 +174,30 @@ int vs_dev_alloc(vs_ctx* ctx, int64_t bytes, void** ptr) {
       *ptr = nullptr;
       throw Error{VS_ERR_OOM, "hipMalloc failed: " + std::string(hipGetErrorString(e))};
     }
+    {
+      std::lock_guard<std::mutex> lk(ctx->dev_mu);
+      ctx->dev_ptrs.insert(*ptr);
+    }
+    ctx_retain(ctx);
   });
 }
 
 int vs_dev_free(vs_ctx* ctx, void* ptr) {
   return guarded([&] {
     VS_CHECK(ctx != nullptr, VS_ERR_ARG, "null context");
-    DeviceGuard g(ctx->device);
-    VS_HIP(hipStreamSynchronize(ctx->stream));
-    if (ptr) VS_HIP(hipFree(ptr));
+    if (!ptr) return;
+    {
+      std::lock_guard<std::mutex> lk(ctx->dev_mu);
+      VS_CHECK(ctx->dev_ptrs.erase(ptr) == 1, VS_ERR_ARG, "pointer was not allocated by vs_dev_alloc on this context");
+    }
+    {
+      DeviceGuard g(ctx->device);
+      hipError_t e1 = hipStreamSynchronize(ctx->stream);
+      hipError_t e2 = hipFree(ptr);
+      ctx_release(ctx);
+      VS_HIP(e1);
+      VS_HIP(e2);
+    }
   });
 }
 
@@ -232,6 +255,7 @@ int vs_op_create_dda(vs_ctx* ctx, int L, int M, int N, const double* r, double k
     op->P = 8 * op->V;
     op->precisions = precisions;
     op_build_dda(op.get(), r, ko, nearby_quad);
+    ctx_retain(ctx);
     *out = op.release();
   });
 }
@@ -280,6 +304,7 @@ int vs_op_create_galerkin(vs_ctx* ctx, int L, int M, int N, double dx, double ko
     op_build_fN(op.get());
     VS_HIP(hipStreamSynchronize(ctx->stream));
     if (!(op->precisions & VS_FP64)) op->fN64.release();
+    ctx_retain(ctx);
     *out = op.release();
   });
 }
@@ -310,15 +335,18 @@ int vs_op_create_fN(vs_ctx* ctx, int L, int M, int N, const double* fN, const do
     op_compress_tensor(op.get());
     VS_HIP(hipStreamSynchronize(ctx->stream));
     if (!(precisions & VS_FP64)) op->fN64.release();
+    ctx_retain(ctx);
     *out = op.release();
   });
 }
 
 void vs_op_destroy(vs_op* op) {
   if (!op) return;
-  hipSetDevice(op->ctx->device);
-  hipStreamSynchronize(op->ctx->stream);
+  vs_ctx* ctx = op->ctx;
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
   delete op;
+  ctx_release(ctx);
 }
 
 int vs_op_get_toeplitz(vs_op* op, double* Toep) {
@@ -543,6 +571,7 @@ int vs_pc_create_circ(vs_op* op, int levels, const double a[2], const double* w,
     pc->sym = levels == 2 && w == nullptr;
     pc_store_inverse(pc.get(), B.as<c128>());
     VS_HIP(hipStreamSynchronize(ctx->stream));
+    ctx_retain(ctx);
     *out = pc.release();
   });
 }
@@ -579,15 +608,18 @@ int vs_pc_create_blocks(vs_ctx* ctx, int levels, int L, int M, int N, const doub
     h2d(ctx, B.p, inv_blocks, sizeof(c128) * nblk * nb * nb);
     pc_store_inverse(pc.get(), B.as<c128>());
     VS_HIP(hipStreamSynchronize(ctx->stream));
+    ctx_retain(ctx);
     *out = pc.release();
   });
 }
 
 void vs_pc_destroy(vs_pc* pc) {
   if (!pc) return;
-  hipSetDevice(pc->ctx->device);
-  hipStreamSynchronize(pc->ctx->stream);
+  vs_ctx* ctx = pc->ctx;
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
   delete pc;
+  ctx_release(ctx);
 }
 
 int vs_pc_apply(vs_pc* pc, const double* x, double* y) {
@@ -738,15 +770,18 @@ int vs_recycle_create(vs_ctx* ctx, vs_recycle** out) {
     VS_CHECK(ctx && out, VS_ERR_ARG, "null pointer");
     auto rec = std::make_unique<vs_recycle>();
     rec->ctx = ctx;
+    ctx_retain(ctx);
     *out = rec.release();
   });
 }
 
 void vs_recycle_destroy(vs_recycle* rec) {
   if (!rec) return;
-  DeviceGuard g(rec->ctx->device);
-  (void)hipStreamSynchronize(rec->ctx->stream);
+  vs_ctx* ctx = rec->ctx;
+  DeviceGuard g(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
   delete rec;
+  ctx_release(ctx);
 }
 
 int vs_recycle_size(const vs_recycle* rec) { return rec ? rec->r.k : 0; }

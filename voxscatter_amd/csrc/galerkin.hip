@@ -365,59 +365,33 @@ void galerkin_singular(vs_ctx* ctx, double ko, double dx, c128 out[5]) {
   }
 }
 
-// nearby_correct.m: for the 8 cells of the 2 x 2 x 2 box, the face pairs whose mask is 0 take the listed
-// singular integral (1 ST, 2 EAc, 3 EAo, 4 VAc, 5 VAo) or zero (code 0); mask 1 keeps the cubature
+// nearby_correct.m: in the 2 x 2 x 2 box the face pairs that touch take a singular integral instead of
+// the cubature: code 1 ST (the squares coincide), 2 EAc / 3 EAo (they share an edge, coplanar /
+// orthogonal), 4 VAc / 5 VAo (they share a vertex only); keep = 1 for disjoint squares, which keep the
+// cubature.  The table is derived here from the face geometry (observation face on the cell at offset
+// (mx, my, mz), source face on cell 0, both closed axis-aligned unit squares); it equals the matrices
+// written out in nearby_correct.m.
+static const int h_face[6][3][3] = {
+    {{0, 0, 0}, {0, 1, 0}, {0, 0, 1}}, {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}}, {{0, 0, 0}, {1, 0, 0}, {0, 0, 1}},
+    {{0, 1, 0}, {1, 0, 0}, {0, 0, 1}}, {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}}, {{0, 0, 1}, {1, 0, 0}, {0, 1, 0}}};
+
 static void nearby_table(int mx, int my, int mz, int code[6][6], int keep[6][6]) {
-  // rows: observation face, columns: source face; cells indexed 0-based (mx, my, mz)
-  static const int C111[6][6] = {{1, 0, 3, 3, 3, 3}, {0, 1, 3, 3, 3, 3}, {3, 3, 1, 0, 3, 3},
-                                 {3, 3, 0, 1, 3, 3}, {3, 3, 3, 3, 1, 0}, {3, 3, 3, 3, 0, 1}};
-  static const int P111[6][6] = {{0, 1, 0, 0, 0, 0}, {1, 0, 0, 0, 0, 0}, {0, 0, 0, 1, 0, 0},
-                                 {0, 0, 1, 0, 0, 0}, {0, 0, 0, 0, 0, 1}, {0, 0, 0, 0, 1, 0}};
-  static const int C211[6][6] = {{0, 1, 3, 3, 3, 3}, {0, 0, 0, 0, 0, 0}, {0, 3, 2, 0, 5, 5},
-                                 {0, 3, 0, 2, 5, 5}, {0, 3, 5, 5, 2, 0}, {0, 3, 5, 5, 0, 2}};
-  static const int P211[6][6] = {{1, 0, 0, 0, 0, 0}, {1, 1, 1, 1, 1, 1}, {1, 0, 0, 1, 0, 0},
-                                 {1, 0, 1, 0, 0, 0}, {1, 0, 0, 0, 0, 1}, {1, 0, 0, 0, 1, 0}};
-  static const int C212[6][6] = {{0, 2, 5, 5, 0, 3}, {0, 0, 0, 0, 0, 0}, {0, 5, 4, 0, 0, 5},
-                                 {0, 5, 0, 4, 0, 5}, {0, 3, 5, 5, 0, 2}, {0, 0, 0, 0, 0, 0}};
-  static const int P212[6][6] = {{1, 0, 0, 0, 1, 0}, {1, 1, 1, 1, 1, 1}, {1, 0, 0, 1, 1, 0},
-                                 {1, 0, 1, 0, 1, 0}, {1, 0, 0, 0, 1, 0}, {1, 1, 1, 1, 1, 1}};
-  static const int C112[6][6] = {{2, 0, 5, 5, 0, 3}, {0, 2, 5, 5, 0, 3}, {5, 5, 2, 0, 0, 3},
-                                 {5, 5, 0, 2, 0, 3}, {3, 3, 3, 3, 0, 1}, {0, 0, 0, 0, 0, 0}};
-  static const int P112[6][6] = {{0, 1, 0, 0, 1, 0}, {1, 0, 0, 0, 1, 0}, {0, 0, 0, 1, 1, 0},
-                                 {0, 0, 1, 0, 1, 0}, {0, 0, 0, 0, 1, 0}, {1, 1, 1, 1, 1, 1}};
-  static const int C121[6][6] = {{2, 0, 0, 3, 5, 5}, {0, 2, 0, 3, 5, 5}, {3, 3, 0, 1, 3, 3},
-                                 {0, 0, 0, 0, 0, 0}, {5, 5, 0, 3, 2, 0}, {5, 5, 0, 3, 0, 2}};
-  static const int P121[6][6] = {{0, 1, 1, 0, 0, 0}, {1, 0, 1, 0, 0, 0}, {0, 0, 1, 0, 0, 0},
-                                 {1, 1, 1, 1, 1, 1}, {0, 0, 1, 0, 0, 1}, {0, 0, 1, 0, 1, 0}};
-  static const int C221[6][6] = {{0, 2, 0, 3, 5, 5}, {0, 0, 0, 0, 0, 0}, {0, 3, 0, 2, 5, 5},
-                                 {0, 0, 0, 0, 0, 0}, {0, 5, 0, 5, 4, 0}, {0, 5, 0, 5, 0, 4}};
-  static const int P221[6][6] = {{1, 0, 1, 0, 0, 0}, {1, 1, 1, 1, 1, 1}, {1, 0, 1, 0, 0, 0},
-                                 {1, 1, 1, 1, 1, 1}, {1, 0, 1, 0, 0, 1}, {1, 0, 1, 0, 1, 0}};
-  static const int C122[6][6] = {{4, 0, 0, 5, 0, 5}, {0, 4, 0, 5, 0, 5}, {5, 5, 0, 2, 0, 3},
-                                 {0, 0, 0, 0, 0, 0}, {5, 5, 0, 3, 0, 2}, {0, 0, 0, 0, 0, 0}};
-  static const int P122[6][6] = {{0, 1, 1, 0, 1, 0}, {1, 0, 1, 0, 1, 0}, {0, 0, 1, 0, 1, 0},
-                                 {1, 1, 1, 1, 1, 1}, {0, 0, 1, 0, 1, 0}, {1, 1, 1, 1, 1, 1}};
-  static const int C222[6][6] = {{0, 4, 0, 5, 0, 5}, {0, 0, 0, 0, 0, 0}, {0, 5, 0, 4, 0, 5},
-                                 {0, 0, 0, 0, 0, 0}, {0, 5, 0, 5, 0, 4}, {0, 0, 0, 0, 0, 0}};
-  static const int P222[6][6] = {{1, 0, 1, 0, 1, 0}, {1, 1, 1, 1, 1, 1}, {1, 0, 1, 0, 1, 0},
-                                 {1, 1, 1, 1, 1, 1}, {1, 0, 1, 0, 1, 0}, {1, 1, 1, 1, 1, 1}};
-  const int id = mx + 2 * my + 4 * mz;
-  const int(*C)[6] = nullptr;
-  const int(*P)[6] = nullptr;
-  switch (id) {
-    case 0: C = C111; P = P111; break;
-    case 1: C = C211; P = P211; break;
-    case 2: C = C121; P = P121; break;
-    case 3: C = C221; P = P221; break;
-    case 4: C = C112; P = P112; break;
-    case 5: C = C212; P = P212; break;
-    case 6: C = C122; P = P122; break;
-    default: C = C222; P = P222; break;
-  }
-  for (int i = 0; i < 6; ++i)
-    for (int j = 0; j < 6; ++j) {
-      code[i][j] = C[i][j];
-      keep[i][j] = P[i][j];
+  const int off[3] = {mx, my, mz};
+  for (int fo = 0; fo < 6; ++fo)
+    for (int fs = 0; fs < 6; ++fs) {
+      int dim = 0, nrm_o = 0, nrm_s = 0;
+      bool empty = false;
+      for (int d = 0; d < 3; ++d) {
+        const int ext_o = h_face[fo][1][d] + h_face[fo][2][d], ext_s = h_face[fs][1][d] + h_face[fs][2][d];
+        if (!ext_o) nrm_o = d;
+        if (!ext_s) nrm_s = d;
+        const int lo = std::max(h_face[fo][0][d] + off[d], h_face[fs][0][d]);
+        const int hi = std::min(h_face[fo][0][d] + off[d] + ext_o, h_face[fs][0][d] + ext_s);
+        if (lo > hi) empty = true;
+        if (hi > lo) ++dim;
+      }
+      keep[fo][fs] = empty ? 1 : 0;
+      code[fo][fs] = empty ? 0 : (dim == 2 ? 1 : dim == 1 ? (nrm_o == nrm_s ? 2 : 3) : (nrm_o == nrm_s ? 4 : 5));
     }
 }
 
@@ -428,10 +402,9 @@ void galerkin_toeplitz(vs_ctx* ctx, int L, int M, int N, double dx, double ko, c
   // in units of dx the face-face sums carry dx^3: dx^4 of area Jacobians and 1/dx of 1/R
   const double scale = dx * dx * dx / (4.0 * M_PI);
   const ProjTable P = projection_table();
-  // medium box (assembly_nop_const.m:21-29) and near box (:96-146; the one-voxel-wide case has only the
-  // cells (0, 0, 0) and (0, 0, 1))
+  // medium box (assembly_nop_const.m:21-29) and near box (:96-146)
   const int nmL = std::min(5, L), nmM = std::min(5, M), nmN = std::min(5, N);
-  const int nnL = L == 1 ? 1 : 2, nnM = L == 1 ? 1 : 2, nnN = 2;
+  const int nnL = 2, nnM = 2, nnN = 2;  // Galerkin grids have L, M, N >= 2
   std::vector<double> rf = rule2d(4), rm = rule2d(10), rn = rule2d(20);
   DevBuf drf(sizeof(double) * rf.size()), drm(sizeof(double) * rm.size()), drn(sizeof(double) * rn.size());
   h2d(ctx, drf.p, rf.data(), sizeof(double) * rf.size());
@@ -483,3 +456,17 @@ void galerkin_toeplitz(vs_ctx* ctx, int L, int M, int N, double dx, double ko, c
 }
 
 }  // namespace vs
+
+extern "C" int vs_galerkin_near_table(int* code, int* keep) {
+  if (!code || !keep) return VS_ERR_ARG;
+  for (int cell = 0; cell < 8; ++cell) {
+    int c[6][6], k[6][6];
+    vs::nearby_table(cell & 1, (cell >> 1) & 1, cell >> 2, c, k);
+    for (int i = 0; i < 6; ++i)
+      for (int j = 0; j < 6; ++j) {
+        code[36 * cell + 6 * i + j] = c[i][j];
+        keep[36 * cell + 6 * i + j] = k[i][j];
+      }
+  }
+  return VS_OK;
+}

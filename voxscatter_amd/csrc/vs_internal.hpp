@@ -10,12 +10,15 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
+#include <unordered_set>
 #include <vector>
 
 #include "../../include/voxscatter.h"
@@ -156,6 +159,12 @@ constexpr int VS_PINNED_C128 = 2048;  // complex doubles in each pinned staging 
 struct vs_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // Lifetime: the caller's reference (released by vs_ctx_destroy) plus one per live operator,
+  // preconditioner, recycle handle and vs_dev_alloc buffer.  The stream and the context's memory are
+  // freed when the last reference goes, so handles may be destroyed in any order.
+  std::atomic<int> refs{1};
+  std::mutex dev_mu;
+  std::unordered_set<void*> dev_ptrs;  // live vs_dev_alloc buffers
   std::map<int, std::unique_ptr<vs::FftPlanHost>> plans;
   vs::DevBuf red;        // reduction partials
   vs::DevBuf red_out;    // reduced scalars
@@ -173,6 +182,10 @@ struct vs_ctx {
 };
 
 namespace vs {
+// context reference counting (see vs_ctx::refs)
+void ctx_retain(vs_ctx* ctx);
+void ctx_release(vs_ctx* ctx);
+
 struct DeviceGuard {
   int prev = -1;
   explicit DeviceGuard(int dev) {
