@@ -118,6 +118,15 @@ int vs_mvp(vs_op* op, const double* x, double* y, int transp);
 /* Device buffers of nrhs consecutive vectors in `precision` (VS_FP64 or VS_FP32), on the context
  * stream, asynchronous. */
 int vs_mvp_dev(vs_op* op, const void* x, void* y, int transp, int precision, int nrhs);
+/* vs_mvp_dev that allocates nothing (no counterpart in the reference: for callers that cannot allocate
+ * device memory -- code under HIP graph capture, tight serving loops, hosts that forbid hipMalloc).  The
+ * FFT scratch is a fixed 64 MiB device array, one per GPU, held until the process ends, instead of the
+ * operator's own buffer.  The operator's tuned variant is used when its scratch fits, else the variant
+ * with the smallest scratch; a call whose scratch exceeds 64 MiB (e.g. more right-hand sides than fit)
+ * returns VS_ERR_ARG, and so does one that would need an FFT plan the operator has not built.  Every call
+ * on one GPU shares that GPU's array (nothing in it needs to start at zero): calls that overlap on one
+ * GPU (two contexts' streams) race. */
+int vs_mvp_dev_noalloc(vs_op* op, const void* x, void* y, int transp, int precision, int nrhs);
 
 /* ---------------------------------------------------------------------------------------------- */
 /* circulant preconditioners                                                                       */

@@ -65,6 +65,7 @@ SIGNATURES = {
     "vs_op_matvec_variant": (C.c_int, [P, C.c_int, C.POINTER(C.c_int)]),
     "vs_mvp": (INT, [P, D, D, INT]),
     "vs_mvp_dev": (INT, [P, P, P, INT, INT, INT]),
+    "vs_mvp_dev_noalloc": (INT, [P, P, P, INT, INT, INT]),
     "vs_circ_1_level": (INT, [P, INT, INT, INT, D, D, D]),
     "vs_circ_2_level": (INT, [P, INT, INT, INT, D, D, D]),
     "vs_invert_blocks": (INT, [P, C.c_int64, INT, D, D, D, D, D]),

@@ -27,6 +27,28 @@ void pc_store_inverse(vs_pc* pc, const c128* inv);
 
 using namespace vs;
 
+// ------------------------------------------------------------------------------------------------
+// Scratch of vs_mvp_dev_noalloc: a fixed device array, one copy per GPU, held until the process ends.
+// In host code `scratch` names a host-side stand-in, never the device memory: its device address is
+// resolved once per device by hipGetSymbolAddress (with that device current, when the first context on
+// it is created) and only that address is handed to kernels.
+// ------------------------------------------------------------------------------------------------
+static constexpr size_t VS_NOALLOC_SCRATCH_BYTES = size_t(64) << 20;
+static constexpr int VS_MAX_DEVICES = 64;
+static __device__ __align__(256) unsigned char scratch[VS_NOALLOC_SCRATCH_BYTES];
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[VS_MAX_DEVICES] = {};
+
+static void resolve_scratch(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  VS_CHECK(device >= 0 && device < VS_MAX_DEVICES, VS_ERR_UNSUPPORTED, "device index beyond the scratch table");
+  if (g_scratch_ptr[device]) return;
+  void* ptr = nullptr;
+  VS_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  VS_CHECK(ptr != nullptr, VS_ERR_HIP, "hipGetSymbolAddress returned no address for the matvec scratch");
+  g_scratch_ptr[device] = ptr;
+}
+
 void vs::ctx_retain(vs_ctx* ctx) { ctx->refs.fetch_add(1, std::memory_order_relaxed); }
 
 void vs::ctx_release(vs_ctx* ctx) {
@@ -144,6 +166,7 @@ int vs_ctx_create(int device, vs_ctx** out) {
     // as soon as the work does (a device whose runtime is already active keeps its flags)
     const char* sp = std::getenv("VS_SPIN");
     if (!(sp && sp[0] == '0') && hipSetDeviceFlags(hipDeviceScheduleSpin) != hipSuccess) (void)hipGetLastError();
+    resolve_scratch(device);  // with `device` current
     auto ctx = std::make_unique<vs_ctx>();
     ctx->device = device;
     VS_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
@@ -440,6 +463,18 @@ int vs_mvp_dev(vs_op* op, const void* x, void* y, int transp, int precision, int
     VS_CHECK(op && x && y, VS_ERR_ARG, "null pointer");
     DeviceGuard g(op->ctx->device);
     mvp_dev(op, x, y, transp ? 1 : 0, precision, nrhs);
+  });
+}
+
+int vs_mvp_dev_noalloc(vs_op* op, const void* x, void* y, int transp, int precision, int nrhs) {
+  return guarded([&] {
+    VS_CHECK(op && x && y, VS_ERR_ARG, "null pointer");
+    const int dev = op->ctx->device;
+    void* sp = (dev >= 0 && dev < VS_MAX_DEVICES) ? g_scratch_ptr[dev] : nullptr;
+    VS_CHECK(sp != nullptr, VS_ERR_INTERNAL, "matvec scratch address not resolved for this device");
+    DeviceGuard g(dev);
+    NoAllocScope no_alloc;  // any allocation on this path becomes a VS_ERR_ARG error instead
+    mvp_dev_scratch(op, x, y, transp ? 1 : 0, precision, nrhs, sp, VS_NOALLOC_SCRATCH_BYTES);
   });
 }
 

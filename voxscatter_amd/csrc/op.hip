@@ -1800,14 +1800,42 @@ static void launch_yz(vs_ctx* ctx, cplx<T>* S, const TensorS<T>& fN, int L, int 
   }
 }
 
+// scratch bytes of one matvec call of a variant: five passes 2L x 2M x N x 3 per right-hand side, slab
+// variants 2L slabs of 3MN, the two-workgroup parity split three such volumes
 template <typename T>
-static void mvp_impl(vs_op* op, const cplx<T>* x, cplx<T>* y, int transp, int nrhs) {
+static size_t mvp_scratch_bytes(const vs_op* op, int variant, int nrhs) {
+  const int64_t Sr = 2LL * op->L * op->M * op->N * 3;
+  const int64_t per = variant == 0 ? 2 * Sr : (variant == 3 ? 3 * Sr : Sr);
+  return sizeof(cplx<T>) * (size_t)per * (size_t)nrhs;
+}
+
+// ext != nullptr: the scratch is the caller's ext_bytes buffer (nothing is allocated), and the variant is
+// the tuned one if its scratch fits, else the available variant with the smallest scratch
+template <typename T>
+static void mvp_impl(vs_op* op, const cplx<T>* x, cplx<T>* y, int transp, int nrhs, cplx<T>* ext = nullptr,
+                     size_t ext_bytes = 0) {
   vs_ctx* ctx = op->ctx;
   const int L = op->L, M = op->M, N = op->N;
   const int L2 = 2 * L, M2 = 2 * M, N2 = 2 * N;
   const int64_t S_rhs = (int64_t)L2 * M2 * N * 3;
-  op->S.ensure(sizeof(cplx<T>) * S_rhs * nrhs);
-  cplx<T>* S = op->S.as<cplx<T>>();
+  int variant = choose_variant<T>(op);
+  cplx<T>* S = nullptr;
+  if (ext) {
+    if (mvp_scratch_bytes<T>(op, variant, nrhs) > ext_bytes) {
+      for (int v : {2, 1, 3, 0})
+        if (variant_available<T>(op, v) && mvp_scratch_bytes<T>(op, v, nrhs) <= ext_bytes) {
+          variant = v;
+          break;
+        }
+    }
+    VS_CHECK(mvp_scratch_bytes<T>(op, variant, nrhs) <= ext_bytes, VS_ERR_ARG,
+             "matvec needs " + std::to_string(mvp_scratch_bytes<T>(op, variant, nrhs)) +
+                 " bytes of scratch, the no-allocation scratch holds " + std::to_string(ext_bytes));
+    S = ext;
+  } else {
+    op->S.ensure(mvp_scratch_bytes<T>(op, variant, nrhs));
+    S = op->S.as<cplx<T>>();
+  }
   TensorC<T> fN;
   fN.f = (sizeof(T) == 8) ? op->fc64.as<cplx<T>>() : op->fc32.as<cplx<T>>();
   VS_CHECK(fN.f != nullptr, VS_ERR_ARG, "operator holds no tensor in the requested precision");
@@ -1818,21 +1846,16 @@ static void mvp_impl(vs_op* op, const cplx<T>* x, cplx<T>* y, int transp, int nr
   const int* map3 = op->map3.as<int>();
   VS_CHECK(map3 != nullptr, VS_ERR_ARG, "operator domain not set (vs_op_set_domain)");
   const int64_t lines = (int64_t)M * N * 3;
-  const int variant = choose_variant<T>(op);
   const cplx<T>* fq = (sizeof(T) == 8) ? op->fq64.as<cplx<T>>() : op->fq32.as<cplx<T>>();
   const cplx<T>* fsl = (sizeof(T) == 8) ? op->fs64.as<cplx<T>>() : op->fs32.as<cplx<T>>();
   if (variant > 0) {
     // slab-major volume 2L x (3MN): x pass, fused y-z slab pass, inverse x pass
     const int64_t SL = lines, Sr = (int64_t)L2 * SL;
-    if (variant == 3) {
-      // input slabs, then the even-ky and odd-ky terms, nrhs slab volumes each
-      op->S.ensure(sizeof(cplx<T>) * 3 * Sr * nrhs);
-      S = op->S.as<cplx<T>>();
-    }
     IoMvFx<T> fx0{x, map3, op->nunk, op->V, L, M, transp, coef_dev<T>(op->b), S, Sr};
     if (!run_rows_p<T, false>(ctx, IoMvFxSlab<T>{fx0, SL}, L, lines, nrhs))
       run_rows<T, false, IoMvFxSlab<T>, false, true>(ctx, IoMvFxSlab<T>{fx0, SL}, L2, lines, L, L2, nrhs);
     if (variant == 3) {
+      // input slabs, then the even-ky and odd-ky terms, nrhs slab volumes each
       TensorQ<T> tq{fq, L, M, N, M / 2 + 1, N / 2 + 1};
       cplx<T>* Se = S + Sr * nrhs;
       cplx<T>* So = S + 2 * Sr * nrhs;
@@ -1892,6 +1915,19 @@ void mvp_dev(vs_op* op, const void* x, void* y, int transp, int precision, int n
     mvp_impl<double>(op, (const c128*)x, (c128*)y, transp, nrhs);
   else if (precision == VS_FP32)
     mvp_impl<float>(op, (const c64*)x, (c64*)y, transp, nrhs);
+  else
+    throw Error{VS_ERR_ARG, "precision must be VS_FP64 or VS_FP32"};
+}
+
+void mvp_dev_scratch(vs_op* op, const void* x, void* y, int transp, int precision, int nrhs, void* scratch,
+                     size_t scratch_bytes) {
+  VS_CHECK(nrhs >= 1, VS_ERR_ARG, "nrhs must be >= 1");
+  VS_CHECK(x != y, VS_ERR_ARG, "matvec input and output must differ");
+  VS_CHECK(scratch != nullptr, VS_ERR_INTERNAL, "null scratch");
+  if (precision == VS_FP64)
+    mvp_impl<double>(op, (const c128*)x, (c128*)y, transp, nrhs, (c128*)scratch, scratch_bytes);
+  else if (precision == VS_FP32)
+    mvp_impl<float>(op, (const c64*)x, (c64*)y, transp, nrhs, (c64*)scratch, scratch_bytes);
   else
     throw Error{VS_ERR_ARG, "precision must be VS_FP64 or VS_FP32"};
 }

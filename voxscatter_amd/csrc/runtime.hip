@@ -10,10 +10,17 @@ static thread_local std::string g_err;
 void set_error(const std::string& msg) { g_err = msg; }
 const char* last_error() { return g_err.c_str(); }
 
+static thread_local int g_forbid_alloc = 0;
+NoAllocScope::NoAllocScope() { ++g_forbid_alloc; }
+NoAllocScope::~NoAllocScope() { --g_forbid_alloc; }
+
 void DevBuf::alloc(size_t n) {
   bytes = 0;
   p = nullptr;
   if (n == 0) return;
+  if (g_forbid_alloc)
+    throw Error{VS_ERR_ARG, "this call may not allocate device memory, and it would need " + std::to_string(n) +
+                                " bytes (an FFT plan or buffer the operator has not built yet)"};
   hipError_t e = hipMalloc(&p, n);
   if (e != hipSuccess) {
     (void)hipGetLastError();

@@ -148,6 +148,9 @@ BatchResult solve_refine_batch(Engine<double>& E64, Engine<float>& E32, const Li
 
 void convert_dev(vs_ctx* ctx, const void* in, int pin, void* out, int pout, int64_t n);
 void mvp_dev(vs_op* op, const void* x, void* y, int transp, int precision, int nrhs);
+// the same matvec with the caller's scratch (scratch_bytes of device memory) instead of the operator's own
+void mvp_dev_scratch(vs_op* op, const void* x, void* y, int transp, int precision, int nrhs, void* scratch,
+                     size_t scratch_bytes);
 void pc_apply_dev(vs_pc* pc, const void* x, void* y, int precision, int nrhs);
 
 }  // namespace vs

@@ -108,6 +108,15 @@ int guarded(F&& f) {
 // ------------------------------------------------------------------------------------------------
 // device memory
 // ------------------------------------------------------------------------------------------------
+// While a NoAllocScope lives on a thread, DevBuf allocations on that thread throw VS_ERR_ARG instead of
+// calling hipMalloc (the guard of the entry points that must not allocate).
+struct NoAllocScope {
+  NoAllocScope();
+  ~NoAllocScope();
+  NoAllocScope(const NoAllocScope&) = delete;
+  NoAllocScope& operator=(const NoAllocScope&) = delete;
+};
+
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
