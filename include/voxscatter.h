@@ -118,6 +118,12 @@ int vs_mvp(vs_op* op, const double* x, double* y, int transp);
 /* Device buffers of nrhs consecutive vectors in `precision` (VS_FP64 or VS_FP32), on the context
  * stream, asynchronous. */
 int vs_mvp_dev(vs_op* op, const void* x, void* y, int transp, int precision, int nrhs);
+/* vs_mvp_dev with one diagonal per right-hand side: diag holds nrhs complex doubles in device memory, and
+ * vector r uses diag[r] in place of the operator's Gram*Mr (Gram*conj(Mr) in the transp branch).  This
+ * is the reference's one-operator-per-incidence DDA system (main_DDA.m:62,75: the LDR polarisability
+ * depends on the incidence and polarisation, getOPERATOR_DDA.m:15-26) for many incidences sharing one
+ * tensor. */
+int vs_mvp_dev_diag(vs_op* op, const void* x, void* y, const void* diag, int transp, int precision, int nrhs);
 /* vs_mvp_dev that allocates nothing (no counterpart in the reference: for callers that cannot allocate
  * device memory -- code under HIP graph capture, tight serving loops, hosts that forbid hipMalloc).  The
  * FFT scratch is a fixed 64 MiB device array, one per GPU, held until the process ends, instead of the
@@ -246,6 +252,14 @@ int vs_bicgstab_batch_dev(vs_op* op, vs_pc* pc, int nrhs, const void* B, void* X
 /* Batched FP32-inner / FP64-refinement solve; B, X are FP64 device vectors. */
 int vs_refine_batch_dev(vs_op* op, vs_pc* pc, int nrhs, const void* B, void* X, double tol, int max_outer,
                         double inner_tol, int inner_maxit, int* flags, double* relres, int* outer, int* inner_total);
+/* The two batched solves with one operator diagonal per right-hand side (diag: nrhs complex doubles on the
+ * host, replacing Gram*Mr for that system as in vs_mvp_dev_diag): every matvec of the solve, the inner
+ * FP32 ones and the FP64 residuals, uses each system's own diagonal; the preconditioner is shared. */
+int vs_bicgstab_batch_diag_dev(vs_op* op, vs_pc* pc, int nrhs, const void* B, void* X, const double* diag,
+                               int precision, double tol, int maxit, int* flags, double* relres, double* iters);
+int vs_refine_batch_diag_dev(vs_op* op, vs_pc* pc, int nrhs, const void* B, void* X, const double* diag, double tol,
+                             int max_outer, double inner_tol, int inner_maxit, int* flags, double* relres, int* outer,
+                             int* inner_total);
 
 /* Timing helpers (no reference counterpart): nrep back-to-back matvecs / preconditioner applies on
  * the context stream, bracketed by HIP events; *ms receives the mean time per call. */

@@ -300,6 +300,12 @@ class Operator:
         check(self.ctx.lib.vs_mvp_dev(self.h, C.c_void_p(x_ptr), C.c_void_p(y_ptr), 1 if transp else 0,
                                       int(precision), int(nrhs)))
 
+    def mvp_dev_diag(self, x_ptr: int, y_ptr: int, diag_ptr: int, transp: bool = False, precision: int = VS_FP64,
+                     nrhs: int = 1):
+        """Device matvec with one diagonal (Gram Mr) per right-hand side, diag_ptr: nrhs complex128 on the device."""
+        check(self.ctx.lib.vs_mvp_dev_diag(self.h, C.c_void_p(x_ptr), C.c_void_p(y_ptr), C.c_void_p(diag_ptr),
+                                           1 if transp else 0, int(precision), int(nrhs)))
+
 
 # --------------------------------------------------------------------------------------------------
 # preconditioner
@@ -725,26 +731,40 @@ def bench_pc(M: Preconditioner, x_ptr: int, y_ptr: int, precision=VS_FP64, nrhs=
     return ms.value
 
 
-def bicgstab_batch_dev(A: Operator, B: "DeviceArray", X: "DeviceArray", tol=1e-6, maxit=2000, M1=None):
-    """Batched BiCGStab over the rows of B (nrhs x n device array, FP64 or FP32); returns (flags, relres, iters)."""
+def _diag_arg(diag, k):
+    if diag is None:
+        return None
+    d = cplx_arr(diag)
+    if d.size != k:
+        raise VoxScatterError(capi.VS_ERR_ARG, f"diag needs one value per right-hand side ({k}), got {d.size}")
+    return d
+
+
+def bicgstab_batch_dev(A: Operator, B: "DeviceArray", X: "DeviceArray", tol=1e-6, maxit=2000, M1=None, diag=None):
+    """Batched BiCGStab over the rows of B (nrhs x n device array, FP64 or FP32); returns (flags, relres, iters).
+    ``diag``: one operator diagonal (Gram Mr, e.g. 1/alpha_LDR of each incidence) per right-hand side."""
     k = B.shape[0]
     flags = (C.c_int * k)()
     relres = np.zeros(k)
     iters = np.zeros(k)
-    check(A.ctx.lib.vs_bicgstab_batch_dev(A.h, None if M1 is None else M1.h, k, C.c_void_p(B.ptr), C.c_void_p(X.ptr),
-                                          B.precision, float(tol), int(maxit), flags, dptr(relres), dptr(iters)))
+    d = _diag_arg(diag, k)
+    check(A.ctx.lib.vs_bicgstab_batch_diag_dev(A.h, None if M1 is None else M1.h, k, C.c_void_p(B.ptr),
+                                               C.c_void_p(X.ptr), dptr(d), B.precision, float(tol), int(maxit), flags,
+                                               dptr(relres), dptr(iters)))
     return np.array(list(flags)), relres, iters
 
 
 def refine_batch_dev(A: Operator, B: "DeviceArray", X: "DeviceArray", M1=None, tol=1e-6, max_outer=10, inner_tol=1e-3,
-                     inner_maxit=2000):
-    """Batched FP32 + FP64-refinement solve; returns (flags, relres, outer, inner_iterations)."""
+                     inner_maxit=2000, diag=None):
+    """Batched FP32 + FP64-refinement solve; returns (flags, relres, outer, inner_iterations).
+    ``diag``: one operator diagonal (Gram Mr) per right-hand side, as in ``bicgstab_batch_dev``."""
     k = B.shape[0]
     flags = (C.c_int * k)()
     relres = np.zeros(k)
     outer = C.c_int()
     tot = C.c_int()
-    check(A.ctx.lib.vs_refine_batch_dev(A.h, None if M1 is None else M1.h, k, C.c_void_p(B.ptr), C.c_void_p(X.ptr),
-                                        float(tol), int(max_outer), float(inner_tol), int(inner_maxit), flags,
-                                        dptr(relres), C.byref(outer), C.byref(tot)))
+    d = _diag_arg(diag, k)
+    check(A.ctx.lib.vs_refine_batch_diag_dev(A.h, None if M1 is None else M1.h, k, C.c_void_p(B.ptr),
+                                             C.c_void_p(X.ptr), dptr(d), float(tol), int(max_outer), float(inner_tol),
+                                             int(inner_maxit), flags, dptr(relres), C.byref(outer), C.byref(tot)))
     return np.array(list(flags)), relres, outer.value, tot.value

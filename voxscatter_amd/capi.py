@@ -66,6 +66,7 @@ SIGNATURES = {
     "vs_mvp": (INT, [P, D, D, INT]),
     "vs_mvp_dev": (INT, [P, P, P, INT, INT, INT]),
     "vs_mvp_dev_noalloc": (INT, [P, P, P, INT, INT, INT]),
+    "vs_mvp_dev_diag": (INT, [P, P, P, P, INT, INT, INT]),
     "vs_circ_1_level": (INT, [P, INT, INT, INT, D, D, D]),
     "vs_circ_2_level": (INT, [P, INT, INT, INT, D, D, D]),
     "vs_invert_blocks": (INT, [P, C.c_int64, INT, D, D, D, D, D]),
@@ -90,6 +91,9 @@ SIGNATURES = {
     "vs_bicgstab_batch_dev": (INT, [P, P, INT, P, P, INT, C.c_double, INT, C.POINTER(C.c_int), D, D]),
     "vs_refine_batch_dev": (INT, [P, P, INT, P, P, C.c_double, INT, C.c_double, INT, C.POINTER(C.c_int), D,
                                   C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "vs_bicgstab_batch_diag_dev": (INT, [P, P, INT, P, P, D, INT, C.c_double, INT, C.POINTER(C.c_int), D, D]),
+    "vs_refine_batch_diag_dev": (INT, [P, P, INT, P, P, D, C.c_double, INT, C.c_double, INT, C.POINTER(C.c_int), D,
+                                       C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "vs_bench_mvp": (INT, [P, P, P, INT, INT, INT, INT, D]),
     "vs_bench_pc": (INT, [P, P, P, INT, INT, INT, D]),
     "vs_pgmres": (INT, [P, P, INT, D, D, INT, C.c_double, INT, D, C.POINTER(C.c_int), D, C.POINTER(C.c_int), D,

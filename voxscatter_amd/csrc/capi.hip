@@ -117,6 +117,7 @@ LinOp<T> make_linop(vs_op* op, vs_pc* pc) {
   if (pc) L.M = [pc, prec](const cplx<T>* x, cplx<T>* y) { pc_apply_dev(pc, x, y, prec, 1); };
   L.Ab = [op, prec](const cplx<T>* x, cplx<T>* y, int k) { mvp_dev(op, x, y, 0, prec, k); };
   if (pc) L.Mb = [pc, prec](const cplx<T>* x, cplx<T>* y, int k) { pc_apply_dev(pc, x, y, prec, k); };
+  L.Abd = [op, prec](const cplx<T>* x, cplx<T>* y, int k, const c128* d) { mvp_dev_diag(op, x, y, 0, prec, k, d); };
   return L;
 }
 
@@ -463,6 +464,14 @@ int vs_mvp_dev(vs_op* op, const void* x, void* y, int transp, int precision, int
     VS_CHECK(op && x && y, VS_ERR_ARG, "null pointer");
     DeviceGuard g(op->ctx->device);
     mvp_dev(op, x, y, transp ? 1 : 0, precision, nrhs);
+  });
+}
+
+int vs_mvp_dev_diag(vs_op* op, const void* x, void* y, const void* diag, int transp, int precision, int nrhs) {
+  return guarded([&] {
+    VS_CHECK(op && x && y && diag, VS_ERR_ARG, "null pointer");
+    DeviceGuard g(op->ctx->device);
+    mvp_dev_diag(op, x, y, transp ? 1 : 0, precision, nrhs, reinterpret_cast<const c128*>(diag));
   });
 }
 
@@ -933,20 +942,21 @@ int vs_refine_dev(vs_op* op, vs_pc* pc, const void* b, void* x, double tol, int 
   });
 }
 
-int vs_bicgstab_batch_dev(vs_op* op, vs_pc* pc, int nrhs, const void* B, void* X, int precision, double tol,
-                          int maxit, int* flags, double* relres, double* iters) {
+int vs_bicgstab_batch_diag_dev(vs_op* op, vs_pc* pc, int nrhs, const void* B, void* X, const double* diag,
+                               int precision, double tol, int maxit, int* flags, double* relres, double* iters) {
   return guarded([&] {
     VS_CHECK(op && B && X && flags && relres && iters && nrhs >= 1 && maxit >= 1, VS_ERR_ARG, "bad arguments");
     VS_CHECK(precision == VS_FP64 || precision == VS_FP32, VS_ERR_ARG, "precision must be VS_FP64 or VS_FP32");
     check_solver_args(op, pc, precision);
     DeviceGuard g(op->ctx->device);
+    const c128* d = reinterpret_cast<const c128*>(diag);
     BatchResult R;
     if (precision == VS_FP64) {
       Engine<double> E(op->ctx, op->nunk);
-      R = solve_bicgstab_batch<double>(E, make_linop<double>(op, pc), nrhs, (const c128*)B, (c128*)X, tol, maxit);
+      R = solve_bicgstab_batch<double>(E, make_linop<double>(op, pc), nrhs, (const c128*)B, (c128*)X, tol, maxit, d);
     } else {
       Engine<float> E(op->ctx, op->nunk);
-      R = solve_bicgstab_batch<float>(E, make_linop<float>(op, pc), nrhs, (const c64*)B, (c64*)X, tol, maxit);
+      R = solve_bicgstab_batch<float>(E, make_linop<float>(op, pc), nrhs, (const c64*)B, (c64*)X, tol, maxit, d);
     }
     for (int i = 0; i < nrhs; ++i) {
       flags[i] = R.flag[i];
@@ -956,8 +966,14 @@ int vs_bicgstab_batch_dev(vs_op* op, vs_pc* pc, int nrhs, const void* B, void* X
   });
 }
 
-int vs_refine_batch_dev(vs_op* op, vs_pc* pc, int nrhs, const void* B, void* X, double tol, int max_outer,
-                        double inner_tol, int inner_maxit, int* flags, double* relres, int* outer, int* inner_total) {
+int vs_bicgstab_batch_dev(vs_op* op, vs_pc* pc, int nrhs, const void* B, void* X, int precision, double tol,
+                          int maxit, int* flags, double* relres, double* iters) {
+  return vs_bicgstab_batch_diag_dev(op, pc, nrhs, B, X, nullptr, precision, tol, maxit, flags, relres, iters);
+}
+
+int vs_refine_batch_diag_dev(vs_op* op, vs_pc* pc, int nrhs, const void* B, void* X, const double* diag, double tol,
+                             int max_outer, double inner_tol, int inner_maxit, int* flags, double* relres, int* outer,
+                             int* inner_total) {
   return guarded([&] {
     VS_CHECK(op && B && X && flags && relres && outer && inner_total && nrhs >= 1, VS_ERR_ARG, "bad arguments");
     VS_CHECK((op->precisions & (VS_FP64 | VS_FP32)) == (VS_FP64 | VS_FP32), VS_ERR_ARG,
@@ -968,12 +984,18 @@ int vs_refine_batch_dev(vs_op* op, vs_pc* pc, int nrhs, const void* B, void* X, 
     Engine<float> E32(op->ctx, op->nunk);
     BatchResult R = solve_refine_batch(E64, E32, make_linop<double>(op, nullptr), make_linop<float>(op, pc), nrhs,
                                        (const c128*)B, (c128*)X, tol, max_outer, inner_tol, inner_maxit, outer,
-                                       inner_total);
+                                       inner_total, reinterpret_cast<const c128*>(diag));
     for (int i = 0; i < nrhs; ++i) {
       flags[i] = R.flag[i];
       relres[i] = R.relres[i];
     }
   });
+}
+
+int vs_refine_batch_dev(vs_op* op, vs_pc* pc, int nrhs, const void* B, void* X, double tol, int max_outer,
+                        double inner_tol, int inner_maxit, int* flags, double* relres, int* outer, int* inner_total) {
+  return vs_refine_batch_diag_dev(op, pc, nrhs, B, X, nullptr, tol, max_outer, inner_tol, inner_maxit, flags, relres,
+                                  outer, inner_total);
 }
 
 }  // extern "C"

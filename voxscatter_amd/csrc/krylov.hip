@@ -1043,7 +1043,7 @@ namespace vs {
 // breakdown.
 template <typename T>
 BatchResult solve_bicgstab_batch(Engine<T>& E, const LinOp<T>& op, int k, const cplx<T>* B, cplx<T>* X, double tol,
-                                 int maxit) {
+                                 int maxit, const c128* diag) {
   using cv = cplx<T>;
   const int64_t n = E.n;
   vs_ctx* ctx = E.ctx;
@@ -1063,10 +1063,25 @@ BatchResult solve_bicgstab_batch(Engine<T>& E, const LinOp<T>& op, int k, const 
   const cv* shv = op.Mb ? sh : s;
   Carve ws;
   const size_t ip = ws.add(sizeof(c128) * (size_t)E.grid * k * 4), in_ = ws.add(sizeof(c128) * k),
-               isc = ws.add(sizeof(BiScal) * k);
+               isc = ws.add(sizeof(BiScal) * k), idsl = ws.add(sizeof(c128) * k), idsy = ws.add(sizeof(c128) * k);
   ws.take(ctx, vs_ctx::WS_KRYLOV_AUX);
   const Span part = ws.at(ip), nrm = ws.at(in_), dsc = ws.at(isc);
   BiScal* sc = dsc.as<BiScal>();
+  // per-system diagonals: dslot follows the systems when the batch is compacted, dsys stays in system order
+  c128* dslot = ws.at(idsl).as<c128>();
+  c128* dsys = ws.at(idsy).as<c128>();
+  VS_CHECK(!diag || op.Abd, VS_ERR_INTERNAL, "operator has no per-right-hand-side diagonal form");
+  if (diag) {
+    VS_HIP(hipMemcpyAsync(dslot, diag, sizeof(c128) * k, hipMemcpyHostToDevice, ctx->stream));
+    VS_HIP(hipMemcpyAsync(dsys, diag, sizeof(c128) * k, hipMemcpyHostToDevice, ctx->stream));
+    VS_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  auto apply_A = [&](const cv* xin, cv* yout, int kk, const c128* d) {
+    if (diag)
+      op.Abd(xin, yout, kk, d);
+    else
+      op.Ab(xin, yout, kk);
+  };
   VS_HIP(hipMemsetAsync(X, 0, sizeof(cv) * n * k, ctx->stream));
   VS_HIP(hipMemsetAsync(xw, 0, sizeof(cv) * n * k, ctx->stream));
   batch_norm2<T>(E, k, B, part.as<c128>(), nrm.as<c128>());
@@ -1099,10 +1114,10 @@ BatchResult solve_bicgstab_batch(Engine<T>& E, const LinOp<T>& op, int k, const 
   auto iteration = [&] {
     bib_stage_rho<T>(E, kc, rt, r, v, p, sc, pt);
     if (op.Mb) op.Mb(p, ph, kc);
-    op.Ab(phv, v, kc);
+    apply_A(phv, v, kc, dslot);
     bib_stage_alpha_half<T>(E, kc, rt, v, phv, xw, r, xh, s, sc, pt);
     if (op.Mb) op.Mb(s, sh, kc);
-    op.Ab(shv, t, kc);
+    apply_A(shv, t, kc, dslot);
     bib_stage_omega_full<T>(E, kc, t, s, shv, xh, xw, r, sc, pt);
     VS_HIP(hipMemcpyAsync(hs, sc, sizeof(BiScal) * kc, hipMemcpyDeviceToHost, ctx->stream));
   };
@@ -1188,6 +1203,8 @@ BatchResult solve_bicgstab_batch(Engine<T>& E, const LinOp<T>& op, int k, const 
         if (j != jn) {
           for (cv* vec : {r, rt, p, v, xw}) copy_vec(vec + (int64_t)j * n, vec + (int64_t)jn * n);
           VS_HIP(hipMemcpyAsync(sc + jn, sc + j, sizeof(BiScal), hipMemcpyDeviceToDevice, ctx->stream));
+          if (diag)
+            VS_HIP(hipMemcpyAsync(dslot + jn, dslot + j, sizeof(c128), hipMemcpyDeviceToDevice, ctx->stream));
           sys[jn] = sys[j];
           tolb[jn] = tolb[j];
           active[jn] = 1;
@@ -1212,7 +1229,7 @@ BatchResult solve_bicgstab_batch(Engine<T>& E, const LinOp<T>& op, int k, const 
   if (gexec) hipGraphExecDestroy(gexec);
   if (graph) hipGraphDestroy(graph);
   // true residuals
-  op.Ab(X, r, k);
+  apply_A(X, r, k, dsys);
   batch_sub<T>(E, k, B, r);
   batch_norm2<T>(E, k, r, part.as<c128>(), nrm.as<c128>());
   std::vector<c128> nr(k);
@@ -1227,14 +1244,15 @@ BatchResult solve_bicgstab_batch(Engine<T>& E, const LinOp<T>& op, int k, const 
 }
 
 template BatchResult solve_bicgstab_batch<double>(Engine<double>&, const LinOp<double>&, int, const c128*, c128*,
-                                                  double, int);
+                                                  double, int, const c128*);
 template BatchResult solve_bicgstab_batch<float>(Engine<float>&, const LinOp<float>&, int, const c64*, c64*, double,
-                                                 int);
+                                                 int, const c128*);
 
 // batched refinement: FP64 residuals of the whole batch, FP32 batched BiCGStab for the corrections
 BatchResult solve_refine_batch(Engine<double>& E64, Engine<float>& E32, const LinOp<double>& A64,
                                const LinOp<float>& A32, int k, const c128* B, c128* X, double tol, int max_outer,
-                               double inner_tol, int inner_maxit, int* outer_out, int* inner_total) {
+                               double inner_tol, int inner_maxit, int* outer_out, int* inner_total,
+                               const c128* diag) {
   const int64_t n = E64.n;
   vs_ctx* ctx = E64.ctx;
   BatchResult R;
@@ -1248,9 +1266,15 @@ BatchResult solve_refine_batch(Engine<double>& E64, Engine<float>& E32, const Li
   const Span R64 = ws.at(i64a), R32 = ws.at(i32a), D32 = ws.at(i32b), D64 = ws.at(i64b);
   Carve wa;
   const size_t ip = wa.add(sizeof(c128) * (size_t)E64.grid * k), in_ = wa.add(sizeof(c128) * k),
-               ic = wa.add(sizeof(double) * k);
+               ic = wa.add(sizeof(double) * k), idg = wa.add(sizeof(c128) * k);
   wa.take(ctx, vs_ctx::WS_REFINE_AUX);
   const Span part = wa.at(ip), nrm = wa.at(in_), coef = wa.at(ic);
+  c128* dsys = wa.at(idg).as<c128>();
+  VS_CHECK(!diag || A64.Abd, VS_ERR_INTERNAL, "operator has no per-right-hand-side diagonal form");
+  if (diag) {
+    VS_HIP(hipMemcpyAsync(dsys, diag, sizeof(c128) * k, hipMemcpyHostToDevice, ctx->stream));
+    VS_HIP(hipStreamSynchronize(ctx->stream));
+  }
   VS_HIP(hipMemsetAsync(X, 0, sizeof(c128) * n * k, ctx->stream));
   batch_norm2<double>(E64, k, B, part.as<c128>(), nrm.as<c128>());
   std::vector<c128> h(k);
@@ -1263,7 +1287,10 @@ BatchResult solve_refine_batch(Engine<double>& E64, Engine<float>& E32, const Li
       // X = 0: R = B exactly, no matvec
       VS_HIP(hipMemcpyAsync(R64.p, B, sizeof(c128) * n * k, hipMemcpyDeviceToDevice, ctx->stream));
     } else {
-      A64.Ab(X, R64.as<c128>(), k);
+      if (diag)
+        A64.Abd(X, R64.as<c128>(), k, dsys);
+      else
+        A64.Ab(X, R64.as<c128>(), k);
       batch_sub<double>(E64, k, B, R64.as<c128>());
       batch_norm2<double>(E64, k, R64.as<c128>(), part.as<c128>(), nrm.as<c128>());
       E64.read(nrm.as<c128>(), k, h.data());
@@ -1285,7 +1312,8 @@ BatchResult solve_refine_batch(Engine<double>& E64, Engine<float>& E32, const Li
     VS_HIP(hipMemcpyAsync(coef.p, ctx->pinned_up, sizeof(double) * k, hipMemcpyHostToDevice, ctx->stream));
     batch_scale<double>(E64, k, R64.as<c128>(), R64.as<c128>(), coef.as<double>(), false);
     convert_dev(ctx, R64.p, VS_FP64, R32.p, VS_FP32, n * k);
-    BatchResult in = solve_bicgstab_batch<float>(E32, A32, k, R32.as<c64>(), D32.as<c64>(), inner_tol, inner_maxit);
+    BatchResult in =
+        solve_bicgstab_batch<float>(E32, A32, k, R32.as<c64>(), D32.as<c64>(), inner_tol, inner_maxit, diag);
     tot += in.iterations;
     convert_dev(ctx, D32.p, VS_FP32, D64.p, VS_FP64, n * k);
     VS_HIP(hipStreamSynchronize(ctx->stream));

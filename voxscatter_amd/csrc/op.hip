@@ -198,6 +198,7 @@ struct IoMvIx {
   const cplx<T>* S;
   int64_t S_rhs;
   T invP;
+  const c128* arhs;  // nullable: per-right-hand-side diagonal (Gram Mr) replacing a / at
   struct Pre {
     int m;
     cplx<T> J;
@@ -216,9 +217,9 @@ struct IoMvIx {
     const cplx<T> conv = invP * v;
     cplx<T> out;
     if (!transp)
-      out = a.at(vox) * p.J - b.at(vox) * conv;
+      out = (arhs ? ccast<T>(arhs[rhs]) : a.at(vox)) * p.J - b.at(vox) * conv;
     else
-      out = at.at(vox) * p.J - cconj(conv);
+      out = (arhs ? ccast<T>(arhs[rhs]) : at.at(vox)) * p.J - cconj(conv);
     y[rhs * nunk + p.m] = out;
   }
   __device__ __forceinline__ void store(int64_t g, int xp, int rhs, cplx<T> v) const {
@@ -1813,7 +1814,7 @@ static size_t mvp_scratch_bytes(const vs_op* op, int variant, int nrhs) {
 // the tuned one if its scratch fits, else the available variant with the smallest scratch
 template <typename T>
 static void mvp_impl(vs_op* op, const cplx<T>* x, cplx<T>* y, int transp, int nrhs, cplx<T>* ext = nullptr,
-                     size_t ext_bytes = 0) {
+                     size_t ext_bytes = 0, const c128* arhs = nullptr) {
   vs_ctx* ctx = op->ctx;
   const int L = op->L, M = op->M, N = op->N;
   const int L2 = 2 * L, M2 = 2 * M, N2 = 2 * N;
@@ -1861,7 +1862,7 @@ static void mvp_impl(vs_op* op, const cplx<T>* x, cplx<T>* y, int transp, int nr
       cplx<T>* So = S + 2 * Sr * nrhs;
       launch_yzq<T>(ctx, S, tq, L, M, N, nrhs, Sr, Se, So);
       IoMvIx<T> ix0{x, y, map3, op->nunk, op->V, L, M, transp, coef_dev<T>(op->a), coef_dev<T>(op->at),
-                    coef_dev<T>(op->b), Se, Sr, (T)(1.0 / (double)op->P)};
+                    coef_dev<T>(op->b), Se, Sr, (T)(1.0 / (double)op->P), arhs};
       if (!run_rows_p<T, true>(ctx, IoMvIxSlab2<T>{ix0, SL, So}, L, lines, nrhs))
         run_rows<T, true, IoMvIxSlab2<T>, true, false>(ctx, IoMvIxSlab2<T>{ix0, SL, So}, L2, lines, L2, L, nrhs);
       return;
@@ -1874,7 +1875,7 @@ static void mvp_impl(vs_op* op, const cplx<T>* x, cplx<T>* y, int transp, int nr
       launch_yz<T>(ctx, S, ts, L, M, N, nrhs, Sr);
     }
     IoMvIx<T> ix0{x, y, map3, op->nunk, op->V, L, M, transp, coef_dev<T>(op->a), coef_dev<T>(op->at),
-                  coef_dev<T>(op->b), S, Sr, (T)(1.0 / (double)op->P)};
+                  coef_dev<T>(op->b), S, Sr, (T)(1.0 / (double)op->P), arhs};
     if (!run_rows_p<T, true>(ctx, IoMvIxSlab<T>{ix0, SL}, L, lines, nrhs))
       run_rows<T, true, IoMvIxSlab<T>, true, false>(ctx, IoMvIxSlab<T>{ix0, SL}, L2, lines, L2, L, nrhs);
     return;
@@ -1904,7 +1905,7 @@ static void mvp_impl(vs_op* op, const cplx<T>* x, cplx<T>* y, int transp, int nr
   run_cols<T, true>(ctx, S, L2, L2, (int64_t)N * 3, (int64_t)L2 * M2, M2, M2, M, 1.0, nrhs, S_rhs);
   // 5. inverse x FFT, crop, diagonal, scatter
   IoMvIx<T> ix{x, y, map3, op->nunk, op->V, L, M, transp, coef_dev<T>(op->a), coef_dev<T>(op->at),
-               coef_dev<T>(op->b), S, S_rhs, (T)(1.0 / (double)op->P)};
+               coef_dev<T>(op->b), S, S_rhs, (T)(1.0 / (double)op->P), arhs};
   run_rows<T, true>(ctx, ix, L2, lines, L2, L, nrhs);
 }
 
@@ -1915,6 +1916,17 @@ void mvp_dev(vs_op* op, const void* x, void* y, int transp, int precision, int n
     mvp_impl<double>(op, (const c128*)x, (c128*)y, transp, nrhs);
   else if (precision == VS_FP32)
     mvp_impl<float>(op, (const c64*)x, (c64*)y, transp, nrhs);
+  else
+    throw Error{VS_ERR_ARG, "precision must be VS_FP64 or VS_FP32"};
+}
+
+void mvp_dev_diag(vs_op* op, const void* x, void* y, int transp, int precision, int nrhs, const c128* arhs) {
+  VS_CHECK(nrhs >= 1, VS_ERR_ARG, "nrhs must be >= 1");
+  VS_CHECK(x != y, VS_ERR_ARG, "matvec input and output must differ");
+  if (precision == VS_FP64)
+    mvp_impl<double>(op, (const c128*)x, (c128*)y, transp, nrhs, nullptr, 0, arhs);
+  else if (precision == VS_FP32)
+    mvp_impl<float>(op, (const c64*)x, (c64*)y, transp, nrhs, nullptr, 0, arhs);
   else
     throw Error{VS_ERR_ARG, "precision must be VS_FP64 or VS_FP32"};
 }

@@ -90,6 +90,8 @@ struct LinOp {
   // batched forms over nrhs consecutive vectors
   std::function<void(const cplx<T>*, cplx<T>*, int)> Ab;
   std::function<void(const cplx<T>*, cplx<T>*, int)> Mb;
+  // batched matvec with a per-right-hand-side diagonal (device array, one complex double per vector)
+  std::function<void(const cplx<T>*, cplx<T>*, int, const c128*)> Abd;
 };
 
 struct BatchResult {
@@ -97,9 +99,11 @@ struct BatchResult {
   std::vector<double> relres, iter;
   int iterations = 0;
 };
+// diag (nullable, host, nrhs values): the diagonal Gram Mr of each system's operator, which otherwise
+// share the operator's own (the DDA polarisability of each incidence, getOPERATOR_DDA.m:15-26)
 template <typename T>
 BatchResult solve_bicgstab_batch(Engine<T>& E, const LinOp<T>& op, int nrhs, const cplx<T>* B, cplx<T>* X, double tol,
-                                 int maxit);
+                                 int maxit, const c128* diag = nullptr);
 
 struct SolveResult {
   int flag = 1;
@@ -144,10 +148,14 @@ SolveResult solve_refine(Engine<double>& E64, Engine<float>& E32, const LinOp<do
 
 BatchResult solve_refine_batch(Engine<double>& E64, Engine<float>& E32, const LinOp<double>& A64,
                                const LinOp<float>& A32, int nrhs, const c128* B, c128* X, double tol, int max_outer,
-                               double inner_tol, int inner_maxit, int* outer_out, int* inner_total);
+                               double inner_tol, int inner_maxit, int* outer_out, int* inner_total,
+                               const c128* diag = nullptr);
 
 void convert_dev(vs_ctx* ctx, const void* in, int pin, void* out, int pout, int64_t n);
 void mvp_dev(vs_op* op, const void* x, void* y, int transp, int precision, int nrhs);
+// the same matvec with a per-right-hand-side diagonal arhs[rhs] (device, nrhs complex doubles) in place of
+// the operator's Gram Mr (and Gram conj(Mr) in the transp branch)
+void mvp_dev_diag(vs_op* op, const void* x, void* y, int transp, int precision, int nrhs, const c128* arhs);
 // the same matvec with the caller's scratch (scratch_bytes of device memory) instead of the operator's own
 void mvp_dev_scratch(vs_op* op, const void* x, void* y, int transp, int precision, int nrhs, void* scratch,
                      size_t scratch_bytes);

@@ -7,9 +7,11 @@ iterative refinement (``vs_refine_batch_dev``), so every matvec and precondition
 serves all of its right-hand sides.  Particles run one after another on the context's stream; a
 particle's set-up (operator, inverse blocks) is timed apart from its solve.
 
-The polarisability is the particle's LDR value for the scatterer's own incidence (``dInc``, ``Eo``):
-one operator serves every right-hand side of the particle, as in the reference, whose operator is built
-for one incidence.
+The reference builds one DDA operator per incidence (main_DDA.m:62,75): the LDR polarisability depends
+on the incidence direction and polarisation through S = sum_j (khat_j e_j)^2 (getOPERATOR_DDA.m:15-26).
+Here the tensor is shared and each right-hand side carries its own diagonal 1/alpha_LDR(khat, e)
+(``vs_refine_batch_diag_dev``), so every solution is that of its own incidence's operator; the shared
+preconditioner uses the mean of those diagonals.
 """
 from __future__ import annotations
 
@@ -18,10 +20,17 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .api import Context, DeviceArray, get_context, refine_batch_dev
+from .api import Context, DeviceArray, Preconditioner, alpha_LDR, get_context, refine_batch_dev
 from .capi import VS_FP32, VS_FP64
 from .dda import DDASystem
 from .geometry import Scatterer, plane_wave_batch
+
+
+def incidence_diagonals(scat: Scatterer, meta) -> np.ndarray:
+    """1/alpha_LDR of each (khat, e) of a plane-wave batch: the operator diagonal of each incidence."""
+    r = scat.r
+    dx = r[1, 0, 0, 0] - r[0, 0, 0, 0]
+    return np.array([1.0 / alpha_LDR(scat.refInd, scat.k0, dx, scat.k0 * np.asarray(kh), e) for kh, e in meta])
 
 
 @dataclass
@@ -52,14 +61,15 @@ def solve_particles(parts: list[Scatterer], ndir: int = 32, tol: float = 1e-6, i
     for i, p in enumerate(parts):
         t0 = time.perf_counter()
         sysd = DDASystem.build(p, precisions=VS_FP64 | VS_FP32, ctx=ctx)
-        pc = sysd.preconditioner(2, precisions=VS_FP32)
-        B, _ = plane_wave_batch(p, ndir=ndir)
+        B, meta = plane_wave_batch(p, ndir=ndir)
+        diag = incidence_diagonals(p, meta)
+        pc = Preconditioner.from_operator(sysd.op, 2, complex(diag.mean()), wscalar=sysd.dV, precisions=VS_FP32)
         Bd = DeviceArray.from_numpy(B, np.complex128, ctx)
         Xd = DeviceArray(B.shape, np.complex128, ctx)
         ctx.synchronize()
         t1 = time.perf_counter()
         flags, relres, outer, inner = refine_batch_dev(sysd.op, Bd, Xd, pc, tol=tol, max_outer=max_outer,
-                                                       inner_tol=inner_tol, inner_maxit=inner_maxit)
+                                                       inner_tol=inner_tol, inner_maxit=inner_maxit, diag=diag)
         ctx.synchronize()
         t2 = time.perf_counter()
         res = ParticleResult(p.name, tuple(p.r.shape[:3]), int(B.shape[1]), int(B.shape[0]), flags, relres,
