@@ -99,9 +99,10 @@ def test_multi_rhs_matvec_and_preconditioner(ctx, nrhs, prec):
 def test_particle_workload_refined_batches(ctx):
     """Orientation-averaging workload: seeded particles of different grids (24^3, 32^3), 64 plane waves
     each (32 Fibonacci directions x 2 polarisations), one batched FP32 + FP64-refinement solve per
-    particle; every right-hand side reaches FP64 relative residual 1e-6, checked again with a fresh
-    FP64 operator, and the first one equals the plain FP64 BiCGStab solution to the tolerance."""
-    from voxscatter_amd.particles import solve_particles
+    particle, each plane wave with its own incidence's diagonal 1/alpha_LDR(khat, e); every right-hand side
+    reaches FP64 relative residual 1e-6, checked again with a fresh FP64 operator of that incidence, and
+    the first one equals the plain FP64 BiCGStab solution of its own operator to the tolerance."""
+    from voxscatter_amd.particles import incidence_diagonals, solve_particles
     parts = [p for p in G.particle_batch(count=5, seed=0) if p.r.shape[0] <= 32]
     assert sorted({p.r.shape[0] for p in parts}) == [24, 32]
     res = solve_particles(parts, ndir=32, tol=1e-6, ctx=ctx, keep_solutions=True)
@@ -110,10 +111,14 @@ def test_particle_workload_refined_batches(ctx):
               f"inner={r.inner} max relres={r.relres.max():.2e} solve={r.solve_s * 1e3:.1f} ms", flush=True)
         assert r.nrhs == 64 and np.all(r.flags == 0) and np.all(r.relres <= 1e-6)
         sysd = DDASystem.build(p, ctx=ctx)
-        B, _ = G.plane_wave_batch(p, ndir=32)
+        B, meta = G.plane_wave_batch(p, ndir=32)
+        diag = incidence_diagonals(p, meta)
         for j in (0, 37, 63):
+            sysd.op.set_domain(sysd.idx3, diag[j], 1.0, sysd.dV)
             assert _rel(sysd.op.mvp(r.X[j]), B[j]) <= 1.01e-6
-        x, flag, relres, it, _ = api.bicgstab(sysd.op, B[0], 1e-6, 2000, sysd.preconditioner(2))
+        sysd.op.set_domain(sysd.idx3, diag[0], 1.0, sysd.dV)
+        pc0 = api.Preconditioner.from_operator(sysd.op, 2, diag[0], wscalar=sysd.dV)
+        x, flag, relres, it, _ = api.bicgstab(sysd.op, B[0], 1e-6, 2000, pc0)
         assert flag == 0
         # both solutions are within 1e-6 (relative residual) of the exact one
         assert _rel(r.X[0], x) <= 1e-4

@@ -83,6 +83,7 @@ struct vs_pc {
   bool mirror = false;   // 2-level blocks stored for kx <= L/2, ky <= M/2 only (the rest by sign symmetry)
   int Lb = 0, Mb = 0;    // 2-level stored block grid (L x M, or L/2+1 x M/2+1 when mirrored)
   vs::DevBuf inv64, inv32;  // apply layout (2-level: [ky][r][s][kx]; 1-level: [kx][r][s])
+  vs::DevBuf invq32;     // mirrored 2-level FP32 blocks for the matrix-core product: full, column-major, block kx + Lb ky
   vs::DevBuf map3;       // own copy of the unknown map
   int64_t nunk = 0;
   vs::DevBuf U, U2;      // work volumes (3V per rhs)

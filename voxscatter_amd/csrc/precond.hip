@@ -688,6 +688,143 @@ __global__ __launch_bounds__(256) void k_blockmv2k(const cplx<T>* __restrict__ B
   }
 }
 
+// Mirrored 2-level apply on the matrix cores (FP32).  For one stored block B (kx <= L/2, ky <= M/2) the
+// product is a GEMM Y = B U over the columns q = 4 rc + v: right-hand side rc, mirror variant v (bit 0:
+// x mirror, bit 1: y mirror) with the sign flips of the mirror symmetry applied to U on the way in and to
+// Y on the way out (as in k_blockmv2m).  Complex arithmetic as four real products on
+// v_mfma_f32_16x16x4f32: Yr += Br Ur - Bi Ui, Yi += Br Ui + Bi Ur.  A wave computes 16 rows x 16 CT
+// columns (CT tiles of 16x16, real and imaginary accumulators); the NW waves of a workgroup take
+// consecutive 16-row tiles and share the U panel, staged through LDS in chunks of KC columns of B.  The
+// blocks are full and column-major (entry (r, s) of stored block b at (b n + s) n + r), so the 16 lanes
+// of a row tile read 128 contiguous bytes.  Operand layouts of the 16x16x4 instruction: A lane l holds
+// (row l % 16, k l / 16), B lane l holds (k l / 16, column l % 16), D lane l holds column l % 16 of rows
+// 4 (l / 16) + i, i = 0..3.
+typedef float vs_f4 __attribute__((ext_vector_type(4)));
+
+template <int CT>
+__global__ __launch_bounds__(256) void k_blockmv2q(const c64* __restrict__ B, const c64* __restrict__ U,
+                                                   c64* __restrict__ U2, int L, int M, int Lb, int Mb, int n,
+                                                   int RG, int CG, int64_t rhs_stride, int nrhs) {
+  constexpr int KC = 32;
+  constexpr int QW = 16 * CT;  // columns of a workgroup
+  c64* u = reinterpret_cast<c64*>(vs_smem);  // [KC][QW]
+  const int64_t G = (int64_t)Lb * Mb * RG * CG;
+  const int64_t per_xcd = (G + 7) / 8;
+  const int64_t lid = (int64_t)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+  if (lid >= G) return;
+  // consecutive logical ids (the same XCD) are consecutive stored blocks kx at one (ky, row group, column
+  // group): a 128-byte line of the gathered volume holds 16 consecutive kx, read by neighbouring workgroups
+  const int64_t nblk = (int64_t)Lb * Mb;
+  const int64_t blk = lid % nblk;
+  const int64_t rest = lid / nblk;
+  const int cg = (int)(rest % CG);
+  const int rg = (int)(rest / CG);
+  const int kx = (int)(blk % Lb), kyb = (int)(blk / Lb);
+  const int NW = blockDim.x >> 6;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int r0 = (rg * NW + wave) * 16;
+  const int q0 = cg * QW;
+  const int NQ = 4 * nrhs;
+  const int N = n / 3;
+  const int64_t sstride = (int64_t)L * M;
+  const bool xl = kx != 0 && 2 * kx != L;
+  const bool ymir = kyb != 0 && 2 * kyb != M;
+  const c64* Bb = B + blk * (int64_t)n * n;
+  vs_f4 accr[CT], acci[CT];
+#pragma unroll
+  for (int c = 0; c < CT; ++c) {
+    accr[c] = vs_f4{0.f, 0.f, 0.f, 0.f};
+    acci[c] = vs_f4{0.f, 0.f, 0.f, 0.f};
+  }
+  const int ar = r0 + (lane & 15);  // A row of this lane
+  const int ak = lane >> 4;         // A / B k index of this lane
+  const int bq = lane & 15;         // B column of this lane within a tile
+  // the A operands of a whole chunk (KC / 4 k-steps) are loaded together, one chunk ahead of their use
+  constexpr int KS = KC / 4;
+  c64 acur[KS], anxt[KS];
+  auto load_a = [&](int s0, c64* dst) {
+#pragma unroll
+    for (int j = 0; j < KS; ++j) {
+      const int s = s0 + 4 * j + ak;
+      const bool al = ar < n && s < n;
+      const c64 a = Bb[al ? (int64_t)s * n + ar : 0];
+      dst[j] = al ? a : mk<float>(0.f, 0.f);
+    }
+  };
+  load_a(0, acur);
+  for (int s0 = 0; s0 < n; s0 += KC) {
+    // stage U(s0 .. s0 + KC - 1, q0 .. q0 + QW - 1), signs applied, zero where the variant or column is
+    // dead; the gathers of a batch of 8 elements per lane are issued before any of them is stored
+    for (int e0 = 0; e0 < KC * QW; e0 += 8 * (int)blockDim.x) {
+      c64 val[8];
+      float sg[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int e = e0 + j * (int)blockDim.x + (int)threadIdx.x;
+        const int ql = e % QW, sl = e / QW;
+        const int s = s0 + sl, q = q0 + ql;
+        const int rc = q >> 2, xm = q & 1, ym = (q >> 1) & 1;
+        const bool live = e < KC * QW && s < n && q < NQ && (!xm || xl) && (!ym || ymir);
+        const int comp = (s >= N) + (s >= 2 * N);
+        const int gx = xm ? L - kx : kx, gy = ym ? M - kyb : kyb;
+        val[j] = U[live ? rc * rhs_stride + (int64_t)s * sstride + gx + (int64_t)L * gy : 0];
+        sg[j] = live ? (((xm && comp == 0) || (ym && comp == 1)) ? -1.f : 1.f) : 0.f;
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int e = e0 + j * (int)blockDim.x + (int)threadIdx.x;
+        if (e < KC * QW) u[e] = mk<float>(sg[j] * val[j].x, sg[j] * val[j].y);
+      }
+    }
+    __syncthreads();
+    if (s0 + KC < n) load_a(s0 + KC, anxt);
+    // k-steps past the end of B multiply zero A operands and zero staged U rows
+#pragma unroll
+    for (int j = 0; j < KS; ++j) {
+      const float a_r = acur[j].x, a_i = acur[j].y;
+      c64 b[CT];
+#pragma unroll
+      for (int c = 0; c < CT; ++c) b[c] = u[(4 * j + ak) * QW + c * 16 + bq];
+      // dependent products on one accumulator are 2 CT instructions apart
+#pragma unroll
+      for (int c = 0; c < CT; ++c) {
+        accr[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(a_r, b[c].x, accr[c], 0, 0, 0);
+        acci[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(a_r, b[c].y, acci[c], 0, 0, 0);
+      }
+#pragma unroll
+      for (int c = 0; c < CT; ++c) {
+        accr[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(-a_i, b[c].y, accr[c], 0, 0, 0);
+        acci[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(a_i, b[c].x, acci[c], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < KS; ++j) acur[j] = anxt[j];
+  }
+  // D lane l: column q0 + 16 c + l % 16, rows r0 + 4 (l / 16) + i
+#pragma unroll
+  for (int c = 0; c < CT; ++c) {
+    const int q = q0 + c * 16 + bq;
+    const int rc = q >> 2, xm = q & 1, ym = (q >> 1) & 1;
+    if (q >= NQ || (xm && !xl) || (ym && !ymir)) continue;
+    const int gx = xm ? L - kx : kx, gy = ym ? M - kyb : kyb;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int r = r0 + 4 * (lane >> 4) + i;
+      if (r >= n) continue;
+      const int comp = r / N;
+      const float sg = ((xm && comp == 0) || (ym && comp == 1)) ? -1.f : 1.f;
+      U2[rc * rhs_stride + (int64_t)r * sstride + gx + (int64_t)L * gy] = mk<float>(sg * accr[c][i], sg * acci[c][i]);
+    }
+  }
+}
+
+// column-major FP64 blocks -> the same order in FP32 (the matrix-core product's layout)
+__global__ void k_blocks_to_f32(const c128* __restrict__ in, c64* __restrict__ out, int64_t tot) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < tot) out[e] = ccast<float>(in[e]);
+}
+
 // per kx: U2(:, kx) = Binv(kx) U(:, kx), one wavefront per row
 template <typename T>
 __global__ __launch_bounds__(256) void k_blockmv1(const cplx<T>* __restrict__ B, const cplx<T>* __restrict__ U,
@@ -775,6 +912,15 @@ void invert_blocks_dev(vs_ctx* ctx, c128* blocks, int64_t nblk, int64_t n) {
   VS_HIP(hipStreamSynchronize(ctx->stream));  // colk/piv are released on return
 }
 
+// the matrix-core block product for mirrored FP32 blocks (VS_BMM_MFMA=0 keeps the vector-unit kernels)
+static bool bmm_mfma_enabled() {
+  static const bool v = [] {
+    const char* e = std::getenv("VS_BMM_MFMA");
+    return !(e && e[0] == '0');
+  }();
+  return v;
+}
+
 // column-major FP64 inverse blocks (device) -> apply layout(s) of pc
 void pc_store_inverse(vs_pc* pc, const c128* inv) {
   vs_ctx* ctx = pc->ctx;
@@ -814,6 +960,18 @@ void pc_store_inverse(vs_pc* pc, const c128* inv) {
   };
   if (pc->precisions & VS_FP64) go(double(0), pc->inv64);
   if (pc->precisions & VS_FP32) go(float(0), pc->inv32);
+  if ((pc->precisions & VS_FP32) && pc->levels == 2 && pc->mirror && bmm_mfma_enabled()) {
+    pc->invq32.ensure(sizeof(c64) * (size_t)tot);
+    hipLaunchKernelGGL(k_blocks_to_f32, dim3(nblocks(tot)), dim3(256), 0, ctx->stream, inv, pc->invq32.as<c64>(), tot);
+    VS_HIP(hipGetLastError());
+  }
+}
+
+// which right-hand-side counts take the matrix-core product (VS_BMM_MFMA_MIN: the smallest count, default 2)
+static bool bmm_mfma_use(int n, int nrhs) {
+  const char* e = std::getenv("VS_BMM_MFMA_MIN");
+  const int mn = e ? std::max(1, std::atoi(e)) : 2;
+  return nrhs >= mn;
 }
 
 // LDS of one CU: the mirrored block product's budget for large blocks with several right-hand sides
@@ -850,7 +1008,29 @@ static void pc_apply_impl(vs_pc* pc, const cplx<T>* x, cplx<T>* y, int nrhs) {
       // ~80 in FP32) with several right-hand sides, a tile of 4 within the 160 KB of LDS of a CU, so that
       // one pass over the blocks still serves up to 4 of them (the blocks are read nrhs / RC times)
       const int Lb = pc->Lb, Mb = pc->Mb;
-      if (n >= 96 && nrhs >= 2) {
+      if (sizeof(T) == 4 && pc->invq32.p && bmm_mfma_use(n, nrhs)) {
+        const int NQ = 4 * nrhs;
+        const int CT = NQ <= 16 ? 1 : (NQ <= 32 ? 2 : (NQ <= 64 ? 4 : 8));
+        const int CG = (NQ + 16 * CT - 1) / (16 * CT);
+        const int NW = std::min(4, (n + 15) / 16);
+        const int RG = (n + 16 * NW - 1) / (16 * NW);
+        const int64_t G = (int64_t)Lb * Mb * RG * CG;
+        const size_t lds = sizeof(c64) * 32 * 16 * CT;
+        pc->U2.ensure(sizeof(cplx<T>) * V3 * nrhs);
+        cplx<T>* U2 = pc->U2.as<cplx<T>>();
+        dim3 grid((unsigned)(((G + 7) / 8) * 8));
+        const c64* Bq = pc->invq32.as<c64>();
+        const c64* Ui = reinterpret_cast<const c64*>(U);
+        c64* Uo = reinterpret_cast<c64*>(U2);
+        switch (CT) {
+          case 1: hipLaunchKernelGGL((k_blockmv2q<1>), grid, dim3(64 * NW), lds, ctx->stream, Bq, Ui, Uo, L, M, Lb, Mb, n, RG, CG, V3, nrhs); break;
+          case 2: hipLaunchKernelGGL((k_blockmv2q<2>), grid, dim3(64 * NW), lds, ctx->stream, Bq, Ui, Uo, L, M, Lb, Mb, n, RG, CG, V3, nrhs); break;
+          case 4: hipLaunchKernelGGL((k_blockmv2q<4>), grid, dim3(64 * NW), lds, ctx->stream, Bq, Ui, Uo, L, M, Lb, Mb, n, RG, CG, V3, nrhs); break;
+          default: hipLaunchKernelGGL((k_blockmv2q<8>), grid, dim3(64 * NW), lds, ctx->stream, Bq, Ui, Uo, L, M, Lb, Mb, n, RG, CG, V3, nrhs); break;
+        }
+        VS_HIP(hipGetLastError());
+        U = U2;
+      } else if (n >= 96 && nrhs >= 2) {
         // large blocks, several right-hand sides: the tiled product, 2 or 4 right-hand sides per pass over
         // the blocks (one right-hand side: the row-split product below is faster)
         constexpr int Wk = 128 / (int)sizeof(cplx<T>);
