@@ -265,6 +265,9 @@ int vs_refine_batch_diag_dev(vs_op* op, vs_pc* pc, int nrhs, const void* B, void
  * the context stream, bracketed by HIP events; *ms receives the mean time per call. */
 int vs_bench_mvp(vs_op* op, const void* x, void* y, int transp, int precision, int nrhs, int nrep, double* ms);
 int vs_bench_pc(vs_pc* pc, const void* x, void* y, int precision, int nrhs, int nrep, double* ms);
+/* The preconditioner's block-product stage alone (the kernel of mvp_circ_2_level.m:13-15 /
+ * chan_mvp_idx.m:10-12 without its FFT stages), nrep times on the preconditioner's own work volume. */
+int vs_bench_pc_blocks(vs_pc* pc, int precision, int nrhs, int nrep, double* ms);
 
 #ifdef __cplusplus
 }

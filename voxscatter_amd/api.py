@@ -740,6 +740,13 @@ def _diag_arg(diag, k):
     return d
 
 
+def bench_pc_blocks(M: Preconditioner, precision=VS_FP32, nrhs=1, nrep=5) -> float:
+    """Mean milliseconds of the preconditioner's block-product stage alone (HIP events, library stream)."""
+    ms = C.c_double()
+    check(M.ctx.lib.vs_bench_pc_blocks(M.h, int(precision), int(nrhs), int(nrep), C.byref(ms)))
+    return ms.value
+
+
 def bicgstab_batch_dev(A: Operator, B: "DeviceArray", X: "DeviceArray", tol=1e-6, maxit=2000, M1=None, diag=None):
     """Batched BiCGStab over the rows of B (nrhs x n device array, FP64 or FP32); returns (flags, relres, iters).
     ``diag``: one operator diagonal (Gram Mr, e.g. 1/alpha_LDR of each incidence) per right-hand side."""

@@ -96,6 +96,7 @@ SIGNATURES = {
                                        C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "vs_bench_mvp": (INT, [P, P, P, INT, INT, INT, INT, D]),
     "vs_bench_pc": (INT, [P, P, P, INT, INT, INT, D]),
+    "vs_bench_pc_blocks": (INT, [P, INT, INT, INT, D]),
     "vs_pgmres": (INT, [P, P, INT, D, D, INT, C.c_double, INT, D, C.POINTER(C.c_int), D, C.POINTER(C.c_int), D,
                         C.POINTER(C.c_int)]),
     "vs_pgmresdr": (INT, [P, P, INT, D, D, INT, C.c_double, INT, INT, D, C.POINTER(C.c_int), D,

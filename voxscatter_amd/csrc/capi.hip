@@ -1038,4 +1038,13 @@ int vs_bench_pc(vs_pc* pc, const void* x, void* y, int precision, int nrhs, int 
   });
 }
 
+int vs_bench_pc_blocks(vs_pc* pc, int precision, int nrhs, int nrep, double* ms) {
+  return guarded([&] {
+    VS_CHECK(pc && ms && nrep >= 1 && nrhs >= 1, VS_ERR_ARG, "bad arguments");
+    DeviceGuard g(pc->ctx->device);
+    pc_blocks_dev(pc, precision, nrhs);
+    *ms = time_on_stream(pc->ctx, nrep, [&] { pc_blocks_dev(pc, precision, nrhs); });
+  });
+}
+
 }  // extern "C"

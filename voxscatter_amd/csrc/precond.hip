@@ -986,8 +986,9 @@ static int bmm_rows_split() {
   return v;
 }
 
+// blocks_only: run the block-product stage alone on the work volume U (a timing helper)
 template <typename T>
-static void pc_apply_impl(vs_pc* pc, const cplx<T>* x, cplx<T>* y, int nrhs) {
+static void pc_apply_impl(vs_pc* pc, const cplx<T>* x, cplx<T>* y, int nrhs, bool blocks_only = false) {
   vs_ctx* ctx = pc->ctx;
   const int prec = sizeof(T) == 8 ? VS_FP64 : VS_FP32;
   const int L = pc->L, M = pc->M, N = pc->N;
@@ -998,9 +999,10 @@ static void pc_apply_impl(vs_pc* pc, const cplx<T>* x, cplx<T>* y, int nrhs) {
   cplx<T>* U = pc->U.as<cplx<T>>();
   const int* map3 = pc->map3.as<int>();
   const int64_t lines = (int64_t)M * N * 3;
-  launch_row_fft_gather(ctx, prec, x, map3, pc->nunk, U, L, lines, nrhs, V3);
+  if (!blocks_only) launch_row_fft_gather(ctx, prec, x, map3, pc->nunk, U, L, lines, nrhs, V3);
   if (pc->levels == 2) {
-    launch_col_fft(ctx, prec, U, L, L, (int64_t)N * 3, (int64_t)L * M, M, M, M, false, 1.0, nrhs, V3);
+    if (!blocks_only)
+      launch_col_fft(ctx, prec, U, L, L, (int64_t)N * 3, (int64_t)L * M, M, M, M, false, 1.0, nrhs, V3);
     const int n = pc->nb;
     if (pc->mirror) {
       // right-hand sides per block pass (times the four mirror points): the largest of 4, 2, 1 that keeps
@@ -1127,6 +1129,7 @@ static void pc_apply_impl(vs_pc* pc, const cplx<T>* x, cplx<T>* y, int nrhs) {
 #undef VS_BMV
     VS_HIP(hipGetLastError());
     }
+    if (blocks_only) return;
     launch_col_fft(ctx, prec, U, L, L, (int64_t)N * 3, (int64_t)L * M, M, M, M, true, 1.0, nrhs, V3);
     launch_row_ifft_scatter(ctx, prec, U, map3, pc->nunk, y, L, lines, 1.0 / ((double)L * M), nrhs, V3);
   } else {
@@ -1138,8 +1141,25 @@ static void pc_apply_impl(vs_pc* pc, const cplx<T>* x, cplx<T>* y, int nrhs) {
     hipLaunchKernelGGL((k_blockmv1<T>), dim3((unsigned)((n + rows - 1) / rows), (unsigned)L, (unsigned)nrhs), dim3(256),
                        lds, ctx->stream, B, U, U2, L, n, rows, V3);
     VS_HIP(hipGetLastError());
+    if (blocks_only) return;
     launch_row_ifft_scatter(ctx, prec, U2, map3, pc->nunk, y, L, lines, 1.0 / (double)L, nrhs, V3);
   }
+}
+
+// the block-product stage alone on the preconditioner's work volume (zeroed at the first call)
+void pc_blocks_dev(vs_pc* pc, int precision, int nrhs) {
+  const size_t es = precision == VS_FP64 ? sizeof(c128) : sizeof(c64);
+  const size_t need = es * 3 * (size_t)pc->V * nrhs;
+  if (pc->U.bytes < need) {
+    pc->U.ensure(need);
+    VS_HIP(hipMemsetAsync(pc->U.p, 0, need, pc->ctx->stream));
+  }
+  if (precision == VS_FP64)
+    pc_apply_impl<double>(pc, nullptr, nullptr, nrhs, true);
+  else if (precision == VS_FP32)
+    pc_apply_impl<float>(pc, nullptr, nullptr, nrhs, true);
+  else
+    throw Error{VS_ERR_ARG, "precision must be VS_FP64 or VS_FP32"};
 }
 
 void pc_apply_dev(vs_pc* pc, const void* x, void* y, int precision, int nrhs) {

@@ -160,5 +160,6 @@ void mvp_dev_diag(vs_op* op, const void* x, void* y, int transp, int precision, 
 void mvp_dev_scratch(vs_op* op, const void* x, void* y, int transp, int precision, int nrhs, void* scratch,
                      size_t scratch_bytes);
 void pc_apply_dev(vs_pc* pc, const void* x, void* y, int precision, int nrhs);
+void pc_blocks_dev(vs_pc* pc, int precision, int nrhs);
 
 }  // namespace vs
