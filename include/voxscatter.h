@@ -77,9 +77,9 @@ int vs_op_create_dda(vs_ctx* ctx, int L, int M, int N, const double* r, double k
 /* Galerkin (piecewise-constant JVIE) N operator on an L x M x N grid of cubes of side dx
  * (replaces getOPERATORS.m:71-76 with op = 'N', type = 'FFTofC': assembly_nop_const.m:1-174, then
  * circulant_nop_const.m and fft_operator.m).  The kernel is exp(-i ko R)/R.  singular: the five
- * singular face-pair integrals [ST, EAc, EAo, VAc, VAo] as interleaved complex doubles (the values
- * singular_st_const.m / singular_ea_const.m / singular_va_const.m produce), or NULL to have the
- * library evaluate them (vs_galerkin_singular).  Grids need L, M, N >= 2. */
+ * singular face-pair integrals [ST, EAc, EAo, VAc, VAo] as interleaved complex doubles, or NULL for the
+ * reference's own values: the direct evaluation method at Np_1D = 20 (vs_galerkin_singular_directfn,
+ * assembly_nop_const.m:16,105-107).  Grids need L, M, N >= 2. */
 int vs_op_create_galerkin(vs_ctx* ctx, int L, int M, int N, double dx, double ko, const double* singular,
                           int precisions, vs_op** out);
 /* The singular integrals int_S int_S' exp(-i ko R)/R dS dS' of squares of side dx: coincident (ST),
@@ -87,6 +87,11 @@ int vs_op_create_galerkin(vs_ctx* ctx, int L, int M, int N, double dx, double ko
  * as 5 interleaved complex doubles (replaces singular_st_const.m, singular_ea_const.m,
  * singular_va_const.m with this library's own semi-analytic method instead of DIRECTFN). */
 int vs_galerkin_singular(vs_ctx* ctx, double ko, double dx, double out[10]);
+/* The same five integrals by the direct evaluation method (DIRECTFN) with np-point Gauss-Legendre rules,
+ * each square split into two triangles (replaces singular_st_const.m, singular_ea_const.m and
+ * singular_va_const.m with method 'DIRECTFN', mexcubatures/directfn_ws_{st,ea,va}_const.cpp).  Host
+ * computation (no device needed); np = 20 gives the reference's values. */
+int vs_galerkin_singular_directfn(double ko, double dx, int np, double out[10]);
 /* The face-pair table of the 2 x 2 x 2 near box (replaces the matrices C and P of nearby_correct.m:1-143):
  * for cell c = mx + 2 my + 4 mz (0-based offsets) and observation / source faces i, j (cube_faces.m order
  * -x, +x, -y, +y, -z, +z), code[36 c + 6 i + j] is 1 ST, 2 EAc, 3 EAo, 4 VAc, 5 VAo or 0, and keep[...] is

@@ -2,7 +2,8 @@
 
 * singular face-pair integrals: the library's own evaluation against the exact static self term and
   against the reference's DIRECTFN, which converges to it as its order grows (fixtures);
-* Toeplitz tensor with the reference's DIRECTFN singular values against the oracle assembly, 1e-12;
+* Toeplitz tensor with the reference's DIRECTFN singular values against the oracle assembly, 1e-12, both
+  passed in and evaluated by the library's own DIRECTFN (its default);
 * preconditioned GMRES on an inhomogeneous cube (form 2, averaged-contrast 2-level preconditioner)
   against the oracle: same iteration count, solution within 1e-10;
 * total field (E_field_Nop_const.m) of that solution against the oracle, 1e-10, and the constitutive
@@ -64,9 +65,12 @@ def test_toeplitz_matches_oracle(ctx, shape):
     with np.errstate(divide="ignore", invalid="ignore"):
         Tref = OG.assembly_nop_const(*shape, dx, ko, sing)
     assert _rel(T, Tref) <= 1e-12
-    # the library's own singular values change only the near box, by the reference's DIRECTFN error
+    # the default: the library's own direct evaluation (DIRECTFN at Np = 20), the reference's values
     op2 = api.Operator.galerkin(shape, dx, ko, ctx=ctx)
-    assert _rel(op2.toeplitz(), Tref) <= 1e-4
+    assert _rel(op2.toeplitz(), Tref) <= 1e-12
+    # the semi-analytic singular values change only the near box, by the reference's DIRECTFN error
+    op3 = api.Operator.galerkin(shape, dx, ko, singular=api.galerkin_singular(ko, dx, ctx=ctx), ctx=ctx)
+    assert 1e-9 <= _rel(op3.toeplitz(), Tref) <= 1e-4
 
 
 def _eps_field(L, M, N, seed=0, sigma=1.5):

@@ -211,8 +211,8 @@ class Operator:
     def galerkin(cls, shape, dx: float, ko: float, singular=None, precisions: int = VS_FP64,
                  ctx: Context | None = None):
         """Galerkin N operator (getOPERATORS(r, f, 'N', 'FFTofC')) on an L x M x N grid of cubes of side dx.
-        singular: the five singular face-pair integrals [ST, EAc, EAo, VAc, VAo]; None lets the library
-        evaluate them (``galerkin_singular``)."""
+        singular: the five singular face-pair integrals [ST, EAc, EAo, VAc, VAo]; None takes the reference's
+        own values (the direct evaluation method at Np = 20, ``galerkin_singular_directfn``)."""
         ctx = ctx or get_context()
         L, M, N = (int(v) for v in shape)
         sg = None if singular is None else cplx_arr(np.asarray(singular, dtype=np.complex128))
@@ -713,6 +713,15 @@ def galerkin_singular(ko: float, dx: float, ctx: Context | None = None) -> np.nd
     ctx = ctx or get_context()
     out = np.zeros(10)
     check(ctx.lib.vs_galerkin_singular(ctx.h, float(ko), float(dx), dptr(out)))
+    return out[0::2] + 1j * out[1::2]
+
+
+def galerkin_singular_directfn(ko: float, dx: float, np_1d: int = 20) -> np.ndarray:
+    """[ST, EAc, EAo, VAc, VAo] by the direct evaluation method (DIRECTFN) with np_1d-point rules, as the
+    reference's singular_*_const.m compute them (np_1d = 20 is its default).  Host computation."""
+    lib = capi.load()
+    out = np.zeros(10)
+    check(lib.vs_galerkin_singular_directfn(float(ko), float(dx), int(np_1d), dptr(out)))
     return out[0::2] + 1j * out[1::2]
 
 

@@ -17,6 +17,7 @@ void build_circ_M(vs_ctx* ctx, const c128* cL, c128* F, int L, int M, int N);
 template <typename T>
 int matvec_variant(const vs_op* op);
 void galerkin_singular(vs_ctx* ctx, double ko, double dx, c128 out[5]);
+void galerkin_singular_directfn(double ko, double dx, int np, c128 out[5]);
 void galerkin_toeplitz(vs_ctx* ctx, int L, int M, int N, double dx, double ko, const c128 sing[5], c128* G);
 void assemble2(vs_ctx* ctx, const c128* F, int L, int M, int N, int Lb, int Mb, c128 a, const c128* w, c128 ws,
                c128* out);
@@ -320,7 +321,7 @@ int vs_op_create_galerkin(vs_ctx* ctx, int L, int M, int N, double dx, double ko
     if (singular) {
       for (int i = 0; i < 5; ++i) sing[i] = mk<double>(singular[2 * i], singular[2 * i + 1]);
     } else {
-      galerkin_singular(ctx, ko, dx, sing);
+      galerkin_singular_directfn(ko, dx, 20, sing);  // the reference's default (assembly_nop_const.m:16)
     }
     op->toep.ensure(sizeof(c128) * 6 * op->V);
     galerkin_toeplitz(ctx, L, M, N, dx, ko, sing, op->toep.as<c128>());
