@@ -100,6 +100,11 @@ int vs_galerkin_near_table(int* code, int* keep);
 
 int vs_op_create_fN(vs_ctx* ctx, int L, int M, int N, const double* fN, const double* Toep,
                     int precisions, vs_op** out);
+/* Dense operator: y = A x with an n x n column-major complex128 matrix (the matrix operand of the Krylov
+ * solvers, iterapp.m:41-58: bicgstab(A, b, ...) with a numeric A).  It has no FFT tensor and no unknown set:
+ * every solver, vs_mvp / vs_mvp_dev (transp: A^H x) and vs_pc_create_op accept it; vs_op_set_domain and the
+ * circulant preconditioners do not. */
+int vs_op_create_dense(vs_ctx* ctx, int64_t n, const double* A, int precisions, vs_op** out);
 void vs_op_destroy(vs_op* op);
 int vs_op_get_toeplitz(vs_op* op, double* Toep); /* L*M*N*6 complex128 */
 int vs_op_get_fN(vs_op* op, double* fN);         /* 2L*2M*2N*6 complex128 */
@@ -161,6 +166,11 @@ int vs_pc_create_circ(vs_op* op, int levels, const double a[2], const double* w,
  * mvp_circ_2_level.m:2 / chan_mvp_idx.m:2); same block ordering as the builders above. */
 int vs_pc_create_blocks(vs_ctx* ctx, int levels, int L, int M, int N, const double* inv_blocks,
                         const int64_t* idx, int64_t nidx, int precisions, vs_pc** out);
+/* Preconditioner whose apply is the matvec of another operator, M^-1 x = mop x (a preconditioner function
+ * handle, iterapp.m:41-69, e.g. the material preconditioner mv_AN_const(J, fN, 1./Mr, 1./Mr - 1, ...) of
+ * JVIE_Solver_const.m:232-234: an operator on the same fN with that diagonal).  mop must outlive the
+ * preconditioner; its precisions are the preconditioner's. */
+int vs_pc_create_op(vs_op* mop, vs_pc** out);
 void vs_pc_destroy(vs_pc* pc);
 /* apply (replaces mvp_circ_2_level.m:2-22 / chan_mvp_idx.m:2-16) */
 int vs_pc_apply(vs_pc* pc, const double* x, double* y);
@@ -240,6 +250,13 @@ int vs_dda_farfield(vs_ctx* ctx, int64_t nvox, const double* r, const double* P,
  * diagonal).  out = {C_ext, C_abs, C_sca = C_ext - C_abs}. */
 int vs_dda_cross_sections(vs_ctx* ctx, int64_t nvox, const double* P, const double* Einc, const double* alpha_inv,
                           int64_t ainv_len, double dV, double k, double E0sq, double out[3]);
+
+/* Plane-wave right-hand sides on the device (replaces PlaneWavePointWise.m:7-19 for a batch of incidences):
+ * r: nvox*3 voxel centres (row per voxel) of the unknowns; khat: ndir unit directions (ndir*3); pol: ndir*npol
+ * polarisation vectors (row (d, p) at 3 (npol d + p)); B: device array of ndir*npol vectors of 3*nvox
+ * complex128, vector npol d + p = [E_x; E_y; E_z] with E = pol(d, p) exp(i k khat_d . r). */
+int vs_plane_wave_batch_dev(vs_ctx* ctx, int64_t nvox, const double* r, int ndir, const double* khat, int npol,
+                            const double* pol, double k, void* B);
 
 /* Adjoint gradient assembly (no reference counterpart): g_j = -dainv_j sum_c lam_{c,j} P_{c,j}, the
  * derivative d(w^T P)/d eps_j of a holomorphic objective, given the forward solution P and the adjoint

@@ -222,6 +222,20 @@ class Operator:
         return cls(h, ctx, (L, M, N), precisions)
 
     @classmethod
+    def dense(cls, A, precisions: int = VS_FP64, ctx: Context | None = None):
+        """Matrix operand (iterapp.m:41-58): y = A x for an n x n complex matrix; ``transp`` gives A^H x."""
+        ctx = ctx or get_context()
+        A = np.asarray(A, dtype=np.complex128)
+        n = A.shape[0]
+        assert A.shape == (n, n)
+        a = _fortran_flat(A, np.complex128)
+        h = C.c_void_p()
+        check(ctx.lib.vs_op_create_dense(ctx.h, n, dptr(a), int(precisions), C.byref(h)))
+        op = cls(h, ctx, (0, 0, 0), precisions)
+        op.n = n
+        return op
+
+    @classmethod
     def from_fN(cls, fN, Toep=None, precisions: int = VS_FP64, ctx: Context | None = None):
         ctx = ctx or get_context()
         fN = np.asarray(fN)
@@ -333,6 +347,17 @@ class Preconditioner:
         check(op.ctx.lib.vs_pc_create_circ(op.h, int(levels), dptr(c2(a)), dptr(wv), dptr(ws), int(precisions),
                                            C.byref(h)))
         return cls(h, op.ctx, levels, op.n)
+
+    @classmethod
+    def of_operator(cls, mop: "Operator"):
+        """Preconditioner function handle whose apply is ``mop``'s matvec (M^-1 x = mop x), e.g. the material
+        preconditioner mv_AN_const(J, fN, 1./Mr, 1./Mr - 1, Gram, ...) of JVIE_Solver_const.m:232-234 as an
+        operator on the same fN with that diagonal.  The preconditioner keeps ``mop`` alive."""
+        h = C.c_void_p()
+        check(mop.ctx.lib.vs_pc_create_op(mop.h, C.byref(h)))
+        pc = cls(h, mop.ctx, 0, mop.n)
+        pc._mop = mop
+        return pc
 
     @classmethod
     def from_blocks(cls, levels: int, inv_blocks, L, M, N, idx, precisions: int = VS_FP64, ctx: Context | None = None):

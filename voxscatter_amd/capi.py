@@ -54,6 +54,8 @@ SIGNATURES = {
     "vs_op_create_dda": (INT, [P, INT, INT, INT, D, C.c_double, INT, INT, C.POINTER(P)]),
     "vs_op_create_fN": (INT, [P, INT, INT, INT, D, D, INT, C.POINTER(P)]),
     "vs_op_destroy": (None, [P]),
+    "vs_op_create_dense": (INT, [P, C.c_int64, D, INT, C.POINTER(P)]),
+    "vs_pc_create_op": (INT, [P, C.POINTER(P)]),
     "vs_op_get_toeplitz": (INT, [P, D]),
     "vs_op_get_fN": (INT, [P, D]),
     "vs_op_set_domain": (INT, [P, I64, C.c_int64, D, C.c_int64, D, C.c_int64, D, C.c_int64]),
@@ -89,6 +91,7 @@ SIGNATURES = {
     "vs_dda_farfield": (INT, [P, C.c_int64, D, D, C.c_double, C.c_double, C.c_int64, D, D]),
     "vs_dda_cross_sections": (INT, [P, C.c_int64, D, D, D, C.c_int64, C.c_double, C.c_double, C.c_double, D]),
     "vs_dda_grad_eps": (INT, [P, C.c_int64, D, D, D, D]),
+    "vs_plane_wave_batch_dev": (INT, [P, C.c_int64, D, INT, D, INT, D, C.c_double, P]),
     "vs_bicgstab_batch_dev": (INT, [P, P, INT, P, P, INT, C.c_double, INT, C.POINTER(C.c_int), D, D]),
     "vs_refine_batch_dev": (INT, [P, P, INT, P, P, C.c_double, INT, C.c_double, INT, C.POINTER(C.c_int), D,
                                   C.POINTER(C.c_int), C.POINTER(C.c_int)]),

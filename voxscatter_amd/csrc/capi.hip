@@ -124,7 +124,7 @@ LinOp<T> make_linop(vs_op* op, vs_pc* pc) {
 
 void check_solver_args(vs_op* op, vs_pc* pc, int prec) {
   VS_CHECK(op != nullptr, VS_ERR_ARG, "null operator");
-  VS_CHECK(op->map3.p != nullptr, VS_ERR_ARG, "operator domain not set");
+  VS_CHECK(op->dense || op->map3.p != nullptr, VS_ERR_ARG, "operator domain not set");
   if (pc) {
     VS_CHECK(pc->ctx == op->ctx, VS_ERR_ARG, "operator and preconditioner belong to different contexts");
     VS_CHECK(pc->nunk == op->nunk, VS_ERR_ARG, "preconditioner and operator unknown counts differ");
@@ -396,6 +396,7 @@ int vs_op_set_domain(vs_op* op, const int64_t* idx, int64_t nidx, const double* 
                      const double* Mr, int64_t mr_len, const double* Mc, int64_t mc_len) {
   return guarded([&] {
     VS_CHECK(op && idx && Gram && Mr && Mc, VS_ERR_ARG, "null pointer");
+    VS_CHECK(!op->dense, VS_ERR_ARG, "a dense operator has no unknown set or diagonal");
     const int64_t V = op->V;
     auto okl = [&](int64_t l) { return l == 1 || l == V; };
     VS_CHECK(okl(gram_len) && okl(mr_len) && okl(mc_len), VS_ERR_ARG, "Gram/Mr/Mc must have length 1 or L*M*N");
@@ -440,6 +441,7 @@ int64_t vs_op_nunknowns(const vs_op* op) { return op ? op->nunk : -1; }
 int vs_op_matvec_variant(const vs_op* op, int precision, int* variant) {
   return guarded([&] {
     VS_CHECK(op && variant, VS_ERR_ARG, "null pointer");
+    VS_CHECK(!op->dense, VS_ERR_ARG, "a dense operator has no FFT matvec variant");
     VS_CHECK(precision == VS_FP64 || precision == VS_FP32, VS_ERR_ARG, "precision must be VS_FP64 or VS_FP32");
     *variant = precision == VS_FP64 ? matvec_variant<double>(op) : matvec_variant<float>(op);
   });
@@ -448,8 +450,9 @@ int vs_op_matvec_variant(const vs_op* op, int precision, int* variant) {
 int vs_mvp(vs_op* op, const double* x, double* y, int transp) {
   return guarded([&] {
     VS_CHECK(op && x && y, VS_ERR_ARG, "null pointer");
-    VS_CHECK(op->fc64.p != nullptr, VS_ERR_ARG, "vs_mvp needs an FP64 tensor (precisions & VS_FP64)");
-    VS_CHECK(op->map3.p != nullptr, VS_ERR_ARG, "operator domain not set");
+    VS_CHECK(op->dense ? op->a64.p != nullptr : op->fc64.p != nullptr, VS_ERR_ARG,
+             "vs_mvp needs an FP64 operator (precisions & VS_FP64)");
+    VS_CHECK(op->dense || op->map3.p != nullptr, VS_ERR_ARG, "operator domain not set");
     DeviceGuard g(op->ctx->device);
     const size_t bytes = sizeof(c128) * (size_t)op->nunk;
     op->hx.ensure(std::max<size_t>(bytes, 16));

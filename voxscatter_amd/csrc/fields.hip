@@ -95,7 +95,43 @@ __global__ void k_grad_eps(const c128* __restrict__ P, const c128* __restrict__ 
 
 using namespace vs;
 
+// plane waves E_p exp(i k khat_d . r_v) at the occupied voxels (PlaneWavePointWise.m:7-19) for ndir
+// directions, npol polarisations each: B[(npol d + p) 3 nvox + c nvox + v] = e[d][p][c] exp(i k khat_d . r_v);
+// one thread per (voxel, direction), the phase in FP64
+__global__ void k_plane_waves(const double* __restrict__ r, int64_t nvox, const double* __restrict__ khat,
+                              const double* __restrict__ pol, int ndir, int npol, double k, c128* __restrict__ B) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int d = blockIdx.y;
+  if (v >= nvox) return;
+  const double ph = k * (khat[3 * d] * r[3 * v] + khat[3 * d + 1] * r[3 * v + 1] + khat[3 * d + 2] * r[3 * v + 2]);
+  double sn, cs;
+  sincos(ph, &sn, &cs);
+  for (int p = 0; p < npol; ++p) {
+    const double* e = pol + 3 * (npol * d + p);
+    c128* row = B + (int64_t)(npol * d + p) * 3 * nvox;
+    for (int c = 0; c < 3; ++c) row[c * nvox + v] = mk<double>(e[c] * cs, e[c] * sn);
+  }
+}
+
 extern "C" {
+
+int vs_plane_wave_batch_dev(vs_ctx* ctx, int64_t nvox, const double* r, int ndir, const double* khat, int npol,
+                            const double* pol, double k, void* B) {
+  return guarded([&] {
+    VS_CHECK(ctx && r && khat && pol && B && nvox >= 1 && ndir >= 1 && npol >= 1 && ndir <= 65535, VS_ERR_ARG,
+             "bad arguments");
+    DeviceGuard g(ctx->device);
+    DevBuf dr(sizeof(double) * 3 * nvox), dk(sizeof(double) * 3 * ndir), dp(sizeof(double) * 3 * ndir * npol);
+    VS_HIP(hipMemcpyAsync(dr.p, r, sizeof(double) * 3 * nvox, hipMemcpyHostToDevice, ctx->stream));
+    VS_HIP(hipMemcpyAsync(dk.p, khat, sizeof(double) * 3 * ndir, hipMemcpyHostToDevice, ctx->stream));
+    VS_HIP(hipMemcpyAsync(dp.p, pol, sizeof(double) * 3 * ndir * npol, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_plane_waves, dim3((unsigned)((nvox + 255) / 256), (unsigned)ndir), dim3(256), 0,
+                       ctx->stream, dr.as<double>(), nvox, dk.as<double>(), dp.as<double>(), ndir, npol, k,
+                       reinterpret_cast<c128*>(B));
+    VS_HIP(hipGetLastError());
+    VS_HIP(hipStreamSynchronize(ctx->stream));  // the staging buffers are released on return
+  });
+}
 
 int vs_dda_farfield(vs_ctx* ctx, int64_t nvox, const double* r, const double* P, double dV, double k, int64_t ndir,
                     const double* dirs, double* F) {

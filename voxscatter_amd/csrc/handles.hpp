@@ -48,6 +48,9 @@ void launch_row_ifft_scatter(vs_ctx* ctx, int prec, const void* in, const int* m
 
 struct vs_op {
   vs_ctx* ctx = nullptr;
+  // dense operator (vs_op_create_dense): y = A x with an n x n column-major matrix, no FFT tensor or domain
+  bool dense = false;
+  vs::DevBuf a64, a32;
   int L = 0, M = 0, N = 0;
   int64_t V = 0, P = 0;
   int precisions = 0;
@@ -74,6 +77,7 @@ struct vs_op {
 
 struct vs_pc {
   vs_ctx* ctx = nullptr;
+  vs_op* mop = nullptr;  // preconditioner that is an operator's matvec (vs_pc_create_op): M^-1 x = mop x
   int levels = 2;
   int L = 0, M = 0, N = 0;
   int64_t V = 0;

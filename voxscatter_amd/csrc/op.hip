@@ -1909,7 +1909,10 @@ static void mvp_impl(vs_op* op, const cplx<T>* x, cplx<T>* y, int transp, int nr
   run_rows<T, true>(ctx, ix, L2, lines, L2, L, nrhs);
 }
 
+void dense_mvp_dev(vs_op* op, const void* x, void* y, int transp, int precision, int nrhs);
+
 void mvp_dev(vs_op* op, const void* x, void* y, int transp, int precision, int nrhs) {
+  if (op->dense) return dense_mvp_dev(op, x, y, transp, precision, nrhs);
   VS_CHECK(nrhs >= 1, VS_ERR_ARG, "nrhs must be >= 1");
   VS_CHECK(x != y, VS_ERR_ARG, "matvec input and output must differ");
   if (precision == VS_FP64)
@@ -1921,6 +1924,7 @@ void mvp_dev(vs_op* op, const void* x, void* y, int transp, int precision, int n
 }
 
 void mvp_dev_diag(vs_op* op, const void* x, void* y, int transp, int precision, int nrhs, const c128* arhs) {
+  VS_CHECK(!op->dense, VS_ERR_UNSUPPORTED, "not available for a dense operator");
   VS_CHECK(nrhs >= 1, VS_ERR_ARG, "nrhs must be >= 1");
   VS_CHECK(x != y, VS_ERR_ARG, "matvec input and output must differ");
   if (precision == VS_FP64)
@@ -1933,6 +1937,7 @@ void mvp_dev_diag(vs_op* op, const void* x, void* y, int transp, int precision, 
 
 void mvp_dev_scratch(vs_op* op, const void* x, void* y, int transp, int precision, int nrhs, void* scratch,
                      size_t scratch_bytes) {
+  VS_CHECK(!op->dense, VS_ERR_UNSUPPORTED, "not available for a dense operator");
   VS_CHECK(nrhs >= 1, VS_ERR_ARG, "nrhs must be >= 1");
   VS_CHECK(x != y, VS_ERR_ARG, "matvec input and output must differ");
   VS_CHECK(scratch != nullptr, VS_ERR_INTERNAL, "null scratch");

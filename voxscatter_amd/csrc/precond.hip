@@ -1148,6 +1148,7 @@ static void pc_apply_impl(vs_pc* pc, const cplx<T>* x, cplx<T>* y, int nrhs, boo
 
 // the block-product stage alone on the preconditioner's work volume (zeroed at the first call)
 void pc_blocks_dev(vs_pc* pc, int precision, int nrhs) {
+  VS_CHECK(!pc->mop, VS_ERR_UNSUPPORTED, "an operator preconditioner has no block stage");
   const size_t es = precision == VS_FP64 ? sizeof(c128) : sizeof(c64);
   const size_t need = es * 3 * (size_t)pc->V * nrhs;
   if (pc->U.bytes < need) {
@@ -1162,8 +1163,11 @@ void pc_blocks_dev(vs_pc* pc, int precision, int nrhs) {
     throw Error{VS_ERR_ARG, "precision must be VS_FP64 or VS_FP32"};
 }
 
+void mvp_dev(vs_op* op, const void* x, void* y, int transp, int precision, int nrhs);
+
 void pc_apply_dev(vs_pc* pc, const void* x, void* y, int precision, int nrhs) {
   VS_CHECK(x != y, VS_ERR_ARG, "preconditioner input and output must differ");
+  if (pc->mop) return mvp_dev(pc->mop, x, y, 0, precision, nrhs);
   if (precision == VS_FP64)
     pc_apply_impl<double>(pc, (const c128*)x, (c128*)y, nrhs);
   else if (precision == VS_FP32)

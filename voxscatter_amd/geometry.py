@@ -338,6 +338,26 @@ def plane_wave_batch(scat: Scatterer, ndir: int = 32):
     return out, meta
 
 
+def plane_wave_batch_dev(scat: Scatterer, ndir: int = 32, ctx=None):
+    """``plane_wave_batch`` evaluated on the device: a (2 ndir, 3 nvox) complex128 DeviceArray and the
+    (khat, e) of each row (vs_plane_wave_batch_dev)."""
+    from . import api
+    from .capi import check, dptr
+    ctx = ctx or api.get_context()
+    rv = np.ascontiguousarray(np.asarray(scat.r, dtype=np.float64).reshape(-1, 3, order="F")[np.asarray(scat.idx)])
+    dirs = np.ascontiguousarray(fibonacci_sphere(ndir))
+    meta, pol = [], []
+    for khat in dirs:
+        for e in polarisations(khat):
+            meta.append((khat, e))
+            pol.append(e)
+    pol = np.ascontiguousarray(np.asarray(pol, dtype=np.float64))
+    B = api.DeviceArray((2 * ndir, 3 * rv.shape[0]), np.complex128, ctx)
+    check(ctx.lib.vs_plane_wave_batch_dev(ctx.h, rv.shape[0], dptr(rv), ndir, dptr(dirs), 2, dptr(pol),
+                                          float(scat.k0), api.C.c_void_p(B.ptr)))
+    return B, meta
+
+
 def _rotation(rng) -> np.ndarray:
     q = rng.normal(size=4)
     q /= np.linalg.norm(q)

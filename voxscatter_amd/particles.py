@@ -2,7 +2,7 @@
 
 Each particle gets its own DDA operator (main_DDA.m:61-75) and 2-level Chan preconditioner
 (main_DDA.m:98-111); its 2 * ndir right-hand sides (PlaneWavePointWise.m:7-19 for ndir Fibonacci-sphere
-directions, two polarisations each) are solved together by one batched FP32 solve inside FP64
+directions, two polarisations each, evaluated on the device) are solved together by one batched FP32 solve inside FP64
 iterative refinement (``vs_refine_batch_dev``), so every matvec and preconditioner launch of a particle
 serves all of its right-hand sides.  Particles run one after another on the context's stream; a
 particle's set-up (operator, inverse blocks) is timed apart from its solve.
@@ -23,7 +23,7 @@ import numpy as np
 from .api import Context, DeviceArray, Preconditioner, alpha_LDR, get_context, refine_batch_dev
 from .capi import VS_FP32, VS_FP64
 from .dda import DDASystem
-from .geometry import Scatterer, plane_wave_batch
+from .geometry import Scatterer, plane_wave_batch_dev
 
 
 def incidence_diagonals(scat: Scatterer, meta) -> np.ndarray:
@@ -61,18 +61,17 @@ def solve_particles(parts: list[Scatterer], ndir: int = 32, tol: float = 1e-6, i
     for i, p in enumerate(parts):
         t0 = time.perf_counter()
         sysd = DDASystem.build(p, precisions=VS_FP64 | VS_FP32, ctx=ctx)
-        B, meta = plane_wave_batch(p, ndir=ndir)
+        Bd, meta = plane_wave_batch_dev(p, ndir=ndir, ctx=ctx)
         diag = incidence_diagonals(p, meta)
         pc = Preconditioner.from_operator(sysd.op, 2, complex(diag.mean()), wscalar=sysd.dV, precisions=VS_FP32)
-        Bd = DeviceArray.from_numpy(B, np.complex128, ctx)
-        Xd = DeviceArray(B.shape, np.complex128, ctx)
+        Xd = DeviceArray(Bd.shape, np.complex128, ctx)
         ctx.synchronize()
         t1 = time.perf_counter()
         flags, relres, outer, inner = refine_batch_dev(sysd.op, Bd, Xd, pc, tol=tol, max_outer=max_outer,
                                                        inner_tol=inner_tol, inner_maxit=inner_maxit, diag=diag)
         ctx.synchronize()
         t2 = time.perf_counter()
-        res = ParticleResult(p.name, tuple(p.r.shape[:3]), int(B.shape[1]), int(B.shape[0]), flags, relres,
+        res = ParticleResult(p.name, tuple(p.r.shape[:3]), int(Bd.shape[1]), int(Bd.shape[0]), flags, relres,
                              outer, inner, t1 - t0, t2 - t1, Xd.numpy() if keep_solutions else None)
         Bd.close()
         Xd.close()
