@@ -157,7 +157,8 @@ def test_fp32_matrix_core_apply_matches_oracle(ctx, shape, monkeypatch):
     """The FP32 2-level apply with the block product on the matrix cores (16x16x4 FP32 MFMA, complex as
     four real products) against the oracle's FP64 apply with every block inverted, for 1, 3, 5, 17 and 33
     right-hand sides (one to nine 16-column tiles, one and two column groups) on even and odd grids and
-    block sizes 3N = 12 ... 108 that are not multiples of 16; and against the vector-unit FP32 product."""
+    block sizes 3N = 12 ... 108 that are not multiples of 16; and against the vector-unit FP32 product.
+    Both variants of the product: one block per workgroup, and tiles of 8 consecutive blocks."""
     from voxscatter_amd.capi import VS_FP32, VS_FP64
     print(f"[mfma apply] shape={shape}", flush=True)
     L, M, N = shape
@@ -174,18 +175,22 @@ def test_fp32_matrix_core_apply_matches_oracle(ctx, shape, monkeypatch):
     inv = Ch.invert_blocks_dda(blk, alpha, d ** 3)
     op = api.Operator.dda(r, k, ctx=ctx)
     op.set_domain(idx3, 1 / alpha, 1.0, d ** 3)
+    # the matrix-core copy of the blocks is built for small blocks only when that product is asked for
+    monkeypatch.setenv("VS_BMM_MFMA_MIN", "1")
     pc = api.Preconditioner.from_operator(op, 2, 1 / alpha, wscalar=d ** 3, precisions=VS_FP32 | VS_FP64)
-    for nrhs in (1, 3, 5, 17, 33):
+    for nrhs, tile in ((1, "1"), (3, "1"), (5, "0"), (5, "1"), (17, "1"), (33, "0"), (33, "1")):
         X = rng.normal(size=(nrhs, idx3.size)) + 1j * rng.normal(size=(nrhs, idx3.size))
         Yref = np.stack([Ch.mvp_circ_2_level(inv, x, L, M, N, idx3) for x in X])
         Xd = api.DeviceArray.from_numpy(X, np.complex64, ctx)
         Yd = api.DeviceArray(X.shape, np.complex64, ctx)
         monkeypatch.setenv("VS_BMM_MFMA_MIN", "1")
+        monkeypatch.setenv("VS_BMM_TILE", tile)
         pc.apply_dev(Xd.ptr, Yd.ptr, VS_FP32, nrhs)
         ym = Yd.numpy().astype(np.complex128)
         monkeypatch.setenv("VS_BMM_MFMA_MIN", "100000")
         pc.apply_dev(Xd.ptr, Yd.ptr, VS_FP32, nrhs)
         yv = Yd.numpy().astype(np.complex128)
         em, ev = _rel(ym, Yref), _rel(ym, yv)
-        print(f"[mfma apply] nrhs={nrhs}: vs oracle {em:.2e}, vs vector-unit FP32 {ev:.2e}", flush=True)
+        print(f"[mfma apply] nrhs={nrhs} tile={tile}: vs oracle {em:.2e}, vs vector-unit FP32 {ev:.2e}",
+              flush=True)
         assert em <= 2e-5 and ev <= 2e-5

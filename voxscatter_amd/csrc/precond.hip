@@ -700,12 +700,15 @@ __global__ __launch_bounds__(256) void k_blockmv2k(const cplx<T>* __restrict__ B
 // (row l % 16, k l / 16), B lane l holds (k l / 16, column l % 16), D lane l holds column l % 16 of rows
 // 4 (l / 16) + i, i = 0..3.
 typedef float vs_f4 __attribute__((ext_vector_type(4)));
+#ifndef VS_BMQ_KC
+#define VS_BMQ_KC 16  // block columns per staged chunk of U
+#endif
 
 template <int CT>
 __global__ __launch_bounds__(256) void k_blockmv2q(const c64* __restrict__ B, const c64* __restrict__ U,
                                                    c64* __restrict__ U2, int L, int M, int Lb, int Mb, int n,
                                                    int RG, int CG, int64_t rhs_stride, int nrhs) {
-  constexpr int KC = 32;
+  constexpr int KC = VS_BMQ_KC;
   constexpr int QW = 16 * CT;  // columns of a workgroup
   c64* u = reinterpret_cast<c64*>(vs_smem);  // [KC][QW]
   const int64_t G = (int64_t)Lb * Mb * RG * CG;
@@ -816,6 +819,172 @@ __global__ __launch_bounds__(256) void k_blockmv2q(const c64* __restrict__ B, co
       const float sg = ((xm && comp == 0) || (ym && comp == 1)) ? -1.f : 1.f;
       U2[rc * rhs_stride + (int64_t)r * sstride + gx + (int64_t)L * gy] = mk<float>(sg * accr[c][i], sg * acci[c][i]);
     }
+  }
+}
+
+// The same product with a workgroup per tile of WB = 8 consecutive stored blocks (one wave per block, the
+// blocks numbered kx fastest, so a tile may wrap to the next ky), so that the staged U values of a (row s,
+// column q) for the tile's blocks are consecutive kx in memory (the gathers read 64 contiguous bytes instead
+// of 8).  Each wave computes 16 RT rows x 16 CT columns of its block; the loads of the next chunk of KC
+// block columns are issued before the products of the current one (two LDS buffers) and their results are
+// touched only after those products.
+template <int RT, int CT, int KC>
+__global__ __launch_bounds__(512) void k_blockmv2t(const c64* __restrict__ B, const c64* __restrict__ U,
+                                                   c64* __restrict__ U2, int L, int M, int Lb, int Mb, int n,
+                                                   int RG, int CG, int64_t rhs_stride, int nrhs) {
+  constexpr int WB = 8, KS = KC / 4;
+  constexpr int QW = 16 * CT;
+  constexpr int PER = WB * KC * QW / 512;  // staged values per lane per chunk
+  c64* lds = reinterpret_cast<c64*>(vs_smem);  // [2][WB][KC][QW + 1]
+  constexpr int QP = QW + 1;                    // padded column stride: rows of a block on other banks
+  const int nblk = Lb * Mb;  // stored blocks, kx fastest; a tile may wrap to the next ky
+  const int tiles = (nblk + WB - 1) / WB;
+  const int64_t G = (int64_t)tiles * RG * CG;
+  const int64_t per_xcd = (G + 7) / 8;
+  const int64_t lid = (int64_t)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+  if (lid >= G) return;
+  // consecutive logical ids: consecutive tiles at one (row group, column group) (the other order, row and
+  // column groups of one tile together, measured 4% slower on 96^3 blocks with 64 right-hand sides)
+  const int64_t tk = lid % tiles, rest = lid / tiles;
+  const int cg = (int)(rest % CG), rg = (int)(rest / CG);
+  const int b0 = (int)tk * WB;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int bw = b0 + wave;
+  const int kx = bw % Lb, kyb = bw / Lb;
+  const int r0 = rg * 16 * RT, q0 = cg * QW;
+  const int NQ = 4 * nrhs;
+  const int N = n / 3;
+  const int64_t sstride = (int64_t)L * M;
+  // staging: element e = j * 512 + thread = ((sl * QW + ql) * WB + w): consecutive lanes take consecutive
+  // blocks; a lane keeps one (block w, column ql) for every chunk and element (WB * QW divides 512), so that
+  // its base offset, liveness and mirror flags are computed once
+  static_assert(512 % (WB * QW) == 0 && KC % (512 / (WB * QW)) == 0, "staging layout");
+  constexpr int SPJ = 512 / (WB * QW);  // staged rows per element index j
+  const int w_l = (int)threadIdx.x % WB, ql_l = ((int)threadIdx.x / WB) % QW, sl_l = (int)threadIdx.x / (WB * QW);
+  int64_t base_l = 0;
+  bool live_l, xm_l, ym_l;
+  {
+    const int q = q0 + ql_l, b = b0 + w_l;
+    const int kxw = b % Lb, kyw = b / Lb;
+    const int rc = q >> 2;
+    xm_l = q & 1;
+    ym_l = (q >> 1) & 1;
+    const bool xl = kxw != 0 && 2 * kxw != L;
+    const bool yl = kyw != 0 && 2 * kyw != M;
+    live_l = b < nblk && q < NQ && (!xm_l || xl) && (!ym_l || yl);
+    const int gx = xm_l ? L - kxw : kxw, gy = ym_l ? M - kyw : kyw;
+    if (live_l) base_l = rc * rhs_stride + gx + (int64_t)L * gy;
+  }
+  // the loaded values stay untouched until they are stored (sign and liveness applied there), so that the
+  // loads of the next chunk are still in flight during the products of the current one
+  auto stage_load = [&](int s0, c64* val) {
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const int s = s0 + j * SPJ + sl_l;
+      val[j] = U[(live_l && s < n) ? base_l + (int64_t)s * sstride : 0];
+    }
+  };
+  auto stage_store = [&](int buf, int s0, const c64* val) {
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const int sl = j * SPJ + sl_l, s = s0 + sl;
+      const bool neg = (xm_l && s < N) || (ym_l && s >= N && s < 2 * N);
+      const float sg = (live_l && s < n) ? (neg ? -1.f : 1.f) : 0.f;
+      lds[((buf * WB + w_l) * KC + sl) * QP + ql_l] = mk<float>(sg * val[j].x, sg * val[j].y);
+    }
+  };
+  const bool bl = bw < nblk;
+  const c64* Bb = B + (int64_t)(bl ? bw : 0) * n * n;
+  const int ak = lane >> 4, am = lane & 15;
+  // raw loads of the block entries; dead entries are zeroed when the chunk is taken up
+  auto load_a = [&](int s0, c64 (*dst)[KS]) {
+#pragma unroll
+    for (int t = 0; t < RT; ++t)
+#pragma unroll
+      for (int j = 0; j < KS; ++j) {
+        const int r = r0 + 16 * t + am, s = s0 + 4 * j + ak;
+        dst[t][j] = Bb[(bl && r < n && s < n) ? (int64_t)s * n + r : 0];
+      }
+  };
+  auto take_a = [&](int s0, c64 (*dst)[KS], const c64 (*src)[KS]) {
+#pragma unroll
+    for (int t = 0; t < RT; ++t)
+#pragma unroll
+      for (int j = 0; j < KS; ++j) {
+        const int r = r0 + 16 * t + am, s = s0 + 4 * j + ak;
+        dst[t][j] = (bl && r < n && s < n) ? src[t][j] : mk<float>(0.f, 0.f);
+      }
+  };
+  vs_f4 accr[RT][CT], acci[RT][CT];
+#pragma unroll
+  for (int t = 0; t < RT; ++t)
+#pragma unroll
+    for (int c = 0; c < CT; ++c) {
+      accr[t][c] = vs_f4{0.f, 0.f, 0.f, 0.f};
+      acci[t][c] = vs_f4{0.f, 0.f, 0.f, 0.f};
+    }
+  c64 sv[PER];
+  c64 acur[RT][KS], anxt[RT][KS];
+  stage_load(0, sv);
+  load_a(0, anxt);
+  take_a(0, acur, anxt);
+  stage_store(0, 0, sv);
+  __syncthreads();
+  int buf = 0;
+  for (int s0 = 0; s0 < n; s0 += KC) {
+    const bool more = s0 + KC < n;
+    if (more) {
+      stage_load(s0 + KC, sv);
+      load_a(s0 + KC, anxt);
+    }
+    const c64* ub = lds + (size_t)(buf * WB + wave) * KC * QP;
+#pragma unroll
+    for (int j = 0; j < KS; ++j) {
+      c64 b[CT];
+#pragma unroll
+      for (int c = 0; c < CT; ++c) b[c] = ub[(4 * j + ak) * QP + c * 16 + am];
+#pragma unroll
+      for (int t = 0; t < RT; ++t)
+#pragma unroll
+        for (int c = 0; c < CT; ++c) {
+          accr[t][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(acur[t][j].x, b[c].x, accr[t][c], 0, 0, 0);
+          acci[t][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(acur[t][j].x, b[c].y, acci[t][c], 0, 0, 0);
+        }
+#pragma unroll
+      for (int t = 0; t < RT; ++t)
+#pragma unroll
+        for (int c = 0; c < CT; ++c) {
+          accr[t][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(-acur[t][j].y, b[c].y, accr[t][c], 0, 0, 0);
+          acci[t][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(acur[t][j].y, b[c].x, acci[t][c], 0, 0, 0);
+        }
+    }
+    if (more) {
+      stage_store(buf ^ 1, s0 + KC, sv);
+      take_a(s0 + KC, acur, anxt);
+    }
+    __syncthreads();
+    buf ^= 1;
+  }
+  if (!bl) return;
+  const bool xl = kx != 0 && 2 * kx != L;
+  const bool ymir = kyb != 0 && 2 * kyb != M;
+#pragma unroll
+  for (int c = 0; c < CT; ++c) {
+    const int q = q0 + c * 16 + am;
+    const int rc = q >> 2, xm = q & 1, ym = (q >> 1) & 1;
+    if (q >= NQ || (xm && !xl) || (ym && !ymir)) continue;
+    const int gx = xm ? L - kx : kx, gy = ym ? M - kyb : kyb;
+#pragma unroll
+    for (int t = 0; t < RT; ++t)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int r = r0 + 16 * t + 4 * ak + i;
+        if (r >= n) continue;
+        const int comp = r / N;
+        const float sg = ((xm && comp == 0) || (ym && comp == 1)) ? -1.f : 1.f;
+        U2[rc * rhs_stride + (int64_t)r * sstride + gx + (int64_t)L * gy] =
+            mk<float>(sg * accr[t][c][i], sg * acci[t][c][i]);
+      }
   }
 }
 
@@ -960,18 +1129,26 @@ void pc_store_inverse(vs_pc* pc, const c128* inv) {
   };
   if (pc->precisions & VS_FP64) go(double(0), pc->inv64);
   if (pc->precisions & VS_FP32) go(float(0), pc->inv32);
-  if ((pc->precisions & VS_FP32) && pc->levels == 2 && pc->mirror && bmm_mfma_enabled()) {
+  if ((pc->precisions & VS_FP32) && pc->levels == 2 && pc->mirror && bmm_mfma_enabled() &&
+      (n >= 96 || std::getenv("VS_BMM_MFMA_MIN"))) {
     pc->invq32.ensure(sizeof(c64) * (size_t)tot);
     hipLaunchKernelGGL(k_blocks_to_f32, dim3(nblocks(tot)), dim3(256), 0, ctx->stream, inv, pc->invq32.as<c64>(), tot);
     VS_HIP(hipGetLastError());
   }
 }
 
-// which right-hand-side counts take the matrix-core product (VS_BMM_MFMA_MIN: the smallest count, default 2)
+// which right-hand-side counts take the matrix-core product (VS_BMM_MFMA_MIN: the smallest count)
+// (default: several right-hand sides on large blocks, 3N >= 96, where it beats the vector-unit kernels)
 static bool bmm_mfma_use(int n, int nrhs) {
   const char* e = std::getenv("VS_BMM_MFMA_MIN");
-  const int mn = e ? std::max(1, std::atoi(e)) : 2;
-  return nrhs >= mn;
+  if (e) return nrhs >= std::max(1, std::atoi(e));
+  return nrhs >= 2 && n >= 96;
+}
+
+// the tile-of-blocks variant of the matrix-core product (VS_BMM_TILE=0: one block per workgroup)
+static bool bmm_tile_use(int n, int nrhs) {
+  const char* e = std::getenv("VS_BMM_TILE");
+  return !(e && e[0] == '0');
 }
 
 // LDS of one CU: the mirrored block product's budget for large blocks with several right-hand sides
@@ -1017,14 +1194,27 @@ static void pc_apply_impl(vs_pc* pc, const cplx<T>* x, cplx<T>* y, int nrhs, boo
         const int NW = std::min(4, (n + 15) / 16);
         const int RG = (n + 16 * NW - 1) / (16 * NW);
         const int64_t G = (int64_t)Lb * Mb * RG * CG;
-        const size_t lds = sizeof(c64) * 32 * 16 * CT;
+        const size_t lds = sizeof(c64) * VS_BMQ_KC * 16 * CT;
         pc->U2.ensure(sizeof(cplx<T>) * V3 * nrhs);
         cplx<T>* U2 = pc->U2.as<cplx<T>>();
         dim3 grid((unsigned)(((G + 7) / 8) * 8));
         const c64* Bq = pc->invq32.as<c64>();
         const c64* Ui = reinterpret_cast<const c64*>(U);
         c64* Uo = reinterpret_cast<c64*>(U2);
-        switch (CT) {
+        if (bmm_tile_use(n, nrhs)) {
+          // tiles of 8 blocks: 48-row groups, 64-column groups (fewer columns: 32 / 16); chunks of 8 block
+          // columns (16 measured 4% slower: 256 VGPRs)
+          const int CTt = NQ >= 64 ? 4 : (NQ >= 32 ? 2 : 1);
+          const int RGt = (n + 47) / 48, CGt = (NQ + 16 * CTt - 1) / (16 * CTt);
+          const int64_t Gt = (int64_t)((Lb * Mb + 7) / 8) * RGt * CGt;
+          const size_t ldst = sizeof(c64) * 2 * 8 * 8 * (16 * CTt + 1);
+          dim3 gridt((unsigned)(((Gt + 7) / 8) * 8));
+          switch (CTt) {
+            case 4: hipLaunchKernelGGL((k_blockmv2t<3, 4, 8>), gridt, dim3(512), ldst, ctx->stream, Bq, Ui, Uo, L, M, Lb, Mb, n, RGt, CGt, V3, nrhs); break;
+            case 2: hipLaunchKernelGGL((k_blockmv2t<3, 2, 8>), gridt, dim3(512), ldst, ctx->stream, Bq, Ui, Uo, L, M, Lb, Mb, n, RGt, CGt, V3, nrhs); break;
+            default: hipLaunchKernelGGL((k_blockmv2t<3, 1, 8>), gridt, dim3(512), ldst, ctx->stream, Bq, Ui, Uo, L, M, Lb, Mb, n, RGt, CGt, V3, nrhs); break;
+          }
+        } else switch (CT) {
           case 1: hipLaunchKernelGGL((k_blockmv2q<1>), grid, dim3(64 * NW), lds, ctx->stream, Bq, Ui, Uo, L, M, Lb, Mb, n, RG, CG, V3, nrhs); break;
           case 2: hipLaunchKernelGGL((k_blockmv2q<2>), grid, dim3(64 * NW), lds, ctx->stream, Bq, Ui, Uo, L, M, Lb, Mb, n, RG, CG, V3, nrhs); break;
           case 4: hipLaunchKernelGGL((k_blockmv2q<4>), grid, dim3(64 * NW), lds, ctx->stream, Bq, Ui, Uo, L, M, Lb, Mb, n, RG, CG, V3, nrhs); break;
