@@ -32,8 +32,10 @@ def test_circ_levels_match_oracle(ctx, shape):
     assert _rel(b2, rb2) <= 1e-13
 
 
-@pytest.mark.parametrize("n,nblk", [(24, 50), (48, 20), (72, 3), (150, 2), (300, 1)])
+@pytest.mark.parametrize("n,nblk", [(24, 50), (48, 20), (72, 3), (96, 2), (150, 2), (288, 3), (300, 1)])
 def test_block_inverse_matches_numpy(ctx, n, nblk):
+    """Whole blocks in LDS up to 3N = 96, the blocked inversion (16-column panels, FP64 matrix-core
+    updates) above, with a partial last panel at 150 and 300."""
     print(f"[inverse] n={n} nblk={nblk}", flush=True)
     rng = np.random.default_rng(n)
     B = (rng.normal(size=(nblk, n, n)) + 1j * rng.normal(size=(nblk, n, n))) / np.sqrt(n)
@@ -41,6 +43,27 @@ def test_block_inverse_matches_numpy(ctx, n, nblk):
     inv = api.invert_blocks(B, a, w)
     ref = np.linalg.inv(a * np.eye(n) - w * B)
     assert _rel(inv, ref) <= 1e-11
+
+
+@pytest.mark.parametrize("n", [150, 288])
+def test_blocked_inverse_pivoting(ctx, n, monkeypatch):
+    """Blocks whose pivots lie far from the diagonal (a scaled random permutation plus a small random part,
+    so every step swaps rows, also across panels): the blocked inversion against numpy and against the
+    unblocked sweep, which picks the same pivots."""
+    rng = np.random.default_rng(n + 1)
+    nblk = 3
+    B = np.zeros((nblk, n, n), complex)
+    for b in range(nblk):
+        perm = rng.permutation(n)
+        B[b, np.arange(n), perm] = rng.uniform(1, 2, n) * np.exp(2j * np.pi * rng.uniform(size=n))
+    B += 1e-3 * (rng.normal(size=B.shape) + 1j * rng.normal(size=B.shape))
+    ref = np.linalg.inv(B)
+    inv = api.invert_blocks(B, 0.0, -1.0)
+    monkeypatch.setenv("VS_GJ_BLOCKED", "0")
+    inv0 = api.invert_blocks(B, 0.0, -1.0)
+    e_ref, e_unb = _rel(inv, ref), _rel(inv, inv0)
+    print(f"[inverse pivoting] n={n}: vs numpy {e_ref:.2e}, vs unblocked {e_unb:.2e}", flush=True)
+    assert e_ref <= 1e-12 and e_unb <= 1e-13
 
 
 def test_block_inverse_vector_weights(ctx):

@@ -296,6 +296,168 @@ __global__ void k_gj_unswap(c128* __restrict__ blocks, int64_t n, const int* __r
   }
 }
 
+// Blocked Gauss-Jordan for blocks too large for LDS, in panels of GJB = 16 columns.  Partial pivoting picks
+// the same rows as the unblocked sweep.  The row swaps of a panel commute with its steps (a swap exchanges
+// two rows that the steps treat alike), so the panel's steps equal one block step over its indices K on
+// the row-permuted block B:
+//   B_KK <- B_KK^-1,  B_KR <- B_KK^-1 B_KR,  B_RK <- -B_RK B_KK^-1,  B_RR <- B_RR - B_RK B_KK^-1 B_KR.
+// k_gjb_panel performs the steps on the panel columns in LDS, which become P = [B_KK^-1; -B_RK B_KK^-1],
+// and records the row permutation; k_gjb_update then sets every other column j to
+//   P X_j + (row-permuted column j with rows K cleared),   X_j = rows K of the row-permuted column j,
+// a complex (n x 16) x (16 x 16) product per strip of 16 columns on the FP64 matrix cores.  The columns
+// are unswapped at the end as in the unblocked sweep.
+constexpr int GJB = 16;
+constexpr int GJB_NMAX = 512;  // 8 waves x 4 row tiles of 16 in the update kernel
+
+// one workgroup per block: the kb <= 16 steps of the panel starting at column k0
+__global__ __launch_bounds__(256) void k_gjb_panel(c128* __restrict__ blocks, int n, int k0,
+                                                   int* __restrict__ perm, int* __restrict__ piv) {
+  c128* P = reinterpret_cast<c128*>(vs_smem);  // [kb][n], column-major panel
+  c128* colk = P + (size_t)GJB * n;             // [n]
+  int* pm = reinterpret_cast<int*>(colk + n);   // [n]: source row of each row after this panel's swaps
+  __shared__ double s_best[4];
+  __shared__ int s_bi[4];
+  __shared__ int s_p;
+  __shared__ c128 s_pinv;
+  const int64_t blk = blockIdx.x;
+  c128* A = blocks + blk * (int64_t)n * n;
+  const int tid = threadIdx.x;
+  const int kb = min(GJB, n - k0);
+  for (int e = tid; e < kb * n; e += 256) P[e] = A[(int64_t)n * k0 + e];
+  for (int i = tid; i < n; i += 256) pm[i] = i;
+  __syncthreads();
+  for (int kk = 0; kk < kb; ++kk) {
+    const int k = k0 + kk;
+    const c128* Pk = P + (size_t)kk * n;
+    double best = -1.0;
+    int bi = k;
+    for (int i = k + tid; i < n; i += 256) {
+      const double v = abs2(Pk[i]);
+      if (v > best) { best = v; bi = i; }
+    }
+    wave_argmax(best, bi);
+    if ((tid & 63) == 0) {
+      s_best[tid >> 6] = best;
+      s_bi[tid >> 6] = bi;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double b0 = s_best[0];
+      int i0 = s_bi[0];
+      for (int q = 1; q < 4; ++q)
+        if (s_best[q] > b0 || (s_best[q] == b0 && s_bi[q] < i0)) { b0 = s_best[q]; i0 = s_bi[q]; }
+      s_p = i0;
+      piv[blk * n + k] = i0;
+      const int t = pm[k];
+      pm[k] = pm[i0];
+      pm[i0] = t;
+    }
+    __syncthreads();
+    const int p = s_p;
+    if (p != k && tid < kb) {
+      c128* c = P + (size_t)tid * n;
+      const c128 t = c[k];
+      c[k] = c[p];
+      c[p] = t;
+    }
+    __syncthreads();
+    for (int i = tid; i < n; i += 256) colk[i] = Pk[i];
+    if (tid == 0) s_pinv = cdiv(mk<double>(1, 0), Pk[k]);
+    __syncthreads();
+    const c128 pinv = s_pinv;
+    if (tid < kb) P[(size_t)tid * n + k] = (tid == kk) ? pinv : P[(size_t)tid * n + k] * pinv;
+    __syncthreads();
+    for (int e = tid; e < kb * n; e += 256) {
+      const int i = e % n, jj = e / n;
+      if (i == k) continue;
+      const c128 f = colk[i];
+      P[e] = (jj == kk) ? -(f * pinv) : P[e] - f * P[(size_t)jj * n + k];
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < kb * n; e += 256) A[(int64_t)n * k0 + e] = P[e];
+  for (int i = tid; i < n; i += 256) perm[blk * n + i] = pm[i];
+}
+
+typedef double vs_d4 __attribute__((ext_vector_type(4)));
+
+// one workgroup per (strip of 16 columns, block), 8 waves; the strip is staged row-permuted in LDS (all of
+// its reads precede its writes, so the update is in place), each wave takes row tiles wave, wave + 8, ...
+__global__ __launch_bounds__(512) void k_gjb_update(c128* __restrict__ blocks, int n, int k0,
+                                                    const int* __restrict__ perm) {
+  constexpr int RT = GJB_NMAX / (16 * 8);
+  const int strip = blockIdx.x;
+  if (strip == k0 / GJB) return;  // the panel itself
+  c128* S = reinterpret_cast<c128*>(vs_smem);  // [16][n + 1] row-permuted strip, column-major
+  const int ns = n + 1;
+  const int64_t blk = blockIdx.y;
+  c128* A = blocks + blk * (int64_t)n * n;
+  const int* pm = perm + blk * n;
+  const int kb = min(GJB, n - k0);
+  const int j0 = strip * GJB;
+  const int w = min(GJB, n - j0);
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int lc = lane & 15, lq = lane >> 4;
+  for (int e = tid; e < w * n; e += 512) {
+    const int i = e % n, jj = e / n;
+    S[jj * ns + i] = A[pm[i] + (int64_t)n * (j0 + jj)];
+  }
+  __syncthreads();
+  // accumulators start from the permuted strip with rows K cleared; the FP64 16x16x4 result layout holds
+  // element (row i, column j) in lane j + 16 (i % 4), register i / 4 (unlike the FP32 16x16x4 layout,
+  // lane j + 16 (i / 4), register i % 4)
+  vs_d4 cr[RT], ci[RT];
+#pragma unroll
+  for (int t = 0; t < RT; ++t) {
+    const int i0 = (wave + 8 * t) * 16;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = i0 + lq + 4 * r;
+      const bool live = i < n && lc < w && (i < k0 || i >= k0 + kb);
+      const c128 v = live ? S[lc * ns + i] : mk<double>(0, 0);
+      cr[t][r] = v.x;
+      ci[t][r] = v.y;
+    }
+  }
+  // X operand (B layout: row kk = 4 ks + lq, column lc), read once per k step
+  c128 xb[4];
+#pragma unroll
+  for (int ks = 0; ks < 4; ++ks) {
+    const int kk = 4 * ks + lq;
+    xb[ks] = (kk < kb && lc < w) ? S[lc * ns + k0 + kk] : mk<double>(0, 0);
+  }
+  const c128* Pc = A + (int64_t)n * k0;  // the panel P, written by k_gjb_panel
+#pragma unroll
+  for (int t = 0; t < RT; ++t) {
+    const int i0 = (wave + 8 * t) * 16;
+    if (i0 >= n) break;
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      const int kk = 4 * ks + lq, ia = i0 + lc;  // A layout: row lc, column lq
+      const c128 a = (ia < n && kk < kb) ? Pc[ia + (int64_t)n * kk] : mk<double>(0, 0);
+      cr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, xb[ks].x, cr[t], 0, 0, 0);
+      cr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, xb[ks].y, cr[t], 0, 0, 0);
+      ci[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, xb[ks].y, ci[t], 0, 0, 0);
+      ci[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, xb[ks].x, ci[t], 0, 0, 0);
+    }
+  }
+  __syncthreads();  // every wave has read the strip
+#pragma unroll
+  for (int t = 0; t < RT; ++t) {
+    const int i0 = (wave + 8 * t) * 16;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = i0 + lq + 4 * r;
+      if (i < n && lc < w) S[lc * ns + i] = mk<double>(cr[t][r], ci[t][r]);
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < w * n; e += 512) {
+    const int i = e % n, jj = e / n;
+    A[i + (int64_t)n * (j0 + jj)] = S[jj * ns + i];
+  }
+}
+
 // column-major blocks (kx + L*ky) -> 2-level apply layout, in precision To
 template <typename To>
 __global__ void k_relayout2(const c128* __restrict__ in, cplx<To>* __restrict__ out, int L, int M, int n) {
@@ -1061,11 +1223,35 @@ void assemble1(vs_ctx* ctx, const c128* cL, int L, int M, int N, c128 a, const c
   VS_HIP(hipGetLastError());
 }
 
+// the blocked (matrix-core) inversion for blocks beyond LDS (VS_GJ_BLOCKED=0: the unblocked sweep)
+static bool gj_blocked_enabled() {
+  const char* e = std::getenv("VS_GJ_BLOCKED");
+  return !(e && e[0] == '0');
+}
+
 void invert_blocks_dev(vs_ctx* ctx, c128* blocks, int64_t nblk, int64_t n) {
+  // a whole block in the LDS of one CU (3N <= 96)
   const size_t lds = sizeof(c128) * (size_t)(n * n + n) + sizeof(int) * (size_t)n;
-  if (lds <= 65536) {
+  if (lds <= 160 * 1024) {
     hipLaunchKernelGGL(k_gj_lds, dim3((unsigned)nblk), dim3(256), lds, ctx->stream, blocks, (int)n);
     VS_HIP(hipGetLastError());
+    return;
+  }
+  if (n <= GJB_NMAX && nblk <= 65535 && gj_blocked_enabled()) {
+    DevBuf perm(sizeof(int) * nblk * n), piv(sizeof(int) * nblk * n);
+    const size_t lds_p = sizeof(c128) * (size_t)(GJB * n + n) + sizeof(int) * (size_t)n;
+    const size_t lds_u = sizeof(c128) * (size_t)GJB * (n + 1);
+    const unsigned strips = (unsigned)((n + GJB - 1) / GJB);
+    for (int k0 = 0; k0 < n; k0 += GJB) {
+      hipLaunchKernelGGL(k_gjb_panel, dim3((unsigned)nblk), dim3(256), lds_p, ctx->stream, blocks, (int)n, k0,
+                         perm.as<int>(), piv.as<int>());
+      hipLaunchKernelGGL(k_gjb_update, dim3(strips, (unsigned)nblk), dim3(512), lds_u, ctx->stream, blocks,
+                         (int)n, k0, perm.as<const int>());
+    }
+    hipLaunchKernelGGL(k_gj_unswap, dim3(nblocks(n), (unsigned)nblk), dim3(256), 0, ctx->stream, blocks, n,
+                       piv.as<int>());
+    VS_HIP(hipGetLastError());
+    VS_HIP(hipStreamSynchronize(ctx->stream));  // perm/piv are released on return
     return;
   }
   DevBuf colk(sizeof(c128) * nblk * n), piv(sizeof(int) * nblk * n);
