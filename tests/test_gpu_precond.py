@@ -32,10 +32,10 @@ def test_circ_levels_match_oracle(ctx, shape):
     assert _rel(b2, rb2) <= 1e-13
 
 
-@pytest.mark.parametrize("n,nblk", [(24, 50), (48, 20), (72, 3), (96, 2), (150, 2), (288, 3), (300, 1)])
+@pytest.mark.parametrize("n,nblk", [(24, 50), (48, 20), (72, 3), (96, 2), (150, 2), (288, 3), (300, 1), (330, 1)])
 def test_block_inverse_matches_numpy(ctx, n, nblk):
-    """Whole blocks in LDS up to 3N = 96, the blocked inversion (16-column panels, FP64 matrix-core
-    updates) above, with a partial last panel at 150 and 300."""
+    """Whole blocks in LDS up to 3N = 96, the blocked inversion (FP64 matrix-core updates) above: 32-column
+    panels up to 300 (a partial last panel at 150 and 300), 16-column panels at 330."""
     print(f"[inverse] n={n} nblk={nblk}", flush=True)
     rng = np.random.default_rng(n)
     B = (rng.normal(size=(nblk, n, n)) + 1j * rng.normal(size=(nblk, n, n))) / np.sqrt(n)
@@ -45,7 +45,7 @@ def test_block_inverse_matches_numpy(ctx, n, nblk):
     assert _rel(inv, ref) <= 1e-11
 
 
-@pytest.mark.parametrize("n", [150, 288])
+@pytest.mark.parametrize("n", [150, 288, 330])
 def test_blocked_inverse_pivoting(ctx, n, monkeypatch):
     """Blocks whose pivots lie far from the diagonal (a scaled random permutation plus a small random part,
     so every step swaps rows, also across panels): the blocked inversion against numpy and against the

@@ -296,7 +296,7 @@ __global__ void k_gj_unswap(c128* __restrict__ blocks, int64_t n, const int* __r
   }
 }
 
-// Blocked Gauss-Jordan for blocks too large for LDS, in panels of GJB = 16 columns.  Partial pivoting picks
+// Blocked Gauss-Jordan for blocks too large for LDS, in panels of NB = 32 columns.  Partial pivoting picks
 // the same rows as the unblocked sweep.  The row swaps of a panel commute with its steps (a swap exchanges
 // two rows that the steps treat alike), so the panel's steps equal one block step over its indices K on
 // the row-permuted block B:
@@ -304,98 +304,113 @@ __global__ void k_gj_unswap(c128* __restrict__ blocks, int64_t n, const int* __r
 // k_gjb_panel performs the steps on the panel columns in LDS, which become P = [B_KK^-1; -B_RK B_KK^-1],
 // and records the row permutation; k_gjb_update then sets every other column j to
 //   P X_j + (row-permuted column j with rows K cleared),   X_j = rows K of the row-permuted column j,
-// a complex (n x 16) x (16 x 16) product per strip of 16 columns on the FP64 matrix cores.  The columns
+// a complex (n x NB) x (NB x 16) product per strip of 16 columns on the FP64 matrix cores.  The columns
 // are unswapped at the end as in the unblocked sweep.
-constexpr int GJB = 16;
 constexpr int GJB_NMAX = 512;  // 8 waves x 4 row tiles of 16 in the update kernel
 
-// one workgroup per block: the kb <= 16 steps of the panel starting at column k0
-__global__ __launch_bounds__(256) void k_gjb_panel(c128* __restrict__ blocks, int n, int k0,
+// one workgroup per block, one thread per row holding its kb <= NB panel entries in registers: the steps of
+// the panel starting at column k0.  A step needs only the pivot (a workgroup argmax) and the two rows it
+// exchanges (through LDS, double-buffered by step parity: two barriers per step)
+template <int NB>
+__global__ __launch_bounds__(512) void k_gjb_panel(c128* __restrict__ blocks, int n, int k0,
                                                    int* __restrict__ perm, int* __restrict__ piv) {
-  c128* P = reinterpret_cast<c128*>(vs_smem);  // [kb][n], column-major panel
-  c128* colk = P + (size_t)GJB * n;             // [n]
-  int* pm = reinterpret_cast<int*>(colk + n);   // [n]: source row of each row after this panel's swaps
-  __shared__ double s_best[4];
-  __shared__ int s_bi[4];
-  __shared__ int s_p;
-  __shared__ c128 s_pinv;
+  __shared__ double s_best[8];
+  __shared__ int s_bi[8];
+  __shared__ c128 s_rk[2][NB], s_nk[2][NB];  // row k before a step, and the new row k (pivot row scaled)
+  __shared__ int s_pm[GJB_NMAX];  // source row of each row after this panel's swaps
   const int64_t blk = blockIdx.x;
   c128* A = blocks + blk * (int64_t)n * n;
-  const int tid = threadIdx.x;
-  const int kb = min(GJB, n - k0);
-  for (int e = tid; e < kb * n; e += 256) P[e] = A[(int64_t)n * k0 + e];
-  for (int i = tid; i < n; i += 256) pm[i] = i;
-  __syncthreads();
-  for (int kk = 0; kk < kb; ++kk) {
-    const int k = k0 + kk;
-    const c128* Pk = P + (size_t)kk * n;
-    double best = -1.0;
-    int bi = k;
-    for (int i = k + tid; i < n; i += 256) {
-      const double v = abs2(Pk[i]);
-      if (v > best) { best = v; bi = i; }
-    }
-    wave_argmax(best, bi);
-    if ((tid & 63) == 0) {
-      s_best[tid >> 6] = best;
-      s_bi[tid >> 6] = bi;
-    }
-    __syncthreads();
-    if (tid == 0) {
-      double b0 = s_best[0];
-      int i0 = s_bi[0];
-      for (int q = 1; q < 4; ++q)
-        if (s_best[q] > b0 || (s_best[q] == b0 && s_bi[q] < i0)) { b0 = s_best[q]; i0 = s_bi[q]; }
-      s_p = i0;
-      piv[blk * n + k] = i0;
-      const int t = pm[k];
-      pm[k] = pm[i0];
-      pm[i0] = t;
-    }
-    __syncthreads();
-    const int p = s_p;
-    if (p != k && tid < kb) {
-      c128* c = P + (size_t)tid * n;
-      const c128 t = c[k];
-      c[k] = c[p];
-      c[p] = t;
-    }
-    __syncthreads();
-    for (int i = tid; i < n; i += 256) colk[i] = Pk[i];
-    if (tid == 0) s_pinv = cdiv(mk<double>(1, 0), Pk[k]);
-    __syncthreads();
-    const c128 pinv = s_pinv;
-    if (tid < kb) P[(size_t)tid * n + k] = (tid == kk) ? pinv : P[(size_t)tid * n + k] * pinv;
-    __syncthreads();
-    for (int e = tid; e < kb * n; e += 256) {
-      const int i = e % n, jj = e / n;
-      if (i == k) continue;
-      const c128 f = colk[i];
-      P[e] = (jj == kk) ? -(f * pinv) : P[e] - f * P[(size_t)jj * n + k];
-    }
-    __syncthreads();
+  const int i = threadIdx.x, nw = (int)(blockDim.x >> 6);
+  const bool live = i < n;
+  const int kb = min(NB, n - k0);
+  c128 r[NB];
+#pragma unroll
+  for (int jj = 0; jj < NB; ++jj) {
+    r[jj] = mk<double>(0, 0);
+    if (live && jj < kb) r[jj] = A[i + (int64_t)n * (k0 + jj)];
   }
-  for (int e = tid; e < kb * n; e += 256) A[(int64_t)n * k0 + e] = P[e];
-  for (int i = tid; i < n; i += 256) perm[blk * n + i] = pm[i];
+  if (live) s_pm[i] = i;
+  // entry kk of the row by selects over constant indices (the step loop stays rolled, r in registers)
+  // (selects written per component: a select of whole complex values becomes a select of their addresses,
+  // which sends the row to scratch memory)
+  auto csel = [](bool c, c128 a, c128 b) { return mk<double>(c ? a.x : b.x, c ? a.y : b.y); };
+  auto pick = [&](int kk) {
+    c128 v = r[0];
+#pragma unroll
+    for (int jj = 1; jj < NB; ++jj) v = csel(jj == kk, r[jj], v);
+    return v;
+  };
+  for (int kk = 0; kk < kb; ++kk) {
+    const int k = k0 + kk, buf = kk & 1;
+    const c128 rkk = pick(kk);
+    double best = (live && i >= k) ? abs2(rkk) : -1.0;
+    int bi = i;
+    wave_argmax(best, bi);
+    if ((i & 63) == 0) {
+      s_best[i >> 6] = best;
+      s_bi[i >> 6] = bi;
+    }
+    __syncthreads();
+    // every thread reduces the wave results in the same order, so all agree on the pivot row p
+    double b0 = s_best[0];
+    int p = s_bi[0];
+    for (int q = 1; q < nw; ++q)
+      if (s_best[q] > b0 || (s_best[q] == b0 && s_bi[q] < p)) { b0 = s_best[q]; p = s_bi[q]; }
+    if (i == 0) {
+      piv[blk * n + k] = p;
+      const int t = s_pm[k];
+      s_pm[k] = s_pm[p];
+      s_pm[p] = t;
+    }
+    if (i == k) {
+#pragma unroll
+      for (int jj = 0; jj < NB; ++jj) s_rk[buf][jj] = r[jj];
+    }
+    if (i == p) {
+      const c128 pv = cdiv(mk<double>(1, 0), rkk);
+#pragma unroll
+      for (int jj = 0; jj < NB; ++jj) s_nk[buf][jj] = csel(jj == kk, pv, r[jj] * pv);
+    }
+    __syncthreads();
+    // the step on the row-swapped panel: row k <- new row k; row p <- old row k - f_p (new row k);
+    // other rows <- own - f_i (new row k); column kk <- -f pinv (pinv on row k)
+    const c128 pinv = s_nk[buf][kk];
+    if (live) {
+      const bool rowk = i == k, rowp = i == p && p != k;
+      const c128 f = csel(rowp, s_rk[buf][kk], rkk);
+#pragma unroll
+      for (int jj = 0; jj < NB; ++jj) {
+        const c128 nk = s_nk[buf][jj];
+        const c128 src = csel(rowp, s_rk[buf][jj], r[jj]);
+        r[jj] = csel(rowk, nk, csel(jj == kk, -(f * pinv), src - f * nk));
+      }
+    }
+  }
+#pragma unroll
+  for (int jj = 0; jj < NB; ++jj)
+    if (live && jj < kb) A[i + (int64_t)n * (k0 + jj)] = r[jj];
+  __syncthreads();
+  if (live) perm[blk * n + i] = s_pm[i];
 }
 
 typedef double vs_d4 __attribute__((ext_vector_type(4)));
 
 // one workgroup per (strip of 16 columns, block), 8 waves; the strip is staged row-permuted in LDS (all of
 // its reads precede its writes, so the update is in place), each wave takes row tiles wave, wave + 8, ...
+template <int NB>
 __global__ __launch_bounds__(512) void k_gjb_update(c128* __restrict__ blocks, int n, int k0,
                                                     const int* __restrict__ perm) {
-  constexpr int RT = GJB_NMAX / (16 * 8);
+  constexpr int RT = GJB_NMAX / (16 * 8), KS = NB / 4;
   const int strip = blockIdx.x;
-  if (strip == k0 / GJB) return;  // the panel itself
+  if (strip * 16 >= k0 && strip * 16 < k0 + NB) return;  // the panel itself
   c128* S = reinterpret_cast<c128*>(vs_smem);  // [16][n + 1] row-permuted strip, column-major
   const int ns = n + 1;
   const int64_t blk = blockIdx.y;
   c128* A = blocks + blk * (int64_t)n * n;
   const int* pm = perm + blk * n;
-  const int kb = min(GJB, n - k0);
-  const int j0 = strip * GJB;
-  const int w = min(GJB, n - j0);
+  const int kb = min(NB, n - k0);
+  const int j0 = strip * 16;
+  const int w = min(16, n - j0);
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int lc = lane & 15, lq = lane >> 4;
   for (int e = tid; e < w * n; e += 512) {
@@ -420,9 +435,9 @@ __global__ __launch_bounds__(512) void k_gjb_update(c128* __restrict__ blocks, i
     }
   }
   // X operand (B layout: row kk = 4 ks + lq, column lc), read once per k step
-  c128 xb[4];
+  c128 xb[KS];
 #pragma unroll
-  for (int ks = 0; ks < 4; ++ks) {
+  for (int ks = 0; ks < KS; ++ks) {
     const int kk = 4 * ks + lq;
     xb[ks] = (kk < kb && lc < w) ? S[lc * ns + k0 + kk] : mk<double>(0, 0);
   }
@@ -432,7 +447,7 @@ __global__ __launch_bounds__(512) void k_gjb_update(c128* __restrict__ blocks, i
     const int i0 = (wave + 8 * t) * 16;
     if (i0 >= n) break;
 #pragma unroll
-    for (int ks = 0; ks < 4; ++ks) {
+    for (int ks = 0; ks < KS; ++ks) {
       const int kk = 4 * ks + lq, ia = i0 + lc;  // A layout: row lc, column lq
       const c128 a = (ia < n && kk < kb) ? Pc[ia + (int64_t)n * kk] : mk<double>(0, 0);
       cr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, xb[ks].x, cr[t], 0, 0, 0);
@@ -1239,13 +1254,15 @@ void invert_blocks_dev(vs_ctx* ctx, c128* blocks, int64_t nblk, int64_t n) {
   }
   if (n <= GJB_NMAX && nblk <= 65535 && gj_blocked_enabled()) {
     DevBuf perm(sizeof(int) * nblk * n), piv(sizeof(int) * nblk * n);
-    const size_t lds_p = sizeof(c128) * (size_t)(GJB * n + n) + sizeof(int) * (size_t)n;
-    const size_t lds_u = sizeof(c128) * (size_t)GJB * (n + 1);
-    const unsigned strips = (unsigned)((n + GJB - 1) / GJB);
-    for (int k0 = 0; k0 < n; k0 += GJB) {
-      hipLaunchKernelGGL(k_gjb_panel, dim3((unsigned)nblk), dim3(256), lds_p, ctx->stream, blocks, (int)n, k0,
-                         perm.as<int>(), piv.as<int>());
-      hipLaunchKernelGGL(k_gjb_update, dim3(strips, (unsigned)nblk), dim3(512), lds_u, ctx->stream, blocks,
+    // panels of 32 columns while the panel fits in LDS (3N <= 300), else 16
+    const int NB = 32;
+    const unsigned pthreads = (unsigned)((n + 63) / 64 * 64);
+    const size_t lds_u = sizeof(c128) * (size_t)16 * (n + 1);
+    const unsigned strips = (unsigned)((n + 15) / 16);
+    for (int k0 = 0; k0 < n; k0 += NB) {
+      hipLaunchKernelGGL(k_gjb_panel<32>, dim3((unsigned)nblk), dim3(pthreads), 0, ctx->stream, blocks, (int)n,
+                         k0, perm.as<int>(), piv.as<int>());
+      hipLaunchKernelGGL(k_gjb_update<32>, dim3(strips, (unsigned)nblk), dim3(512), lds_u, ctx->stream, blocks,
                          (int)n, k0, perm.as<const int>());
     }
     hipLaunchKernelGGL(k_gj_unswap, dim3(nblocks(n), (unsigned)nblk), dim3(256), 0, ctx->stream, blocks, n,
