@@ -413,9 +413,12 @@ __global__ __launch_bounds__(512) void k_gjb_update(c128* __restrict__ blocks, i
   const int w = min(16, n - j0);
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int lc = lane & 15, lq = lane >> 4;
-  for (int e = tid; e < w * n; e += 512) {
-    const int i = e % n, jj = e / n;
-    S[jj * ns + i] = A[pm[i] + (int64_t)n * (j0 + jj)];
+  // one row per thread (n <= 512): its source row is read once, its 16 entries are 16 coalesced loads
+  if (tid < n) {
+    const c128* src = A + pm[tid] + (int64_t)n * j0;
+#pragma unroll
+    for (int jj = 0; jj < 16; ++jj)
+      if (jj < w) S[jj * ns + tid] = src[(int64_t)n * jj];
   }
   __syncthreads();
   // accumulators start from the permuted strip with rows K cleared; the FP64 16x16x4 result layout holds
@@ -467,9 +470,11 @@ __global__ __launch_bounds__(512) void k_gjb_update(c128* __restrict__ blocks, i
     }
   }
   __syncthreads();
-  for (int e = tid; e < w * n; e += 512) {
-    const int i = e % n, jj = e / n;
-    A[i + (int64_t)n * (j0 + jj)] = S[jj * ns + i];
+  if (tid < n) {
+    c128* dst = A + tid + (int64_t)n * j0;
+#pragma unroll
+    for (int jj = 0; jj < 16; ++jj)
+      if (jj < w) dst[(int64_t)n * jj] = S[jj * ns + tid];
   }
 }
 
