@@ -738,23 +738,32 @@ __global__ __launch_bounds__(256) void k_col2(const cplx<T>* __restrict__ tw, cp
   }
 }
 
+// columns (kx) per workgroup of the z pass: as many as 1024 threads and 80 KB of LDS (two workgroups per
+// CU) allow, at most 32; the z lines of a column tile are read and written in runs of W consecutive kx, so
+// long lines (2N = 128 ... 256) still get runs of 128 bytes (W = 16 in FP32, 8 in FP64)
+template <typename T>
+constexpr int fz_lds_elems() { return 80 * 1024 / (int)sizeof(cplx<T>); }
 template <typename T, int N2>
 struct FzCfg {
   static constexpr int R1 = Split<N2>::R1, R2 = Split<N2>::R2, TPL = cmax(R1, R2);
-  static constexpr int WA = 512 / (3 * TPL), WB = lds_budget<T>() / (3 * N2);
+  static constexpr int WA = 1024 / (3 * TPL), WB = fz_lds_elems<T>() / (3 * N2);
   static constexpr int W0 = pow2_floor(WA < WB ? WA : WB);
-  static constexpr int W = W0 > 32 ? 32 : (W0 < 1 ? 1 : W0);  // columns (kx) per workgroup
+  static constexpr int W = W0 > 32 ? 32 : (W0 < 1 ? 1 : W0);
   static constexpr int THREADS = 3 * TPL * W;
 };
 
-// z pass of the matvec: zero-pad z to 2N, FFT, 3x3 symmetric tensor product, inverse FFT, crop to N
+// z pass of the matvec: zero-pad z to 2N, FFT, 3x3 symmetric tensor product, inverse FFT, crop to N.
+// One LDS buffer holds the three components' lines; each stage reads all of its inputs before a barrier
+// and writes after it.  The tensor stage takes kz and 2N - kz together: they share one octant entry (the
+// xz and yz components change sign), so each entry is read once per (kx, ky) instead of twice.
 template <typename T, int N2>
-__global__ __launch_bounds__(512) void k_fz2(const cplx<T>* __restrict__ tw, cplx<T>* S, TensorC<T> fN, int L2, int M2,
-                                             int N, int64_t S_rhs) {
+__global__ __launch_bounds__((FzCfg<T, N2>::THREADS)) void k_fz2(const cplx<T>* __restrict__ tw, cplx<T>* S,
+                                                                TensorC<T> fN, int L2, int M2, int N, int64_t S_rhs) {
   using C = FzCfg<T, N2>;
-  constexpr int R1 = C::R1, R2 = C::R2, W = C::W;
-  __shared__ cplx<T> xch[3 * N2 * W];
-  __shared__ cplx<T> spec[3 * N2 * W];
+  constexpr int R1 = C::R1, R2 = C::R2, W = C::W, NH = N2 / 2;
+  __shared__ cplx<T> buf[3 * N2 * W];
+  cplx<T>* xch = buf;
+  cplx<T>* spec = buf;
   const int t = threadIdx.x;
   const int col = t % W, r = t / W;
   const int c = r % 3, s = r / 3;
@@ -787,41 +796,57 @@ __global__ __launch_bounds__(512) void k_fz2(const cplx<T>* __restrict__ tw, cpl
     for (int k2 = 0; k2 < R2; ++k2) xch[(c * N2 + s * R2 + k2) * W + col] = v[k2];
   }
   __syncthreads();
+  cplx<T> u[R1];
   if (s < R2) {
-    cplx<T> u[R1];
 #pragma unroll
     for (int n1 = 0; n1 < R1; ++n1) u[n1] = xch[(c * N2 + n1 * R2 + s) * W + col];
     fft2_step2<T, N2, false>(u);
+  }
+  __syncthreads();
+  if (s < R2) {
 #pragma unroll
     for (int k1 = 0; k1 < R1; ++k1) spec[(c * N2 + s + R2 * k1) * W + col] = u[k1];
   }
   __syncthreads();
-  // tensor product, one (kx, kz) per iteration
-  for (int e = t; e < W * N2; e += C::THREADS) {
+  // tensor product, one (kx, kz <= N) per iteration, applied at kz and at 2N - kz
+  for (int e = t; e < W * (NH + 1); e += C::THREADS) {
     const int cc = e % W, kz = e / W;
     if (kx0 + cc < L2) {
       cplx<T> f[6];
       fN.load6(kx0 + cc, ky, kz, f);
       cplx<T>* sp = spec + kz * W + cc;
-      const cplx<T> jx = sp[0], jy = sp[N2 * W], jz = sp[2 * N2 * W];
+      cplx<T> jx = sp[0], jy = sp[N2 * W], jz = sp[2 * N2 * W];
       sp[0] = f[0] * jx + f[1] * jy + f[2] * jz;
       sp[N2 * W] = f[1] * jx + f[3] * jy + f[4] * jz;
       sp[2 * N2 * W] = f[2] * jx + f[4] * jy + f[5] * jz;
+      if (kz != 0 && kz != NH) {
+        sp = spec + (N2 - kz) * W + cc;
+        jx = sp[0];
+        jy = sp[N2 * W];
+        jz = sp[2 * N2 * W];
+        sp[0] = f[0] * jx + f[1] * jy - f[2] * jz;
+        sp[N2 * W] = f[1] * jx + f[3] * jy - f[4] * jz;
+        sp[2 * N2 * W] = -(f[2] * jx) - f[4] * jy + f[5] * jz;
+      }
     }
   }
   __syncthreads();
   // inverse transform, keep z < N
-  if (s < R1) {
+  {
     cplx<T> v[R2];
+    if (s < R1) {
 #pragma unroll
-    for (int n2 = 0; n2 < R2; ++n2) v[n2] = spec[(c * N2 + s + R1 * n2) * W + col];
-    fft2_step1<T, N2, true>(v, s, tw);
+      for (int n2 = 0; n2 < R2; ++n2) v[n2] = spec[(c * N2 + s + R1 * n2) * W + col];
+      fft2_step1<T, N2, true>(v, s, tw);
+    }
+    __syncthreads();
+    if (s < R1) {
 #pragma unroll
-    for (int k2 = 0; k2 < R2; ++k2) xch[(c * N2 + s * R2 + k2) * W + col] = v[k2];
+      for (int k2 = 0; k2 < R2; ++k2) xch[(c * N2 + s * R2 + k2) * W + col] = v[k2];
+    }
   }
   __syncthreads();
   if (s < R2) {
-    cplx<T> u[R1];
 #pragma unroll
     for (int n1 = 0; n1 < R1; ++n1) u[n1] = xch[(c * N2 + n1 * R2 + s) * W + col];
     fft2_step2<T, N2, true>(u);
@@ -1634,7 +1659,7 @@ static bool launch_fz2(vs_ctx* ctx, cplx<T>* S, const TensorC<T>& fN, int L2, in
 #define VS_FZ_CASE(NN)                                                                                   \
   case NN: {                                                                                             \
     using Cf = FzCfg<T, NN>;                                                                             \
-    if (3 * NN * Cf::W > lds_budget<T>()) return false;                                                  \
+    if (3 * NN * Cf::W > fz_lds_elems<T>()) return false;                                                \
     const int tiles = (L2 + Cf::W - 1) / Cf::W;                                                          \
     dim3 grid((unsigned)(((tiles * M2 + 7) / 8) * 8), (unsigned)nrhs);                                   \
     hipLaunchKernelGGL((k_fz2<T, NN>), grid, dim3(Cf::THREADS), 0, ctx->stream, pd.tw, S, fN, L2, M2, N, \
@@ -1812,6 +1837,17 @@ static size_t mvp_scratch_bytes(const vs_op* op, int variant, int nrhs) {
 
 // ext != nullptr: the scratch is the caller's ext_bytes buffer (nothing is allocated), and the variant is
 // the tuned one if its scratch fits, else the available variant with the smallest scratch
+// right-hand sides per group of the five-pass matvec: as many as keep the group's intermediate volumes
+// within VS_P5_GROUP_MB (default 192) MB; 0 = all at once
+static int pass5_rhs_group(int64_t bytes_per_rhs) {
+  static const int64_t budget = [] {
+    const char* e = std::getenv("VS_P5_GROUP_MB");
+    return (int64_t)(e ? std::max(0, std::atoi(e)) : 192) << 20;
+  }();
+  if (budget == 0) return 1 << 30;
+  return (int)std::max<int64_t>(1, budget / std::max<int64_t>(1, bytes_per_rhs));
+}
+
 template <typename T>
 static void mvp_impl(vs_op* op, const cplx<T>* x, cplx<T>* y, int transp, int nrhs, cplx<T>* ext = nullptr,
                      size_t ext_bytes = 0, const c128* arhs = nullptr) {
@@ -1820,6 +1856,17 @@ static void mvp_impl(vs_op* op, const cplx<T>* x, cplx<T>* y, int transp, int nr
   const int L2 = 2 * L, M2 = 2 * M, N2 = 2 * N;
   const int64_t S_rhs = (int64_t)L2 * M2 * N * 3;
   int variant = choose_variant<T>(op);
+  if (variant == 0 && nrhs > 1) {
+    // five passes over many right-hand sides: in groups whose intermediate volumes stay within the
+    // 256 MB last-level cache, so that each pass reads what the previous one wrote from there
+    const int ck = pass5_rhs_group(S_rhs * (int64_t)sizeof(cplx<T>));
+    if (nrhs > ck) {
+      for (int j0 = 0; j0 < nrhs; j0 += ck)
+        mvp_impl<T>(op, x + j0 * op->nunk, y + j0 * op->nunk, transp, std::min(ck, nrhs - j0), ext, ext_bytes,
+                    arhs ? arhs + j0 : nullptr);
+      return;
+    }
+  }
   cplx<T>* S = nullptr;
   if (ext) {
     if (mvp_scratch_bytes<T>(op, variant, nrhs) > ext_bytes) {
