@@ -73,7 +73,10 @@ int vs_op_create_dda(vs_ctx* ctx, int L, int M, int N, const double* r, double k
                      int precisions, vs_op** out);
 
 /* Operator from a caller-supplied circulant FFT fN (2L*2M*2N*6 complex128), e.g. a Galerkin operator
- * (the fN argument of MVP.m:1).  Toep (L*M*N*6, nullable) enables vs_pc_create_circ. */
+ * (the fN argument of MVP.m:1).  Toep (L*M*N*6, nullable) enables vs_pc_create_circ.  The operator keeps
+ * one octant of fN and rebuilds the rest from the mirror parity of each component (xy odd in x and y, xz
+ * in x and z, yz in y and z), which the circulant embedding of a dyadic tensor has; an fN without it
+ * (deviation above 1e-9 of the component's largest entry) is refused with VS_ERR_ARG. */
 /* Galerkin (piecewise-constant JVIE) N operator on an L x M x N grid of cubes of side dx
  * (replaces getOPERATORS.m:71-76 with op = 'N', type = 'FFTofC': assembly_nop_const.m:1-174, then
  * circulant_nop_const.m and fft_operator.m).  The kernel is exp(-i ko R)/R.  singular: the five

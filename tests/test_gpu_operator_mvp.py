@@ -83,6 +83,19 @@ def test_mvp_per_voxel_coefficients_and_external_fN(ctx):
         y = api.MVP(x, fref, Mr, Mc, Gram, flag, idx3)
         yref = OM.MVP(x, fref, Mr, Mc, Gram, flag, idx3)
         assert _rel(y, yref) <= 1e-12
+    # an in-place edit of the same host array (same address, head and tail unchanged) is a new operator
+    f2 = fref.copy()
+    y0 = api.MVP(x, f2, Mr, Mc, Gram, "notransp", idx3)
+    mid = (L, M, N, 0)
+    f2[mid] *= 1.5  # the Nyquist entry (its own mirror in x, y and z), away from the array's ends
+    y1 = api.MVP(x, f2, Mr, Mc, Gram, "notransp", idx3)
+    assert _rel(y1, OM.MVP(x, f2, Mr, Mc, Gram, "notransp", idx3)) <= 1e-12 and _rel(y1, y0) > 1e-6
+    # an fN without the mirror parity the operator rebuilds its octants from is refused
+    f3 = fref.copy()
+    f3[1, 2, 3, 1] += 0.5 * np.abs(fref[..., 1]).max()
+    with pytest.raises(capi.VoxScatterError) as ei:
+        api.Operator.from_fN(f3, ctx=ctx)
+    assert ei.value.code == capi.VS_ERR_ARG and "parity" in str(ei.value)
 
 
 def test_mvp_fp32_and_multi_rhs(ctx):

@@ -125,6 +125,23 @@ def _digest(*arrays) -> str:
     return h.hexdigest()
 
 
+def _full_digest(a: np.ndarray) -> str:
+    """Digest of every element of a (in memory order, no copy when a is C- or F-contiguous): the key under
+    which a device copy made from a host array is reused, so that an in-place edit anywhere in the array,
+    or another array at a reused address, makes a new copy."""
+    a = np.asarray(a)
+    flat = a.reshape(-1, order="A") if (a.flags.c_contiguous or a.flags.f_contiguous) else np.ascontiguousarray(a).ravel()
+    buf = memoryview(flat.view(np.uint8))
+    try:
+        import xxhash
+        h = xxhash.xxh3_128()
+        h.update(buf)
+        body = h.hexdigest()
+    except ImportError:
+        body = hashlib.blake2b(buf, digest_size=16).hexdigest()
+    return f"{a.dtype.str}{a.shape}{'F' if a.flags.f_contiguous and not a.flags.c_contiguous else 'C'}{body}"
+
+
 class DeviceArray:
     """A complex vector (or batch of vectors) in HBM, allocated through the library's own runtime."""
 
@@ -440,7 +457,7 @@ def _as_operator(fN) -> Operator:
     if isinstance(fN, Operator):
         return fN
     fN = np.asarray(fN)
-    key = (fN.__array_interface__["data"][0], fN.shape, _digest(fN.reshape(-1)[:512], fN.reshape(-1)[-512:]))
+    key = _full_digest(fN)
     op = _fn_cache.get(key)
     if op is None:
         op = Operator.from_fN(fN)
@@ -510,7 +527,7 @@ _pc_cache: dict = {}
 def _pc_from_blocks(levels, inv_blocks, L, M, N, idx):
     idx = np.asarray(idx, dtype=np.int64)
     ib = np.asarray(inv_blocks)
-    key = (levels, ib.__array_interface__["data"][0], ib.shape, _digest(ib.reshape(-1)[:256], idx))
+    key = (levels, L, M, N, _full_digest(ib), _digest(idx))
     pc = _pc_cache.get(key)
     if pc is None:
         pc = Preconditioner.from_blocks(levels, ib, L, M, N, idx)

@@ -334,6 +334,38 @@ int vs_op_create_galerkin(vs_ctx* ctx, int L, int M, int N, double dx, double ko
   });
 }
 
+// The operator keeps one octant of fN and rebuilds the rest from the parity of each component (xy odd in
+// x and y, xz in x and z, yz in y and z), which every circulant embedding of a dyadic Green's tensor has
+// (circulant_nop_const.m, gperiodic_coeff_nop.m).  A caller's fN without that symmetry is refused rather
+// than silently replaced: every mirrored entry must equal the signed entry it is rebuilt from to 1e-9
+// of the largest magnitude of its component.
+static void check_fN_parity(const double* fN, int L, int M, int N) {
+  const int64_t L2 = 2 * L, M2 = 2 * M, N2 = 2 * N, P = L2 * M2 * N2;
+  static const int sx[6] = {1, -1, -1, 1, 1, 1}, sy[6] = {1, -1, 1, 1, -1, 1}, sz[6] = {1, 1, -1, 1, -1, 1};
+  auto at = [&](int p, int64_t kx, int64_t ky, int64_t kz) {
+    const double* e = fN + 2 * (p * P + kx + L2 * (ky + M2 * kz));
+    return std::complex<double>(e[0], e[1]);
+  };
+  for (int p = 0; p < 6; ++p) {
+    double mx = 0;
+    for (int64_t i = 0; i < P; ++i) mx = std::max(mx, std::abs(std::complex<double>(fN[2 * (p * P + i)], fN[2 * (p * P + i) + 1])));
+    const double tol = 1e-9 * mx;
+    double worst = 0;
+    for (int64_t kz = 0; kz < N2; ++kz)
+      for (int64_t ky = 0; ky < M2; ++ky)
+        for (int64_t kx = 0; kx < L2; ++kx) {
+          const std::complex<double> v = at(p, kx, ky, kz);
+          if (kx > L) worst = std::max(worst, std::abs(v - (double)sx[p] * at(p, L2 - kx, ky, kz)));
+          if (ky > M) worst = std::max(worst, std::abs(v - (double)sy[p] * at(p, kx, M2 - ky, kz)));
+          if (kz > N) worst = std::max(worst, std::abs(v - (double)sz[p] * at(p, kx, ky, N2 - kz)));
+        }
+    VS_CHECK(worst <= tol, VS_ERR_ARG,
+             "fN component " + std::to_string(p) +
+                 " lacks the mirror parity of a circulant-embedded dyadic tensor (deviation " + std::to_string(worst) +
+                 " against a largest entry of " + std::to_string(mx) + ")");
+  }
+}
+
 int vs_op_create_fN(vs_ctx* ctx, int L, int M, int N, const double* fN, const double* Toep, int precisions,
                     vs_op** out) {
   return guarded([&] {
@@ -341,6 +373,7 @@ int vs_op_create_fN(vs_ctx* ctx, int L, int M, int N, const double* fN, const do
     *out = nullptr;
     check_dims(L, M, N);
     VS_CHECK((precisions & (VS_FP64 | VS_FP32)) != 0, VS_ERR_ARG, "precisions must include VS_FP64 or VS_FP32");
+    check_fN_parity(fN, L, M, N);
     DeviceGuard g(ctx->device);
     auto op = std::make_unique<vs_op>();
     op->ctx = ctx;
