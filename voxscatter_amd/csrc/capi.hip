@@ -1,5 +1,6 @@
 // extern "C" boundary: argument checks, host<->device staging, status codes.
 #include <algorithm>
+#include <atomic>
 #include <complex>
 #include <cstring>
 
@@ -116,6 +117,16 @@ LinOp<T> make_linop(vs_op* op, vs_pc* pc) {
   LinOp<T> L;
   L.A = [op, prec](const cplx<T>* x, cplx<T>* y) { mvp_dev(op, x, y, 0, prec, 1); };
   if (pc) L.M = [pc, prec](const cplx<T>* x, cplx<T>* y) { pc_apply_dev(pc, x, y, prec, 1); };
+  if (pc && bi_fused_available<T>(op, pc)) {
+    L.F1 = [op, pc](const cplx<T>* r, const cplx<T>* p0, const cplx<T>* v0, cplx<T>* p, cplx<T>* ph, cplx<T>* v,
+                    const cplx<T>* rt, const BiScal* sc, BiParts& P) {
+      bi_fused_h1<T>(op, pc, r, p0, v0, p, ph, v, rt, sc, P);
+    };
+    L.F2 = [op, pc](const cplx<T>* x, const cplx<T>* ph, const cplx<T>* r, const cplx<T>* v, cplx<T>* xh, cplx<T>* s,
+                    cplx<T>* sh, cplx<T>* t, const BiScal* sc, BiParts& P) {
+      bi_fused_h2<T>(op, pc, x, ph, r, v, xh, s, sh, t, sc, P);
+    };
+  }
   L.Ab = [op, prec](const cplx<T>* x, cplx<T>* y, int k) { mvp_dev(op, x, y, 0, prec, k); };
   if (pc) L.Mb = [pc, prec](const cplx<T>* x, cplx<T>* y, int k) { pc_apply_dev(pc, x, y, prec, k); };
   L.Abd = [op, prec](const cplx<T>* x, cplx<T>* y, int k, const c128* d) { mvp_dev_diag(op, x, y, 0, prec, k, d); };
@@ -439,6 +450,8 @@ int vs_op_set_domain(vs_op* op, const int64_t* idx, int64_t nidx, const double* 
     h2d(op->ctx, op->map3.p, map.data(), sizeof(int) * 3 * V);
     op->nunk = nidx;
     op->idx_host.assign(idx, idx + nidx);
+    static std::atomic<uint64_t> next_dom{1};
+    op->dom_id = next_dom++;
     auto at = [](const double* p, int64_t len, int64_t v) {
       return len == 1 ? mk<double>(p[0], p[1]) : mk<double>(p[2 * v], p[2 * v + 1]);
     };
@@ -617,6 +630,7 @@ int vs_pc_create_circ(vs_op* op, int levels, const double a[2], const double* w,
     pc->V = V;
     pc->precisions = precisions;
     pc->nunk = op->nunk;
+    pc->dom_id = op->dom_id;
     pc->map3.ensure(sizeof(int) * 3 * V);
     VS_HIP(hipMemcpyAsync(pc->map3.p, op->map3.p, sizeof(int) * 3 * V, hipMemcpyDeviceToDevice, ctx->stream));
     const int64_t nb = levels == 2 ? 3LL * N : 3LL * M * N;

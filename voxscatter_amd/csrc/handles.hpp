@@ -69,6 +69,7 @@ struct vs_op {
   int64_t nunk = 0;
   vs::DevBuf map3;  // int32[3V]: global unknown -> compressed position or -1
   std::vector<int64_t> idx_host;
+  uint64_t dom_id = 0;  // identifies this domain (process-unique, renewed by every vs_op_set_domain)
   vs::Coef a, at, b;  // a = Gram*Mr, at = Gram*conj(Mr), b = Mc
   // scratch
   vs::DevBuf S;          // padded spectra 2L*2M*N*3 per rhs
@@ -90,6 +91,7 @@ struct vs_pc {
   vs::DevBuf invq32;     // mirrored 2-level FP32 blocks for the matrix-core product: full, column-major, block kx + Lb ky
   vs::DevBuf map3;       // own copy of the unknown map
   int64_t nunk = 0;
+  uint64_t dom_id = 0;   // the operator domain the map was copied from (0: a caller's unknown set)
   vs::DevBuf U, U2;      // work volumes (3V per rhs)
   vs::DevBuf hx, hy;
 };

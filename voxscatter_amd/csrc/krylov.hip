@@ -152,9 +152,23 @@ SolveResult solve_bicgstab(Engine<T>& E, const LinOp<T>& op, const cplx<T>* b, c
   VS_HIP(hipMemcpyAsync(sc, up, sizeof(BiScal), hipMemcpyHostToDevice, ctx->stream));
   bi_stage_init<T>(E, rt, r, sc);
   EventPair ev;
+  // Fused halves (operator and preconditioner permitting) on the fast path only: a first half that must
+  // finish its own half step (moresteps > 0) and a second half recomputed from a replaced s use the
+  // separate stages, so they never depend on the fused passes' vectors.
+  const bool fused = op.F1 && op.F2 && op.M;
+  BiParts bp;
+  if (fused) {
+    bi_parts_take<T>(E, bp);
+    bi_stage_xnorm<T>(E, x, bp);
+  }
 
   auto first_half = [&](int i, bool finish_half) {
     cv* pi = P(i);
+    if (fused && !finish_half) {
+      op.F1(r, P(i - 1), V(i - 1), pi, ph, V(i), rt, sc, bp);
+      bi_stage_alpha_fused<T>(E, sc, bp);
+      return;
+    }
     bi_stage_rho<T>(E, r, V(i - 1), P(i - 1), pi, sc);
     const cv* phv = op.M ? ph : pi;
     if (op.M) op.M(pi, ph);
@@ -163,6 +177,11 @@ SolveResult solve_bicgstab(Engine<T>& E, const LinOp<T>& op, const cplx<T>* b, c
   };
   // fold_half: the half step's ||s|| is reduced together with the omega stage
   auto second_half = [&](int i, bool fold_half) {
+    if (fused && fold_half) {
+      op.F2(x, ph, r, V(i), XH(i), S(i), sh, t, sc, bp);
+      bi_stage_omega_full_fused<T>(E, t, S(i), sh, XH(i), rt, x, r, sc, bp);
+      return;
+    }
     const cv* shv = op.M ? sh : S(i);
     if (op.M) op.M(S(i), sh);
     op.A(shv, t);

@@ -14,6 +14,7 @@
 // volume slab-major, Sx(kx)[c][z][y], and one workgroup per kx slab does the y FFT, the z FFT, the tensor
 // product, the inverse z FFT and the inverse y FFT in LDS (k_yz), so HBM sees the slab once in and once
 // out plus the octant of the tensor for that kx, stored slab-major as well.
+#include "bi_device.hpp"
 #include "fft2.hpp"
 #include "fft_device.hpp"
 #include "handles.hpp"
@@ -21,6 +22,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 
 namespace vs {
 
@@ -211,16 +213,16 @@ struct IoMvIx {
     const int m = map3[g * L + xp];
     return {m, x[rhs * nunk + (m >= 0 ? m : 0)]};
   }
-  __device__ __forceinline__ void store(int64_t g, int xp, int rhs, cplx<T> v, const Pre& p) const {
-    if (p.m < 0) return;
+  // the output value at an unknown (p.m >= 0) from the cropped inverse transform v
+  __device__ __forceinline__ cplx<T> value(int64_t g, int xp, int rhs, cplx<T> v, const Pre& p) const {
     const int64_t vox = (g * L + xp) % V;
     const cplx<T> conv = invP * v;
-    cplx<T> out;
-    if (!transp)
-      out = (arhs ? ccast<T>(arhs[rhs]) : a.at(vox)) * p.J - b.at(vox) * conv;
-    else
-      out = (arhs ? ccast<T>(arhs[rhs]) : at.at(vox)) * p.J - cconj(conv);
-    y[rhs * nunk + p.m] = out;
+    if (!transp) return (arhs ? ccast<T>(arhs[rhs]) : a.at(vox)) * p.J - b.at(vox) * conv;
+    return (arhs ? ccast<T>(arhs[rhs]) : at.at(vox)) * p.J - cconj(conv);
+  }
+  __device__ __forceinline__ void store(int64_t g, int xp, int rhs, cplx<T> v, const Pre& p) const {
+    if (p.m < 0) return;
+    y[rhs * nunk + p.m] = value(g, xp, rhs, v, p);
   }
   __device__ __forceinline__ void store(int64_t g, int xp, int rhs, cplx<T> v) const {
     store(g, xp, rhs, v, pre(g, xp, rhs));
@@ -272,6 +274,45 @@ struct IoMvIxSlab2 {
     out.store(g, xp, rhs, v, p);
   }
   __device__ __forceinline__ void store(int64_t g, int xp, int rhs, cplx<T> v) const { out.store(g, xp, rhs, v); }
+};
+
+// The last matvec pass of a slab-major variant (Base = IoMvIxSlab or IoMvIxSlab2) with inner products of its
+// output y formed on the way, in FP64 per workgroup: KD = 1: conj(w) . y; KD = 2: ||y||^2 and conj(y) . w, w
+// read at the same unknowns (one right-hand side).
+template <typename T, class Base, int KD>
+struct IoDot {
+  Base io;
+  const cplx<T>* w;
+  c128* partial;
+  static constexpr int KRED = KD;
+  using Pre = typename Base::Pre;
+  __device__ __forceinline__ cplx<T> load(int64_t g, int kx, int rhs) const { return io.load(g, kx, rhs); }
+  __device__ __forceinline__ Pre pre(int64_t g, int xp, int rhs) const { return io.pre(g, xp, rhs); }
+  __device__ __forceinline__ void store(int64_t g, int xp, int rhs, cplx<T> v, const Pre& p, double (&re)[KD],
+                                        double (&im)[KD]) const {
+    if (p.m < 0) return;
+    const cplx<T> o = io.out.value(g, xp, rhs, v, p);
+    io.out.y[rhs * io.out.nunk + p.m] = o;
+    const cplx<T> c = w[rhs * io.out.nunk + p.m];
+    if (KD == 1) {
+      re[0] += (double)c.x * o.x + (double)c.y * o.y;
+      im[0] += (double)c.x * o.y - (double)c.y * o.x;
+    } else {
+      re[0] += (double)o.x * o.x + (double)o.y * o.y;
+      re[KD - 1] += (double)o.x * c.x + (double)o.y * c.y;
+      im[KD - 1] += (double)o.x * c.y - (double)o.y * c.x;
+    }
+  }
+};
+
+// number of FP64 sums an IO accumulates over its stores (0 for the plain IOs)
+template <class IO, class = void>
+struct RedK {
+  static constexpr int K = 0;
+};
+template <class IO>
+struct RedK<IO, std::void_t<decltype(IO::KRED)>> {
+  static constexpr int K = IO::KRED;
 };
 
 // Compressed tensor: the octant kx <= L, ky <= M, kz <= N of fN.  Mirrored points pick up the parity
@@ -636,7 +677,7 @@ __global__ __launch_bounds__(512) void k_row2p(const cplx<T>* __restrict__ tw, c
     fft2_step2<T, NH, INV>(u);
   }
   __syncthreads();  // the exchange buffer is reused below
-  if (!INV) {
+  if constexpr (!INV) {
     // X(2 j + par), j = s + R2 k1, staged at [kx][line]
     if (s < R2) {
 #pragma unroll
@@ -669,6 +710,8 @@ __global__ __launch_bounds__(512) void k_row2p(const cplx<T>* __restrict__ tw, c
     }
     __syncthreads();
     constexpr int NO = (NH * LPW + TH - 1) / TH;
+    constexpr int KR = RedK<IO>::K;
+    double rre[KR > 0 ? KR : 1] = {}, rim[KR > 0 ? KR : 1] = {};
 #pragma unroll
     for (int i0 = 0; i0 < NO; i0 += PB) {
       typename IO::Pre pr[PB];
@@ -685,12 +728,215 @@ __global__ __launch_bounds__(512) void k_row2p(const cplx<T>* __restrict__ tw, c
         const int x = e % NH, ll = e / NH;
         if (i0 + j < NO && e < NH * LPW && g0 + ll < nlines) {
           const cplx<T> y = lds[x * HLP + 2 * ll] + cconj(tw2[x]) * lds[x * HLP + 2 * ll + 1];
-          io.store(g0 + ll, x, rhs, y, pr[j]);
+          if constexpr (KR > 0)
+            io.store(g0 + ll, x, rhs, y, pr[j], rre, rim);
+          else
+            io.store(g0 + ll, x, rhs, y, pr[j]);
         }
       }
     }
+    // per-workgroup partials of the IO's sums (one right-hand side per launch)
+    if constexpr (KR > 0) block_partials<TH, KR>(rre, rim, io.partial + (int64_t)blockIdx.x * KR);
   }
 }
+// Preconditioner output -> matvec input in one pass over the lines g = (y, z, c) of length NH = L: the
+// inverse x transform of the preconditioner's volume U (pitch NH) scaled by `scale` and scattered into the
+// compressed vector y (the preconditioned vector, as k_row2 with IoScatter), and the same values, zero off
+// the domain, zero-padded to 2 NH and transformed forward into the slab-major matvec volume S (as k_row2p
+// with IoMvFxSlab); one partial of ||y||^2 per workgroup.  Phase A: group hl < LPW transforms line g0 + hl;
+// phase B: group (line, parity) as in k_row2p.
+template <typename T, int NH>
+__global__ __launch_bounds__(512) void k_row_pcmv(const cplx<T>* __restrict__ tw, const cplx<T>* __restrict__ tw2,
+                                                  const cplx<T>* __restrict__ U, const int* __restrict__ map3, T scale,
+                                                  cplx<T>* __restrict__ y, cplx<T>* __restrict__ S, int64_t SL,
+                                                  int64_t nlines, c128* __restrict__ partial) {
+  using C = RowPCfg<T, NH>;
+  constexpr int R1 = C::R1, R2 = C::R2, TPL = C::TPL, LPW = C::LPW, LP = C::LP, HLP = C::HLP, TH = C::THREADS;
+  constexpr int N2 = 2 * NH;
+  __shared__ cplx<T> lds[C::LDS];
+  const int t = threadIdx.x;
+  const int hl = t / TPL, s = t - hl * TPL;
+  const int64_t g0 = (int64_t)blockIdx.x * LPW;
+  double acc[1] = {0.0}, acci[1] = {0.0};
+  const bool la = hl < LPW;
+  const bool lva = la && g0 + hl < nlines;
+  const int64_t ga = lva ? g0 + hl : 0;
+  if (la && s < R1) {
+    cplx<T> v[R2];
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) {
+      const int n = s + R1 * n2;
+      v[n2] = U[ga * NH + n];
+      if (!lva) v[n2] = mk<T>(0, 0);
+    }
+    fft2_step1<T, NH, true>(v, s, tw);
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) lds[(s * R2 + k2) * HLP + hl] = v[k2];
+  }
+  __syncthreads();
+  {
+    cplx<T> u[R1];
+    int mm[R1];
+    if (la && s < R2) {
+#pragma unroll
+      for (int n1 = 0; n1 < R1; ++n1) u[n1] = lds[(n1 * R2 + s) * HLP + hl];
+      fft2_step2<T, NH, true>(u);
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) mm[k1] = lva ? map3[ga * NH + s + R2 * k1] : -1;
+    }
+    __syncthreads();  // the exchange area is reused by the staging below
+    if (la && s < R2) {
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) {
+        const int x = s + R2 * k1;
+        cplx<T> val = scale * u[k1];
+        if (mm[k1] >= 0) {
+          y[mm[k1]] = val;
+          acc[0] += (double)val.x * val.x + (double)val.y * val.y;
+        } else {
+          val = mk<T>(0, 0);
+        }
+        lds[x * LP + hl] = val;
+      }
+    }
+  }
+  __syncthreads();
+  const int l = hl >> 1, par = hl & 1;
+  cplx<T> v[R2];
+  if (s < R1) {
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) {
+      const int n = s + R1 * n2;
+      const cplx<T> xv = lds[n * LP + l];
+      v[n2] = par ? xv * tw2[n] : xv;
+    }
+  }
+  __syncthreads();  // the staging is read before the exchange overwrites it
+  if (s < R1) {
+    fft2_step1<T, NH, false>(v, s, tw);
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) lds[(s * R2 + k2) * HLP + hl] = v[k2];
+  }
+  __syncthreads();
+  cplx<T> u[R1];
+  if (s < R2) {
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1) u[n1] = lds[(n1 * R2 + s) * HLP + hl];
+    fft2_step2<T, NH, false>(u);
+  }
+  __syncthreads();
+  if (s < R2) {
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) lds[(2 * (s + R2 * k1) + par) * LP + l] = u[k1];
+  }
+  __syncthreads();
+  constexpr int NO = (N2 * LPW + TH - 1) / TH;
+#pragma unroll
+  for (int i = 0; i < NO; ++i) {
+    const int e = t + i * TH;
+    const int ll = e % LPW, k = e / LPW;
+    if (e < N2 * LPW && g0 + ll < nlines) S[k * SL + g0 + ll] = lds[k * LP + ll];
+  }
+  block_partials<TH, 1>(acc, acci, partial + blockIdx.x);
+}
+
+// The preconditioner's first pass (gather + x transform of length N = L into the volume U, pitch L, as
+// k_row2 with IoGather) on a vector formed on the way from the Krylov vectors at each unknown, with the
+// scalars of the stage read from the device as the separate update kernels read them:
+//   MODE 0 (start of the iteration): p = r + beta (p0 - omega v0) (p = r at the first), written to w0;
+//          a0 = r, a1 = p0, a2 = v0; nothing is written after a breakdown
+//   MODE 1 (half step): xh = x + alpha ph -> w0, s = r - alpha v -> w1 (transformed); a0 = x, a1 = ph, a2 = r,
+//          a3 = v; partials ||s||^2 and ||xh||^2 per workgroup (nothing written once the flag is set)
+template <typename T>
+struct BiRowArgs {
+  const int* map3;
+  cplx<T>* U;
+  const cplx<T>* a0;
+  const cplx<T>* a1;
+  const cplx<T>* a2;
+  const cplx<T>* a3;
+  cplx<T>* w0;
+  cplx<T>* w1;
+  const BiScal* sc;
+  c128* partial;
+};
+
+template <typename T, int N, int MODE>
+__global__ __launch_bounds__(256) void k_row_bi(const cplx<T>* __restrict__ tw, BiRowArgs<T> a, int64_t nlines) {
+  using C = RowCfg<T, N>;
+  constexpr int R1 = C::R1, R2 = C::R2, TPL = C::TPL, LPW = C::LPW, LP = LPW + 1;
+  __shared__ cplx<T> lds[N * LP];
+  const int t = threadIdx.x;
+  const int ll = t / TPL, s = t - ll * TPL;
+  const int64_t g = (int64_t)blockIdx.x * LPW + ll;
+  const bool live = g < nlines;
+  const BiScal& S = *a.sc;
+  bool ok = S.flag == 0;
+  bool first = false;
+  cplx<T> c0 = mk<T>(0, 0), c1 = mk<T>(0, 0);
+  if (MODE == 0) {
+    c128 rho, beta;
+    if (ok && bi_rho_beta(S, rho, beta)) ok = false;
+    first = S.first != 0;
+    c0 = ccast<T>(beta);
+    c1 = ccast<T>(S.omega);
+  } else {
+    c0 = ccast<T>(S.alpha);
+  }
+  double re[2] = {0.0, 0.0}, im[2] = {0.0, 0.0};
+  cplx<T> v[R2];
+  if (s < R1) {
+    int mm[R2];
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) mm[n2] = live ? a.map3[g * N + s + R1 * n2] : -1;
+    cplx<T> q0[R2], q1[R2], q2[R2], q3[R2];
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) {
+      const int m = mm[n2] >= 0 ? mm[n2] : 0;
+      q0[n2] = a.a0[m];
+      q1[n2] = a.a1[m];
+      q2[n2] = a.a2[m];
+      if (MODE == 1) q3[n2] = a.a3[m];
+    }
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) {
+      const bool on = ok && mm[n2] >= 0;
+      cplx<T> val;
+      if (MODE == 0) {
+        val = first ? q0[n2] : q0[n2] + c0 * (q1[n2] - c1 * q2[n2]);
+        if (on) a.w0[mm[n2]] = val;
+      } else {
+        const cplx<T> xh = q0[n2] + c0 * q1[n2];
+        val = q2[n2] - c0 * q3[n2];
+        if (on) {
+          a.w0[mm[n2]] = xh;
+          a.w1[mm[n2]] = val;
+          re[0] += (double)val.x * val.x + (double)val.y * val.y;
+          re[1] += (double)xh.x * xh.x + (double)xh.y * xh.y;
+        }
+      }
+      v[n2] = on ? val : mk<T>(0, 0);
+    }
+    fft2_step1<T, N, false>(v, s, tw);
+  }
+  if (s < R1) {
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) lds[(s * R2 + k2) * LP + ll] = v[k2];
+  }
+  __syncthreads();
+  if (s < R2) {
+    cplx<T> u[R1];
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1) u[n1] = lds[(n1 * R2 + s) * LP + ll];
+    fft2_step2<T, N, false>(u);
+    if (live) {
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) a.U[g * N + s + R2 * k1] = u[k1];
+    }
+  }
+  if (MODE == 1) block_partials<C::THREADS, 2>(re, im, a.partial + (int64_t)blockIdx.x * 2);
+}
+
 template <typename T, int N>
 struct ColCfg {
   static constexpr int R1 = Split<N>::R1, R2 = Split<N>::R2, TPL = cmax(R1, R2);
@@ -2002,6 +2248,203 @@ int matvec_variant(const vs_op* op) {
 }
 template int matvec_variant<float>(const vs_op*);
 template int matvec_variant<double>(const vs_op*);
+
+// ------------------------------------------------------------------------------------------------
+// fused BiCGStab halves: FFT operator with a slab-major matvec and a circulant preconditioner on the same
+// domain, one right-hand side.  Per half step: the preconditioner's first x pass with the Krylov update
+// (k_row_bi), its middle stages (pc_mid_dev), its last x pass merged with the matvec's first (k_row_pcmv),
+// the slab pass, and the matvec's last pass with the half step's inner products (k_row2p with IoDot).
+// ------------------------------------------------------------------------------------------------
+template <typename T, int NN>
+constexpr bool bi_len_ok() {
+  return NN >= 8 && RowPCfg<T, NN>::LDS * sizeof(cplx<T>) + 256 <= 65536 && RowPCfg<T, NN>::THREADS <= 512 &&
+         RowCfg<T, NN>::THREADS <= 256;
+}
+
+template <typename T>
+static bool bi_len_compiled(int n) {
+  switch (n) {
+#define VS_BI_LEN(NN) \
+  case NN: return bi_len_ok<T, NN>();
+    VS_FFT2_SIZES(VS_BI_LEN)
+#undef VS_BI_LEN
+    default: return false;
+  }
+}
+
+static bool bi_fused_env() {
+  static const bool on = [] {
+    const char* e = std::getenv("VS_BIFUSED");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+
+template <typename T>
+bool bi_fused_available(vs_op* op, vs_pc* pc) {
+  if (!bi_fused_env() || !op || !pc || op->dense || pc->mop) return false;
+  if (op->dom_id == 0 || pc->dom_id != op->dom_id) return false;
+  if (pc->L != op->L || pc->M != op->M || pc->N != op->N || pc->nunk != op->nunk) return false;
+  const int prec = sizeof(T) == 8 ? VS_FP64 : VS_FP32;
+  if (!(op->precisions & prec) || !(pc->precisions & prec)) return false;
+  if (!(sizeof(T) == 8 ? pc->inv64.p : pc->inv32.p)) return false;
+  if (choose_variant<T>(op) == 0) return false;
+  const char* rp = std::getenv("VS_ROWP");
+  if (rp && rp[0] == '0') return false;
+  return bi_len_compiled<T>(op->L);
+}
+template bool bi_fused_available<float>(vs_op*, vs_pc*);
+template bool bi_fused_available<double>(vs_op*, vs_pc*);
+
+// returns the number of workgroups (partials)
+template <typename T, int MODE>
+static int launch_row_bi(vs_ctx* ctx, const BiRowArgs<T>& a, int L, int64_t lines) {
+  const cplx<T>* tw = ctx->plan(L).dev<T>().tw;
+  switch (L) {
+#define VS_BI_ROW(NN)                                                                                   \
+  case NN:                                                                                              \
+    if constexpr (bi_len_ok<T, NN>()) {                                                                 \
+      using Cf = RowCfg<T, NN>;                                                                         \
+      const int64_t g = (lines + Cf::LPW - 1) / Cf::LPW;                                                \
+      hipLaunchKernelGGL((k_row_bi<T, NN, MODE>), dim3((unsigned)g), dim3(Cf::THREADS), 0, ctx->stream, tw, a, \
+                         lines);                                                                        \
+      VS_HIP(hipGetLastError());                                                                        \
+      return (int)g;                                                                                    \
+    }                                                                                                   \
+    break;
+    VS_FFT2_SIZES(VS_BI_ROW)
+#undef VS_BI_ROW
+    default: break;
+  }
+  throw Error{VS_ERR_INTERNAL, "fused BiCGStab pass for an unsupported line length"};
+}
+
+template <typename T>
+static int launch_row_pcmv(vs_ctx* ctx, const cplx<T>* U, const int* map3, double scale, cplx<T>* y, cplx<T>* S,
+                           int64_t SL, int L, int64_t lines, c128* part) {
+  const cplx<T>* tw = ctx->plan(L).dev<T>().tw;
+  const cplx<T>* tw2 = ctx->plan(2 * L).dev<T>().tw;
+  switch (L) {
+#define VS_BI_PCMV(NN)                                                                                  \
+  case NN:                                                                                              \
+    if constexpr (bi_len_ok<T, NN>()) {                                                                 \
+      using Cf = RowPCfg<T, NN>;                                                                        \
+      const int64_t g = (lines + Cf::LPW - 1) / Cf::LPW;                                                \
+      hipLaunchKernelGGL((k_row_pcmv<T, NN>), dim3((unsigned)g), dim3(Cf::THREADS), 0, ctx->stream, tw, tw2, U, \
+                         map3, (T)scale, y, S, SL, lines, part);                                        \
+      VS_HIP(hipGetLastError());                                                                        \
+      return (int)g;                                                                                    \
+    }                                                                                                   \
+    break;
+    VS_FFT2_SIZES(VS_BI_PCMV)
+#undef VS_BI_PCMV
+    default: break;
+  }
+  throw Error{VS_ERR_INTERNAL, "fused BiCGStab pass for an unsupported line length"};
+}
+
+template <typename T>
+static int row_groups(int L, int64_t lines) {
+  switch (L) {
+#define VS_BI_G(NN) \
+  case NN: return (int)((lines + RowCfg<T, NN>::LPW - 1) / RowCfg<T, NN>::LPW);
+    VS_FFT2_SIZES(VS_BI_G)
+#undef VS_BI_G
+    default: return 0;
+  }
+}
+
+template <typename T>
+static int rowp_groups(int L, int64_t lines) {
+  switch (L) {
+#define VS_BI_G(NN) \
+  case NN: return (int)((lines + RowPCfg<T, NN>::LPW - 1) / RowPCfg<T, NN>::LPW);
+    VS_FFT2_SIZES(VS_BI_G)
+#undef VS_BI_G
+    default: return 0;
+  }
+}
+
+// Y (the preconditioner's volume before its inverse x transform, scale still to apply) -> xin = M^{-1}(...)
+// with ||xin||^2 partials, and y = A xin with the inner products of IoDot<KD> against w
+template <typename T, int KD>
+static void bi_mv_tail(vs_op* op, const cplx<T>* Y, double scale, cplx<T>* xin, c128* part_in, int& n_in, cplx<T>* y,
+                       const cplx<T>* w, c128* part_out, int& n_out, int cap) {
+  vs_ctx* ctx = op->ctx;
+  const int L = op->L, M = op->M, N = op->N;
+  const int64_t lines = (int64_t)M * N * 3, SL = lines, Sr = (int64_t)2 * L * SL;
+  const int variant = choose_variant<T>(op);
+  VS_CHECK(variant > 0, VS_ERR_INTERNAL, "fused BiCGStab half without a slab-major matvec");
+  const int* map3 = op->map3.as<int>();
+  const int g = rowp_groups<T>(L, lines);
+  VS_CHECK(g > 0 && g <= cap, VS_ERR_UNSUPPORTED, "fused BiCGStab: too many partial sums");
+  op->S.ensure(mvp_scratch_bytes<T>(op, variant, 1));
+  cplx<T>* S = op->S.as<cplx<T>>();
+  n_in = launch_row_pcmv<T>(ctx, Y, map3, scale, xin, S, SL, L, lines, part_in);
+  const cplx<T>* fq = (sizeof(T) == 8) ? op->fq64.as<cplx<T>>() : op->fq32.as<cplx<T>>();
+  const cplx<T>* fsl = (sizeof(T) == 8) ? op->fs64.as<cplx<T>>() : op->fs32.as<cplx<T>>();
+  IoMvIx<T> ix0{xin, y, map3, op->nunk, op->V, L, M, 0, coef_dev<T>(op->a), coef_dev<T>(op->at),
+                coef_dev<T>(op->b), S, Sr, (T)(1.0 / (double)op->P), nullptr};
+  bool ok;
+  if (variant == 3) {
+    TensorQ<T> tq{fq, L, M, N, M / 2 + 1, N / 2 + 1};
+    cplx<T>* Se = S + Sr;
+    cplx<T>* So = S + 2 * Sr;
+    launch_yzq<T>(ctx, S, tq, L, M, N, 1, Sr, Se, So);
+    ix0.S = Se;
+    IoDot<T, IoMvIxSlab2<T>, KD> io{IoMvIxSlab2<T>{ix0, SL, So}, w, part_out};
+    ok = run_rows_p<T, true>(ctx, io, L, lines, 1);
+  } else {
+    if (variant == 2) {
+      TensorQ<T> tq{fq, L, M, N, M / 2 + 1, N / 2 + 1};
+      launch_yzq<T>(ctx, S, tq, L, M, N, 1, Sr);
+    } else {
+      TensorS<T> ts{fsl, L, M, N};
+      launch_yz<T>(ctx, S, ts, L, M, N, 1, Sr);
+    }
+    IoDot<T, IoMvIxSlab<T>, KD> io{IoMvIxSlab<T>{ix0, SL}, w, part_out};
+    ok = run_rows_p<T, true>(ctx, io, L, lines, 1);
+  }
+  VS_CHECK(ok, VS_ERR_INTERNAL, "fused BiCGStab: no parity-split row transform for this length");
+  n_out = g;
+}
+
+template <typename T>
+void bi_fused_h1(vs_op* op, vs_pc* pc, const cplx<T>* r, const cplx<T>* p0, const cplx<T>* v0, cplx<T>* p, cplx<T>* ph,
+                 cplx<T>* v, const cplx<T>* rt, const BiScal* sc, BiParts& P) {
+  const int prec = sizeof(T) == 8 ? VS_FP64 : VS_FP32;
+  const int64_t lines = (int64_t)op->M * op->N * 3;
+  cplx<T>* U = static_cast<cplx<T>*>(pc_work_dev(pc, prec));
+  BiRowArgs<T> a{op->map3.as<int>(), U, r, p0, v0, nullptr, p, nullptr, sc, nullptr};
+  launch_row_bi<T, 0>(op->ctx, a, op->L, lines);
+  double scale = 1.0;
+  const cplx<T>* Y = static_cast<const cplx<T>*>(pc_mid_dev(pc, prec, &scale));
+  bi_mv_tail<T, 1>(op, Y, scale, ph, P.ph, P.n_ph, v, rt, P.v, P.n_v, P.cap);
+}
+
+template <typename T>
+void bi_fused_h2(vs_op* op, vs_pc* pc, const cplx<T>* x, const cplx<T>* ph, const cplx<T>* r, const cplx<T>* v,
+                 cplx<T>* xh, cplx<T>* s, cplx<T>* sh, cplx<T>* t, const BiScal* sc, BiParts& P) {
+  const int prec = sizeof(T) == 8 ? VS_FP64 : VS_FP32;
+  const int64_t lines = (int64_t)op->M * op->N * 3;
+  cplx<T>* U = static_cast<cplx<T>*>(pc_work_dev(pc, prec));
+  BiRowArgs<T> a{op->map3.as<int>(), U, x, ph, r, v, xh, s, sc, P.sx};
+  const int gs = row_groups<T>(op->L, lines);
+  VS_CHECK(gs > 0 && gs <= P.cap, VS_ERR_UNSUPPORTED, "fused BiCGStab: too many partial sums");
+  P.n_sx = launch_row_bi<T, 1>(op->ctx, a, op->L, lines);
+  double scale = 1.0;
+  const cplx<T>* Y = static_cast<const cplx<T>*>(pc_mid_dev(pc, prec, &scale));
+  bi_mv_tail<T, 2>(op, Y, scale, sh, P.sh, P.n_sh, t, s, P.t, P.n_t, P.cap);
+}
+
+#define VS_BIF_INST(T)                                                                                          \
+  template void bi_fused_h1<T>(vs_op*, vs_pc*, const cplx<T>*, const cplx<T>*, const cplx<T>*, cplx<T>*, cplx<T>*, \
+                               cplx<T>*, const cplx<T>*, const BiScal*, BiParts&);                              \
+  template void bi_fused_h2<T>(vs_op*, vs_pc*, const cplx<T>*, const cplx<T>*, const cplx<T>*, const cplx<T>*,      \
+                               cplx<T>*, cplx<T>*, cplx<T>*, cplx<T>*, const BiScal*, BiParts&);
+VS_BIF_INST(float)
+VS_BIF_INST(double)
+#undef VS_BIF_INST
 
 // Autotuning: time every available matvec variant in each precision the operator holds (one warm-up and
 // three timed applications on a unit vector, HIP events on the context stream) and keep the fastest.

@@ -1371,9 +1371,14 @@ static int bmm_rows_split() {
   return v;
 }
 
-// blocks_only: run the block-product stage alone on the work volume U (a timing helper)
+// mode: PC_FULL the whole apply x -> y; PC_BLOCKS the block stage alone on the work volume; PC_MID the stages
+// between the first and the last x pass (y FFTs and blocks).  Returns the volume that holds the result of
+// the last stage run.
+enum PcMode : int { PC_FULL = 0, PC_BLOCKS = 1, PC_MID = 2 };
 template <typename T>
-static void pc_apply_impl(vs_pc* pc, const cplx<T>* x, cplx<T>* y, int nrhs, bool blocks_only = false) {
+static cplx<T>* pc_apply_impl(vs_pc* pc, const cplx<T>* x, cplx<T>* y, int nrhs, int mode = PC_FULL) {
+  const bool blocks_only = mode == PC_BLOCKS;
+  const bool first_pass = mode == PC_FULL, y_passes = mode != PC_BLOCKS, last_pass = mode == PC_FULL;
   vs_ctx* ctx = pc->ctx;
   const int prec = sizeof(T) == 8 ? VS_FP64 : VS_FP32;
   const int L = pc->L, M = pc->M, N = pc->N;
@@ -1384,9 +1389,9 @@ static void pc_apply_impl(vs_pc* pc, const cplx<T>* x, cplx<T>* y, int nrhs, boo
   cplx<T>* U = pc->U.as<cplx<T>>();
   const int* map3 = pc->map3.as<int>();
   const int64_t lines = (int64_t)M * N * 3;
-  if (!blocks_only) launch_row_fft_gather(ctx, prec, x, map3, pc->nunk, U, L, lines, nrhs, V3);
+  if (first_pass) launch_row_fft_gather(ctx, prec, x, map3, pc->nunk, U, L, lines, nrhs, V3);
   if (pc->levels == 2) {
-    if (!blocks_only)
+    if (y_passes)
       launch_col_fft(ctx, prec, U, L, L, (int64_t)N * 3, (int64_t)L * M, M, M, M, false, 1.0, nrhs, V3);
     const int n = pc->nb;
     if (pc->mirror) {
@@ -1527,9 +1532,10 @@ static void pc_apply_impl(vs_pc* pc, const cplx<T>* x, cplx<T>* y, int nrhs, boo
 #undef VS_BMV
     VS_HIP(hipGetLastError());
     }
-    if (blocks_only) return;
+    if (blocks_only) return U;
     launch_col_fft(ctx, prec, U, L, L, (int64_t)N * 3, (int64_t)L * M, M, M, M, true, 1.0, nrhs, V3);
-    launch_row_ifft_scatter(ctx, prec, U, map3, pc->nunk, y, L, lines, 1.0 / ((double)L * M), nrhs, V3);
+    if (last_pass) launch_row_ifft_scatter(ctx, prec, U, map3, pc->nunk, y, L, lines, 1.0 / ((double)L * M), nrhs, V3);
+    return U;
   } else {
     pc->U2.ensure(sizeof(cplx<T>) * V3 * nrhs);
     cplx<T>* U2 = pc->U2.as<cplx<T>>();
@@ -1539,8 +1545,8 @@ static void pc_apply_impl(vs_pc* pc, const cplx<T>* x, cplx<T>* y, int nrhs, boo
     hipLaunchKernelGGL((k_blockmv1<T>), dim3((unsigned)((n + rows - 1) / rows), (unsigned)L, (unsigned)nrhs), dim3(256),
                        lds, ctx->stream, B, U, U2, L, n, rows, V3);
     VS_HIP(hipGetLastError());
-    if (blocks_only) return;
-    launch_row_ifft_scatter(ctx, prec, U2, map3, pc->nunk, y, L, lines, 1.0 / (double)L, nrhs, V3);
+    if (last_pass) launch_row_ifft_scatter(ctx, prec, U2, map3, pc->nunk, y, L, lines, 1.0 / (double)L, nrhs, V3);
+    return U2;
   }
 }
 
@@ -1554,11 +1560,25 @@ void pc_blocks_dev(vs_pc* pc, int precision, int nrhs) {
     VS_HIP(hipMemsetAsync(pc->U.p, 0, need, pc->ctx->stream));
   }
   if (precision == VS_FP64)
-    pc_apply_impl<double>(pc, nullptr, nullptr, nrhs, true);
+    pc_apply_impl<double>(pc, nullptr, nullptr, nrhs, PC_BLOCKS);
   else if (precision == VS_FP32)
-    pc_apply_impl<float>(pc, nullptr, nullptr, nrhs, true);
+    pc_apply_impl<float>(pc, nullptr, nullptr, nrhs, PC_BLOCKS);
   else
     throw Error{VS_ERR_ARG, "precision must be VS_FP64 or VS_FP32"};
+}
+
+void* pc_work_dev(vs_pc* pc, int precision) {
+  const size_t es = precision == VS_FP64 ? sizeof(c128) : sizeof(c64);
+  pc->U.ensure(es * 3 * (size_t)pc->V);
+  return pc->U.p;
+}
+
+void* pc_mid_dev(vs_pc* pc, int precision, double* scale) {
+  VS_CHECK(!pc->mop, VS_ERR_UNSUPPORTED, "an operator preconditioner has no circulant stages");
+  *scale = pc->levels == 2 ? 1.0 / ((double)pc->L * pc->M) : 1.0 / (double)pc->L;
+  if (precision == VS_FP64) return pc_apply_impl<double>(pc, nullptr, nullptr, 1, PC_MID);
+  if (precision == VS_FP32) return pc_apply_impl<float>(pc, nullptr, nullptr, 1, PC_MID);
+  throw Error{VS_ERR_ARG, "precision must be VS_FP64 or VS_FP32"};
 }
 
 void mvp_dev(vs_op* op, const void* x, void* y, int transp, int precision, int nrhs);

@@ -8,6 +8,7 @@
 //  gmres    : MATLAB gmres(A,b,restart,tol,maxit,M) -- left preconditioned, restarted; Arnoldi by
 //             classical Gram-Schmidt applied twice, Givens rotations on the host
 //  pgmres   : pgmres.m:70-180 / pgmres_iter.m:22-150 -- one-pass CGS, L1 or R1 preconditioner
+#include "bi_device.hpp"
 #include "handles.hpp"
 #include "solver.hpp"
 
@@ -433,24 +434,8 @@ void Engine<T>::combine(const cplx<T>* X, int64_t ldx, int q, const c128* P_host
 // device-resident BiCGStab scalars: every scalar of an iteration is formed on the device, so a whole
 // iteration runs without host round trips (the host reads a copy once per iteration, behind the next one)
 // ------------------------------------------------------------------------------------------------
-__device__ __forceinline__ bool cz(c128 a) { return a.x == 0.0 && a.y == 0.0; }
-__device__ __forceinline__ bool cinf(c128 a) { return isinf(a.x) || isinf(a.y); }
-__device__ __forceinline__ bool cfin(c128 a) { return isfinite(a.x) && isfinite(a.y); }
-
 // BI_RHO: the batched kernels form rho in a stage of its own
 enum BiStage : int { BI_INIT = 0, BI_ALPHA = 1, BI_HALF = 2, BI_OMEGA = 3, BI_FULL = 4, BI_RHO = 5 };
-
-// rho of this iteration (conj(rt) . r, formed when r was) and beta; brk = 1 / 2 on a breakdown
-__device__ __forceinline__ int bi_rho_beta(const BiScal& S, c128& rho, c128& beta) {
-  rho = S.rho_n;
-  beta = mk<double>(0, 0);
-  if (cz(rho) || cinf(rho)) return 1;
-  if (!S.first) {
-    beta = cdiv(rho, S.rho1) * cdiv(S.alpha, S.omega);
-    if (cz(beta) || !cfin(beta)) return 2;
-  }
-  return 0;
-}
 
 // MATLAB bicgstab's scalar logic of one stage, on the reduced sums tot[]
 __device__ void bi_logic(int stage, const c128* tot, BiScal& S) {
@@ -567,15 +552,18 @@ __global__ __launch_bounds__(RED_THREADS) void k_bi_p(const cplx<T>* r, const cp
 
 // y = x0 + c x1 ; z = x2 - c x3 with c = alpha (half step) or omega (full step); partials of ||z||^2 and,
 // for the full step, of conj(rt) . z (the next iteration's rho): partial[blk * K + k], K = 1 + full
+// partial_y (nullable): partials of ||y||^2 as well (the fused solve's ||x|| for the next alpha stage)
 template <typename T>
 __global__ __launch_bounds__(RED_THREADS) void k_bi_axpy2(const cplx<T>* x0, const cplx<T>* x1, const cplx<T>* x2,
                                                           const cplx<T>* x3, cplx<T>* y, cplx<T>* z, const cplx<T>* rt,
-                                                          int64_t n, const BiScal* sc, int full, c128* partial) {
-  double re[2] = {0.0, 0.0}, im[2] = {0.0, 0.0};
+                                                          int64_t n, const BiScal* sc, int full, c128* partial,
+                                                          c128* partial_y) {
+  double re[3] = {0.0, 0.0, 0.0}, im[3] = {0.0, 0.0, 0.0};
   if (!sc->flag) {
     const cplx<T> c = ccast<T>(full ? sc->omega : sc->alpha);
     for (int64_t i = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * RED_THREADS) {
-      y[i] = x0[i] + c * x1[i];
+      const cplx<T> yv = x0[i] + c * x1[i];
+      y[i] = yv;
       const cplx<T> o = x2[i] - c * x3[i];
       z[i] = o;
       re[0] += (double)o.x * o.x + (double)o.y * o.y;
@@ -584,14 +572,48 @@ __global__ __launch_bounds__(RED_THREADS) void k_bi_axpy2(const cplx<T>* x0, con
         re[1] += (double)a.x * o.x + (double)a.y * o.y;
         im[1] += (double)a.x * o.y - (double)a.y * o.x;
       }
+      if (partial_y) re[2] += (double)yv.x * yv.x + (double)yv.y * yv.y;
     }
   }
-  c128 tot[2];
-  block_sum<2>(re, im, tot);
+  c128 tot[3];
+  block_sum<3>(re, im, tot);
   if (threadIdx.x == 0) {
     partial[blockIdx.x * (1 + full)] = tot[0];
     if (full) partial[blockIdx.x * 2 + 1] = tot[1];
+    if (partial_y) partial_y[blockIdx.x] = tot[2];
   }
+}
+
+// Reduced sums from several partial buffers (one wavefront per sum: lane l adds partials l, l + 64, ... in
+// order, then a fixed shuffle tree), then the scalar logic of `stage`; half >= 0 names the sum that is the
+// half step's ||s||^2, applied first (folded into the omega stage).
+struct BiSrc {
+  const c128* p[6];
+  int nblk[6], K[6], k[6];
+  int n, half;
+};
+__global__ __launch_bounds__(384) void k_bi_scalar_src(BiSrc src, int stage, BiScal* sc) {
+  __shared__ c128 tot[6];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (wave < src.n) {
+    const c128* pp = src.p[wave];
+    const int KK = src.K[wave], kk = src.k[wave], nb = src.nblk[wave];
+    double a = 0, b = 0;
+    for (int i = lane; i < nb; i += 64) {
+      const c128 v = pp[i * KK + kk];
+      a += v.x;
+      b += v.y;
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+      a += __shfl_xor(a, off);
+      b += __shfl_xor(b, off);
+    }
+    if (lane == 0) tot[wave] = mk<double>(a, b);
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  if (src.half >= 0) bi_logic(BI_HALF, tot + src.half, *sc);
+  bi_logic(stage, tot, *sc);
 }
 
 // partial buffers of the stages: the half step's ||s|| partials live apart so that they can be reduced
@@ -634,7 +656,7 @@ void bi_stage_alpha_half(Engine<T>& E, const cplx<T>* rt, const cplx<T>* v, cons
   hipLaunchKernelGGL((k_bi_dots<T, 3>), dim3(E.grid), dim3(RED_THREADS), 0, ctx->stream, a, E.n, part_main(ctx));
   launch_scalar(ctx, part_main(ctx), E.grid, 3, nullptr, (int)BI_ALPHA, sc);
   hipLaunchKernelGGL((k_bi_axpy2<T>), dim3(E.grid), dim3(RED_THREADS), 0, ctx->stream, x, ph, r, v, xh, s,
-                     (const cplx<T>*)nullptr, E.n, sc, 0, part_half(ctx));
+                     (const cplx<T>*)nullptr, E.n, sc, 0, part_half(ctx), (c128*)nullptr);
   if (finish_half) launch_scalar(ctx, part_half(ctx), E.grid, 1, nullptr, (int)BI_HALF, sc);
   VS_HIP(hipGetLastError());
 }
@@ -655,7 +677,71 @@ void bi_stage_omega_full(Engine<T>& E, const cplx<T>* t, const cplx<T>* s, const
   hipLaunchKernelGGL((k_bi_dots<T, 4>), dim3(E.grid), dim3(RED_THREADS), 0, ctx->stream, a, E.n, part_main(ctx));
   launch_scalar(ctx, part_main(ctx), E.grid, 4, fold_half ? part_half(ctx) : nullptr, (int)BI_OMEGA, sc);
   hipLaunchKernelGGL((k_bi_axpy2<T>), dim3(E.grid), dim3(RED_THREADS), 0, ctx->stream, xh, sh, s, t, x, r, rt, E.n,
-                     sc, 1, part_main(ctx));
+                     sc, 1, part_main(ctx), (c128*)nullptr);
+  launch_scalar(ctx, part_main(ctx), E.grid, 2, nullptr, (int)BI_FULL, sc);
+  VS_HIP(hipGetLastError());
+}
+
+// The fused halves' partial buffers: six regions of the context's reduction area past the stages' own
+// (each region holds up to `cap` partials of two values).
+template <typename T>
+void bi_parts_take(Engine<T>& E, BiParts& P) {
+  constexpr int64_t base = 16384, cap = 32768;
+  static_assert(base + 6 * 2 * cap <= 1024 * 512, "fused partials exceed the reduction area");
+  c128* r = part_main(E.ctx);
+  P.ph = r + base;
+  P.v = P.ph + 2 * cap;
+  P.sx = P.v + 2 * cap;
+  P.sh = P.sx + 2 * cap;
+  P.t = P.sh + 2 * cap;
+  P.x = P.t + 2 * cap;
+  P.cap = (int)cap;
+}
+
+template <typename T>
+void bi_stage_xnorm(Engine<T>& E, const cplx<T>* x, BiParts& P) {
+  DotArgs<T> a{};
+  a.a[0] = x;
+  a.b[0] = x;
+  hipLaunchKernelGGL((k_bi_dots<T, 1>), dim3(E.grid), dim3(RED_THREADS), 0, E.ctx->stream, a, E.n, P.x);
+  P.n_x = E.grid;
+  VS_HIP(hipGetLastError());
+}
+
+static void src_add(BiSrc& s, const c128* p, int nblk, int K, int k) {
+  s.p[s.n] = p;
+  s.nblk[s.n] = nblk;
+  s.K[s.n] = K;
+  s.k[s.n] = k;
+  ++s.n;
+}
+
+template <typename T>
+void bi_stage_alpha_fused(Engine<T>& E, BiScal* sc, const BiParts& P) {
+  BiSrc s{};
+  s.half = -1;
+  src_add(s, P.v, P.n_v, 1, 0);
+  src_add(s, P.ph, P.n_ph, 1, 0);
+  src_add(s, P.x, P.n_x, 1, 0);
+  hipLaunchKernelGGL(k_bi_scalar_src, dim3(1), dim3(64 * s.n), 0, E.ctx->stream, s, (int)BI_ALPHA, sc);
+  VS_HIP(hipGetLastError());
+}
+
+template <typename T>
+void bi_stage_omega_full_fused(Engine<T>& E, const cplx<T>* t, const cplx<T>* s, const cplx<T>* sh, const cplx<T>* xh,
+                               const cplx<T>* rt, cplx<T>* x, cplx<T>* r, BiScal* sc, BiParts& P) {
+  BiSrc q{};
+  src_add(q, P.t, P.n_t, 2, 0);    // ||t||^2
+  src_add(q, P.t, P.n_t, 2, 1);    // conj(t) . s
+  src_add(q, P.sh, P.n_sh, 1, 0);  // ||sh||^2
+  src_add(q, P.sx, P.n_sx, 2, 1);  // ||xh||^2
+  src_add(q, P.sx, P.n_sx, 2, 0);  // ||s||^2 of the half step
+  q.half = 4;
+  vs_ctx* ctx = E.ctx;
+  hipLaunchKernelGGL(k_bi_scalar_src, dim3(1), dim3(64 * q.n), 0, ctx->stream, q, (int)BI_OMEGA, sc);
+  hipLaunchKernelGGL((k_bi_axpy2<T>), dim3(E.grid), dim3(RED_THREADS), 0, ctx->stream, xh, sh, s, t, x, r, rt, E.n,
+                     sc, 1, part_main(ctx), P.x);
+  P.n_x = E.grid;
   launch_scalar(ctx, part_main(ctx), E.grid, 2, nullptr, (int)BI_FULL, sc);
   VS_HIP(hipGetLastError());
 }
@@ -666,7 +752,12 @@ void bi_stage_omega_full(Engine<T>& E, const cplx<T>* t, const cplx<T>* s, const
   template void bi_stage_alpha_half<T>(Engine<T>&, const cplx<T>*, const cplx<T>*, const cplx<T>*, const cplx<T>*, \
                                        const cplx<T>*, cplx<T>*, cplx<T>*, BiScal*, bool);                       \
   template void bi_stage_omega_full<T>(Engine<T>&, const cplx<T>*, const cplx<T>*, const cplx<T>*, const cplx<T>*, \
-                                       const cplx<T>*, cplx<T>*, cplx<T>*, BiScal*, bool);
+                                       const cplx<T>*, cplx<T>*, cplx<T>*, BiScal*, bool);                       \
+  template void bi_parts_take<T>(Engine<T>&, BiParts&);                                                           \
+  template void bi_stage_xnorm<T>(Engine<T>&, const cplx<T>*, BiParts&);                                          \
+  template void bi_stage_alpha_fused<T>(Engine<T>&, BiScal*, const BiParts&);                                     \
+  template void bi_stage_omega_full_fused<T>(Engine<T>&, const cplx<T>*, const cplx<T>*, const cplx<T>*,          \
+                                             const cplx<T>*, const cplx<T>*, cplx<T>*, cplx<T>*, BiScal*, BiParts&);
 VS_BI_INST(float)
 VS_BI_INST(double)
 #undef VS_BI_INST

@@ -61,6 +61,32 @@ void bi_stage_omega_full(Engine<T>& E, const cplx<T>* t, const cplx<T>* s, const
 template <typename T>
 void bi_stage_init(Engine<T>& E, const cplx<T>* rt, const cplx<T>* r, BiScal* sc);
 
+// Per-workgroup FP64 partial sums written by the fused BiCGStab halves (device buffers carved from the
+// context's reduction area; each count is set by the launch that writes the buffer).
+struct BiParts {
+  c128* ph = nullptr;  // ||ph||^2 (first half, the preconditioner's last pass)
+  c128* v = nullptr;   // conj(rt) . v (first half, the matvec's last pass)
+  c128* sx = nullptr;  // ||s||^2 and ||xh||^2, interleaved (second half, the preconditioner's first pass)
+  c128* sh = nullptr;  // ||sh||^2 (second half, the preconditioner's last pass)
+  c128* t = nullptr;   // ||t||^2 and conj(t) . s, interleaved (second half, the matvec's last pass)
+  c128* x = nullptr;   // ||x||^2 (the full step's update, or the start of the solve)
+  int n_ph = 0, n_v = 0, n_sx = 0, n_sh = 0, n_t = 0, n_x = 0;
+  int cap = 0;         // capacity of each buffer, in partials per value
+};
+template <typename T>
+void bi_parts_take(Engine<T>& E, BiParts& P);
+// ||x||^2 partials for the first fused alpha stage
+template <typename T>
+void bi_stage_xnorm(Engine<T>& E, const cplx<T>* x, BiParts& P);
+// alpha from the partials of the fused first half (conj(rt) . v, ||ph||^2, ||x||^2)
+template <typename T>
+void bi_stage_alpha_fused(Engine<T>& E, BiScal* sc, const BiParts& P);
+// omega (with the half step's ||s|| folded in) from the partials of the fused second half, then the full
+// step x = xh + omega sh, r = s - omega t with ||r||, conj(rt) . r and ||x||^2 partials, and its scalars
+template <typename T>
+void bi_stage_omega_full_fused(Engine<T>& E, const cplx<T>* t, const cplx<T>* s, const cplx<T>* sh, const cplx<T>* xh,
+                               const cplx<T>* rt, cplx<T>* x, cplx<T>* r, BiScal* sc, BiParts& P);
+
 // batched forms (nrhs systems, vectors stored consecutively, one BiScal per system)
 template <typename T>
 void bib_stage_rho(Engine<T>& E, int nrhs, const cplx<T>* rt, const cplx<T>* r, const cplx<T>* v, cplx<T>* p,
@@ -92,6 +118,20 @@ struct LinOp {
   std::function<void(const cplx<T>*, cplx<T>*, int)> Mb;
   // batched matvec with a per-right-hand-side diagonal (device array, one complex double per vector)
   std::function<void(const cplx<T>*, cplx<T>*, int, const c128*)> Abd;
+  // Fused halves of one BiCGStab iteration (one right-hand side), set when the operator is an FFT operator
+  // with a slab-major matvec and the preconditioner a circulant one on the same domain; they produce the
+  // same vectors as the separate stages, with the Krylov updates and inner products formed inside the
+  // FFT passes.
+  //   F1(r, p0, v0, p, ph, v, rt, sc, P): p = r + beta (p0 - omega v0) (p = r at the first iteration),
+  //       ph = M^{-1} p, v = A ph; partials P.ph, P.v
+  std::function<void(const cplx<T>*, const cplx<T>*, const cplx<T>*, cplx<T>*, cplx<T>*, cplx<T>*, const cplx<T>*,
+                     const BiScal*, BiParts&)>
+      F1;
+  //   F2(x, ph, r, v, xh, s, sh, t, sc, P): xh = x + alpha ph, s = r - alpha v, sh = M^{-1} s, t = A sh;
+  //       partials P.sx, P.sh, P.t
+  std::function<void(const cplx<T>*, const cplx<T>*, const cplx<T>*, const cplx<T>*, cplx<T>*, cplx<T>*, cplx<T>*,
+                     cplx<T>*, const BiScal*, BiParts&)>
+      F2;
 };
 
 struct BatchResult {
@@ -161,5 +201,21 @@ void mvp_dev_scratch(vs_op* op, const void* x, void* y, int transp, int precisio
                      size_t scratch_bytes);
 void pc_apply_dev(vs_pc* pc, const void* x, void* y, int precision, int nrhs);
 void pc_blocks_dev(vs_pc* pc, int precision, int nrhs);
+// The preconditioner's stages between its first and last x pass, on its work volume (2-level: y FFT, block
+// product, inverse y FFT; 1-level: block product): returns the volume holding the result and, in *scale,
+// the scale its inverse x transform still applies.
+void* pc_mid_dev(vs_pc* pc, int precision, double* scale);
+// the preconditioner's work volume for one right-hand side (the first x pass writes it)
+void* pc_work_dev(vs_pc* pc, int precision);
+
+// fused BiCGStab halves of an FFT operator and a circulant preconditioner (see LinOp::F1 / F2)
+template <typename T>
+bool bi_fused_available(vs_op* op, vs_pc* pc);
+template <typename T>
+void bi_fused_h1(vs_op* op, vs_pc* pc, const cplx<T>* r, const cplx<T>* p0, const cplx<T>* v0, cplx<T>* p, cplx<T>* ph,
+                 cplx<T>* v, const cplx<T>* rt, const BiScal* sc, BiParts& P);
+template <typename T>
+void bi_fused_h2(vs_op* op, vs_pc* pc, const cplx<T>* x, const cplx<T>* ph, const cplx<T>* r, const cplx<T>* v,
+                 cplx<T>* xh, cplx<T>* s, cplx<T>* sh, cplx<T>* t, const BiScal* sc, BiParts& P);
 
 }  // namespace vs
