@@ -107,7 +107,7 @@ SolveResult solve_bicgstab(Engine<T>& E, const LinOp<T>& op, const cplx<T>* b, c
   auto XH = [&](int i) { return W[10 + (i & 1)]; };
   auto S = [&](int i) { return W[12 + (i & 1)]; };
   int flag = 1;
-  E.copy(x, xmin);
+  if (verify) E.copy(x, xmin);  // the minimum-residual iterate is kept only by verified solves
   double imin = 0;
   const double tolb = tol * n2b;
   double normr;
@@ -139,7 +139,6 @@ SolveResult solve_bicgstab(Engine<T>& E, const LinOp<T>& op, const cplx<T>* b, c
 
   BiScal* sc = static_cast<BiScal*>(ctx->workspace(vs_ctx::WS_KRYLOV_AUX, sizeof(BiScal)));
   BiScal* hs = reinterpret_cast<BiScal*>(ctx->pinned);  // two read-back slots (iteration parity)
-  BiScal* up = reinterpret_cast<BiScal*>(ctx->pinned_up);  // [0] initial scalars, [1] restore staging
   BiScal h{};
   h.rho = C(1);
   h.rho1 = C(1);
@@ -148,8 +147,7 @@ SolveResult solve_bicgstab(Engine<T>& E, const LinOp<T>& op, const cplx<T>* b, c
   h.beta = C(0);
   h.first = 1;
   h.has_pc = op.M ? 1 : 0;
-  up[0] = h;
-  VS_HIP(hipMemcpyAsync(sc, up, sizeof(BiScal), hipMemcpyHostToDevice, ctx->stream));
+  bi_set_scalars(ctx, sc, h);
   bi_stage_init<T>(E, rt, r, sc);
   EventPair ev;
   // Fused halves (operator and preconditioner permitting) on the fast path only: a first half that must
@@ -198,15 +196,12 @@ SolveResult solve_bicgstab(Engine<T>& E, const LinOp<T>& op, const cplx<T>* b, c
   bool spec = false;  // the first half step of iteration ii + 1 is in flight
   auto cancel_spec = [&] {
     if (!spec) return;
-    up[1] = h;  // the previous restore has completed: every wait on the stream since came after it
-    VS_HIP(hipMemcpyAsync(sc, up + 1, sizeof(BiScal), hipMemcpyHostToDevice, ctx->stream));
+    bi_set_scalars(ctx, sc, h);
     spec = false;
   };
   auto clear_flag = [&] {
     // a breakdown seen only by a speculative second half step must not block its recomputation
-    VS_HIP(hipStreamSynchronize(ctx->stream));
-    *reinterpret_cast<int*>(up) = 0;
-    VS_HIP(hipMemcpyAsync(&sc->flag, up, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    bi_clear_flag(ctx, sc);
   };
 
   for (ii = 1; ii <= maxit; ++ii) {

@@ -285,15 +285,21 @@ struct IoDot {
   const cplx<T>* w;
   c128* partial;
   static constexpr int KRED = KD;
-  using Pre = typename Base::Pre;
+  struct Pre {
+    typename Base::Pre b;
+    cplx<T> w;
+  };
   __device__ __forceinline__ cplx<T> load(int64_t g, int kx, int rhs) const { return io.load(g, kx, rhs); }
-  __device__ __forceinline__ Pre pre(int64_t g, int xp, int rhs) const { return io.pre(g, xp, rhs); }
+  __device__ __forceinline__ Pre pre(int64_t g, int xp, int rhs) const {
+    const typename Base::Pre b = io.pre(g, xp, rhs);
+    return {b, w[rhs * io.out.nunk + (b.m >= 0 ? b.m : 0)]};
+  }
   __device__ __forceinline__ void store(int64_t g, int xp, int rhs, cplx<T> v, const Pre& p, double (&re)[KD],
                                         double (&im)[KD]) const {
-    if (p.m < 0) return;
-    const cplx<T> o = io.out.value(g, xp, rhs, v, p);
-    io.out.y[rhs * io.out.nunk + p.m] = o;
-    const cplx<T> c = w[rhs * io.out.nunk + p.m];
+    if (p.b.m < 0) return;
+    const cplx<T> o = io.out.value(g, xp, rhs, v, p.b);
+    io.out.y[rhs * io.out.nunk + p.b.m] = o;
+    const cplx<T> c = p.w;
     if (KD == 1) {
       re[0] += (double)c.x * o.x + (double)c.y * o.y;
       im[0] += (double)c.x * o.y - (double)c.y * o.x;
@@ -486,12 +492,22 @@ __global__ void k_compress_tensor_q(const c128* __restrict__ fN, cplx<To>* __res
 template <typename T> constexpr int lds_budget() { return sizeof(T) == 8 ? 2048 : 4096; }
 constexpr int pow2_floor(int x) { return x >= 2 ? 2 * pow2_floor(x / 2) : 1; }
 
+// Step-1 -> step-2 exchange of a line transform, laid out [line][k2][n1] with an odd pitch XP >= R1: step 1
+// (thread n1) writes consecutive addresses and step 2 (thread k2) reads at the odd stride XP, so neither
+// side has LDS bank conflicts.
+template <int R1>
+struct Xch {
+  static constexpr int XP = R1 % 2 == 0 ? R1 + 1 : R1;
+  static __device__ __forceinline__ int at(int line, int R2, int k2, int n1) { return (line * R2 + k2) * XP + n1; }
+};
+
 template <typename T, int N>
 struct RowCfg {
   static constexpr int R1 = Split<N>::R1, R2 = Split<N>::R2, TPL = cmax(R1, R2);
   static constexpr int LPW0 = 256 / TPL < lds_budget<T>() / N ? 256 / TPL : lds_budget<T>() / N;
   static constexpr int LPW = LPW0 < 1 ? 1 : LPW0;  // lines per workgroup
   static constexpr int THREADS = TPL * LPW;
+  static constexpr int LDS = cmax(N * (LPW + 1), LPW * R2 * Xch<R1>::XP);
 };
 
 // lines along the contiguous axis; IO supplies load(line, pos, rhs) / store(line, k, rhs, v).
@@ -502,7 +518,8 @@ __global__ __launch_bounds__(256) void k_row2(const cplx<T>* __restrict__ tw, IO
                                               int n_out) {
   using C = RowCfg<T, N>;
   constexpr int R1 = C::R1, R2 = C::R2, TPL = C::TPL, LPW = C::LPW, LP = LPW + 1;
-  __shared__ cplx<T> lds[N * LP];
+  using X = Xch<R1>;
+  __shared__ cplx<T> lds[C::LDS];
   const int t = threadIdx.x;
   const int ll = t / TPL, s = t - ll * TPL;
   const int64_t g0 = (int64_t)blockIdx.x * LPW;
@@ -510,6 +527,19 @@ __global__ __launch_bounds__(256) void k_row2(const cplx<T>* __restrict__ tw, IO
   const int rhs = blockIdx.y;
   const bool live = g < nlines;
   constexpr int NI = N / TPL;  // staged elements per thread (THREADS = TPL * LPW, TPL divides N)
+  // a small Pre (the unknown map alone) is requested for all of a thread's outputs before the transform
+  constexpr bool EARLY = !TOUT && sizeof(typename IO::Pre) <= 4;
+  typename IO::Pre pre0[EARLY ? R1 : 1];
+  if constexpr (EARLY) {
+    if (s < R2) {
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) {
+        const int k = s + R2 * k1;
+        const bool ok = live && k < n_out;
+        pre0[k1] = io.pre(ok ? g : 0, ok ? k : 0, rhs);
+      }
+    }
+  }
   if (TIN) {
     cplx<T> st[NI];
 #pragma unroll
@@ -545,17 +575,22 @@ __global__ __launch_bounds__(256) void k_row2(const cplx<T>* __restrict__ tw, IO
   if (TIN) __syncthreads();  // every thread has read its inputs before the exchange overwrites them
   if (s < R1) {
 #pragma unroll
-    for (int k2 = 0; k2 < R2; ++k2) lds[(s * R2 + k2) * LP + ll] = v[k2];
+    for (int k2 = 0; k2 < R2; ++k2) lds[X::at(ll, R2, k2, s)] = v[k2];
   }
   __syncthreads();
+  cplx<T> u[R1];
   if (s < R2) {
-    cplx<T> u[R1];
 #pragma unroll
-    for (int n1 = 0; n1 < R1; ++n1) u[n1] = lds[(n1 * R2 + s) * LP + ll];
+    for (int n1 = 0; n1 < R1; ++n1) u[n1] = lds[X::at(ll, R2, s, n1)];
     fft2_step2<T, N, INV>(u);
     if (TOUT) {
+      // staged below, once every thread has read the exchange
+    } else if constexpr (EARLY) {
 #pragma unroll
-      for (int k1 = 0; k1 < R1; ++k1) lds[(s + R2 * k1) * LP + ll] = u[k1];  // the locations just read
+      for (int k1 = 0; k1 < R1; ++k1) {
+        const int k = s + R2 * k1;
+        if (live && k < n_out) io.store(g, k, rhs, u[k1], pre0[k1]);
+      }
     } else {
       // the loads the stores depend on, PB outputs at a time (bounded register use)
       constexpr int PB = sizeof(T) == 8 ? 4 : 8;
@@ -577,6 +612,11 @@ __global__ __launch_bounds__(256) void k_row2(const cplx<T>* __restrict__ tw, IO
     }
   }
   if (TOUT) {
+    __syncthreads();
+    if (s < R2) {
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) lds[(s + R2 * k1) * LP + ll] = u[k1];
+    }
     __syncthreads();
     constexpr int PB = sizeof(T) == 8 ? 4 : 8;
 #pragma unroll
@@ -613,7 +653,8 @@ struct RowPCfg {
   static constexpr int HL = 2 * LPW;                   // (line, parity) groups
   static constexpr int THREADS = TPL * HL;
   static constexpr int LP = LPW + 1, HLP = HL + 1;
-  static constexpr int LDS = cmax(2 * NH * LP, NH * HLP);  // staging [k][line] or exchange [n][group]
+  // staging [k][line], outputs [x][group] or the exchange [group][k2][n1]
+  static constexpr int LDS = cmax(cmax(2 * NH * LP, NH * HLP), HL * R2 * Xch<R1>::XP);
 };
 
 template <typename T, int NH, bool INV, class IO>
@@ -630,6 +671,19 @@ __global__ __launch_bounds__(512) void k_row2p(const cplx<T>* __restrict__ tw, c
   const int64_t g0 = (int64_t)blockIdx.x * LPW;
   const int rhs = blockIdx.y;
   cplx<T> v[R2];
+  // inverse: the loads the stores depend on (unknown map, input value, inner-product partner) are issued
+  // first, so that their round trips overlap the staging loads and the transform
+  constexpr int NOI = (NH * LPW + TH - 1) / TH;
+  typename IO::Pre pri[INV ? NOI : 1];
+  if constexpr (INV) {
+#pragma unroll
+    for (int j = 0; j < NOI; ++j) {
+      const int e = t + j * TH;
+      const int x = e % NH, ll = e / NH;
+      const bool ok = e < NH * LPW && g0 + ll < nlines;
+      pri[j] = io.pre(ok ? g0 + ll : 0, ok ? x : 0, rhs);
+    }
+  }
   if (INV) {
     constexpr int NI = (N2 * LPW + TH - 1) / TH;
     cplx<T> st[NI];
@@ -667,13 +721,13 @@ __global__ __launch_bounds__(512) void k_row2p(const cplx<T>* __restrict__ tw, c
   // exchange [n1 R2 + k2][group]
   if (s < R1) {
 #pragma unroll
-    for (int k2 = 0; k2 < R2; ++k2) lds[(s * R2 + k2) * HLP + hl] = v[k2];
+    for (int k2 = 0; k2 < R2; ++k2) lds[Xch<R1>::at(hl, R2, k2, s)] = v[k2];
   }
   __syncthreads();
   cplx<T> u[R1];
   if (s < R2) {
 #pragma unroll
-    for (int n1 = 0; n1 < R1; ++n1) u[n1] = lds[(n1 * R2 + s) * HLP + hl];
+    for (int n1 = 0; n1 < R1; ++n1) u[n1] = lds[Xch<R1>::at(hl, R2, s, n1)];
     fft2_step2<T, NH, INV>(u);
   }
   __syncthreads();  // the exchange buffer is reused below
@@ -709,30 +763,18 @@ __global__ __launch_bounds__(512) void k_row2p(const cplx<T>* __restrict__ tw, c
       for (int k1 = 0; k1 < R1; ++k1) lds[(s + R2 * k1) * HLP + hl] = u[k1];
     }
     __syncthreads();
-    constexpr int NO = (NH * LPW + TH - 1) / TH;
     constexpr int KR = RedK<IO>::K;
     double rre[KR > 0 ? KR : 1] = {}, rim[KR > 0 ? KR : 1] = {};
 #pragma unroll
-    for (int i0 = 0; i0 < NO; i0 += PB) {
-      typename IO::Pre pr[PB];
-#pragma unroll
-      for (int j = 0; j < PB; ++j) {
-        const int e = t + (i0 + j) * TH;
-        const int x = e % NH, ll = e / NH;
-        const bool ok = i0 + j < NO && e < NH * LPW && g0 + ll < nlines;
-        pr[j] = io.pre(ok ? g0 + ll : 0, ok ? x : 0, rhs);
-      }
-#pragma unroll
-      for (int j = 0; j < PB; ++j) {
-        const int e = t + (i0 + j) * TH;
-        const int x = e % NH, ll = e / NH;
-        if (i0 + j < NO && e < NH * LPW && g0 + ll < nlines) {
-          const cplx<T> y = lds[x * HLP + 2 * ll] + cconj(tw2[x]) * lds[x * HLP + 2 * ll + 1];
-          if constexpr (KR > 0)
-            io.store(g0 + ll, x, rhs, y, pr[j], rre, rim);
-          else
-            io.store(g0 + ll, x, rhs, y, pr[j]);
-        }
+    for (int j = 0; j < NOI; ++j) {
+      const int e = t + j * TH;
+      const int x = e % NH, ll = e / NH;
+      if (e < NH * LPW && g0 + ll < nlines) {
+        const cplx<T> y = lds[x * HLP + 2 * ll] + cconj(tw2[x]) * lds[x * HLP + 2 * ll + 1];
+        if constexpr (KR > 0)
+          io.store(g0 + ll, x, rhs, y, pri[j], rre, rim);
+        else
+          io.store(g0 + ll, x, rhs, y, pri[j]);
       }
     }
     // per-workgroup partials of the IO's sums (one right-hand side per launch)
@@ -761,6 +803,12 @@ __global__ __launch_bounds__(512) void k_row_pcmv(const cplx<T>* __restrict__ tw
   const bool la = hl < LPW;
   const bool lva = la && g0 + hl < nlines;
   const int64_t ga = lva ? g0 + hl : 0;
+  // the unknown map of this thread's phase-A outputs, requested before the transform
+  int mm[R1];
+  if (la && s < R2) {
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) mm[k1] = lva ? map3[ga * NH + s + R2 * k1] : -1;
+  }
   if (la && s < R1) {
     cplx<T> v[R2];
 #pragma unroll
@@ -771,18 +819,15 @@ __global__ __launch_bounds__(512) void k_row_pcmv(const cplx<T>* __restrict__ tw
     }
     fft2_step1<T, NH, true>(v, s, tw);
 #pragma unroll
-    for (int k2 = 0; k2 < R2; ++k2) lds[(s * R2 + k2) * HLP + hl] = v[k2];
+    for (int k2 = 0; k2 < R2; ++k2) lds[Xch<R1>::at(hl, R2, k2, s)] = v[k2];
   }
   __syncthreads();
   {
     cplx<T> u[R1];
-    int mm[R1];
     if (la && s < R2) {
 #pragma unroll
-      for (int n1 = 0; n1 < R1; ++n1) u[n1] = lds[(n1 * R2 + s) * HLP + hl];
+      for (int n1 = 0; n1 < R1; ++n1) u[n1] = lds[Xch<R1>::at(hl, R2, s, n1)];
       fft2_step2<T, NH, true>(u);
-#pragma unroll
-      for (int k1 = 0; k1 < R1; ++k1) mm[k1] = lva ? map3[ga * NH + s + R2 * k1] : -1;
     }
     __syncthreads();  // the exchange area is reused by the staging below
     if (la && s < R2) {
@@ -815,13 +860,13 @@ __global__ __launch_bounds__(512) void k_row_pcmv(const cplx<T>* __restrict__ tw
   if (s < R1) {
     fft2_step1<T, NH, false>(v, s, tw);
 #pragma unroll
-    for (int k2 = 0; k2 < R2; ++k2) lds[(s * R2 + k2) * HLP + hl] = v[k2];
+    for (int k2 = 0; k2 < R2; ++k2) lds[Xch<R1>::at(hl, R2, k2, s)] = v[k2];
   }
   __syncthreads();
   cplx<T> u[R1];
   if (s < R2) {
 #pragma unroll
-    for (int n1 = 0; n1 < R1; ++n1) u[n1] = lds[(n1 * R2 + s) * HLP + hl];
+    for (int n1 = 0; n1 < R1; ++n1) u[n1] = lds[Xch<R1>::at(hl, R2, s, n1)];
     fft2_step2<T, NH, false>(u);
   }
   __syncthreads();
@@ -864,8 +909,8 @@ struct BiRowArgs {
 template <typename T, int N, int MODE>
 __global__ __launch_bounds__(256) void k_row_bi(const cplx<T>* __restrict__ tw, BiRowArgs<T> a, int64_t nlines) {
   using C = RowCfg<T, N>;
-  constexpr int R1 = C::R1, R2 = C::R2, TPL = C::TPL, LPW = C::LPW, LP = LPW + 1;
-  __shared__ cplx<T> lds[N * LP];
+  constexpr int R1 = C::R1, R2 = C::R2, TPL = C::TPL, LPW = C::LPW;
+  __shared__ cplx<T> lds[C::LDS];
   const int t = threadIdx.x;
   const int ll = t / TPL, s = t - ll * TPL;
   const int64_t g = (int64_t)blockIdx.x * LPW + ll;
@@ -921,13 +966,13 @@ __global__ __launch_bounds__(256) void k_row_bi(const cplx<T>* __restrict__ tw, 
   }
   if (s < R1) {
 #pragma unroll
-    for (int k2 = 0; k2 < R2; ++k2) lds[(s * R2 + k2) * LP + ll] = v[k2];
+    for (int k2 = 0; k2 < R2; ++k2) lds[Xch<R1>::at(ll, R2, k2, s)] = v[k2];
   }
   __syncthreads();
   if (s < R2) {
     cplx<T> u[R1];
 #pragma unroll
-    for (int n1 = 0; n1 < R1; ++n1) u[n1] = lds[(n1 * R2 + s) * LP + ll];
+    for (int n1 = 0; n1 < R1; ++n1) u[n1] = lds[Xch<R1>::at(ll, R2, s, n1)];
     fft2_step2<T, N, false>(u);
     if (live) {
 #pragma unroll

@@ -83,6 +83,26 @@ __global__ __launch_bounds__(RED_THREADS) void k_dots(DotArgs<T> args, int64_t n
     for (int k = 0; k < K; ++k) partial[blockIdx.x * K + k] = tot[k];
 }
 
+// lane's share of a fixed-order partial sum: entries lane, lane + 64, ... of p[i * K + k], i < nblk, added in
+// that order; the loads of 8 entries are issued together (out-of-range slots add zero)
+__device__ __forceinline__ void lane_sum(const c128* __restrict__ p, int nblk, int K, int k, int lane, double& a,
+                                         double& b) {
+  constexpr int U = 8;
+  for (int i0 = lane; i0 < nblk; i0 += 64 * U) {
+    c128 v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = i0 + 64 * u;
+      v[u] = i < nblk ? p[(int64_t)i * K + k] : mk<double>(0, 0);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      a += v[u].x;
+      b += v[u].y;
+    }
+  }
+}
+
 // out[k] = sum over nblk partials, fixed order
 // out[k] = sum over nblk partials, fixed order: one wavefront per k (lane l sums blocks l, l + 64, ..., then a
 // shuffle tree); workgroup b covers k = 4 b ... 4 b + 3
@@ -90,11 +110,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_reduce_partials(const c128* par
   const int k = blockIdx.x * (RED_THREADS / 64) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (k >= K) return;
   double a = 0, b = 0;
-  for (int i = lane; i < nblk; i += 64) {
-    const c128 v = partial[(int64_t)i * K + k];
-    a += v.x;
-    b += v.y;
-  }
+  lane_sum(partial, nblk, K, k, lane, a, b);
   for (int off = 32; off >= 1; off >>= 1) {
     a += __shfl_xor(a, off);
     b += __shfl_xor(b, off);
@@ -497,11 +513,7 @@ __global__ __launch_bounds__(320) void k_bi_scalar(const c128* partial, int nblk
     const c128* pp = half ? part_half : partial;
     const int kk = half ? 0 : wave, KK = half ? 1 : K;
     double a = 0, b = 0;
-    for (int i = lane; i < nblk; i += 64) {
-      const c128 v = pp[i * KK + kk];
-      a += v.x;
-      b += v.y;
-    }
+    lane_sum(pp, nblk, KK, kk, lane, a, b);
     for (int off = 32; off >= 1; off >>= 1) {
       a += __shfl_xor(a, off);
       b += __shfl_xor(b, off);
@@ -599,11 +611,7 @@ __global__ __launch_bounds__(384) void k_bi_scalar_src(BiSrc src, int stage, BiS
     const c128* pp = src.p[wave];
     const int KK = src.K[wave], kk = src.k[wave], nb = src.nblk[wave];
     double a = 0, b = 0;
-    for (int i = lane; i < nb; i += 64) {
-      const c128 v = pp[i * KK + kk];
-      a += v.x;
-      b += v.y;
-    }
+    lane_sum(pp, nb, KK, kk, lane, a, b);
     for (int off = 32; off >= 1; off >>= 1) {
       a += __shfl_xor(a, off);
       b += __shfl_xor(b, off);
@@ -624,6 +632,23 @@ static c128* part_half(vs_ctx* ctx) { return ctx->red.template as<c128>() + 4 * 
 static void launch_scalar(vs_ctx* ctx, const c128* part, int nblk, int K, const c128* ph, int stage, BiScal* sc) {
   hipLaunchKernelGGL(k_bi_scalar, dim3(1), dim3(64 * (K + (ph ? 1 : 0))), 0, ctx->stream, part, nblk, K, ph, stage,
                      sc);
+}
+
+__global__ void k_bi_set(BiScal* sc, BiScal h, int flag_only) {
+  if (flag_only)
+    sc->flag = 0;
+  else
+    *sc = h;
+}
+
+void bi_set_scalars(vs_ctx* ctx, BiScal* sc, const BiScal& h) {
+  hipLaunchKernelGGL(k_bi_set, dim3(1), dim3(1), 0, ctx->stream, sc, h, 0);
+  VS_HIP(hipGetLastError());
+}
+
+void bi_clear_flag(vs_ctx* ctx, BiScal* sc) {
+  hipLaunchKernelGGL(k_bi_set, dim3(1), dim3(1), 0, ctx->stream, sc, BiScal{}, 1);
+  VS_HIP(hipGetLastError());
 }
 
 template <typename T>

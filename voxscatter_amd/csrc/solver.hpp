@@ -47,6 +47,9 @@ struct BiScal {
   int active;                 // batched solves: 0 once this right-hand side is finished
 };
 
+// the scalars of a solve set from the host (a one-thread kernel taking them by value), and the flag cleared
+void bi_set_scalars(vs_ctx* ctx, BiScal* sc, const BiScal& h);
+void bi_clear_flag(vs_ctx* ctx, BiScal* sc);
 // p = r + beta (p0 - omega v0) (p = r in the first iteration), out of place
 template <typename T>
 void bi_stage_rho(Engine<T>& E, const cplx<T>* r, const cplx<T>* v0, const cplx<T>* p0, cplx<T>* p, BiScal* sc);
