@@ -417,9 +417,9 @@ __global__ __launch_bounds__(FFT_THREADS) void k_mv_fzmul(FftDev<T> plan, cplx<T
       fN.load6(kx0 + w, ky, kz, f);
       cplx<T>* rowp = buf + kz * ld + w;
       const cplx<T> jx = rowp[0], jy = rowp[W], jz = rowp[2 * W];
-      rowp[0] = f[0] * jx + f[1] * jy + f[2] * jz;
-      rowp[W] = f[1] * jx + f[3] * jy + f[4] * jz;
-      rowp[2 * W] = f[2] * jx + f[4] * jy + f[5] * jz;
+      rowp[0] = cfma(f[2], jz, cfma(f[1], jy, f[0] * jx));
+      rowp[W] = cfma(f[4], jz, cfma(f[3], jy, f[1] * jx));
+      rowp[2 * W] = cfma(f[5], jz, cfma(f[4], jy, f[2] * jx));
     }
   }
   __syncthreads();
@@ -1107,17 +1107,17 @@ __global__ __launch_bounds__((FzCfg<T, N2>::THREADS)) void k_fz2(const cplx<T>* 
       fN.load6(kx0 + cc, ky, kz, f);
       cplx<T>* sp = spec + kz * W + cc;
       cplx<T> jx = sp[0], jy = sp[N2 * W], jz = sp[2 * N2 * W];
-      sp[0] = f[0] * jx + f[1] * jy + f[2] * jz;
-      sp[N2 * W] = f[1] * jx + f[3] * jy + f[4] * jz;
-      sp[2 * N2 * W] = f[2] * jx + f[4] * jy + f[5] * jz;
+      sp[0] = cfma(f[2], jz, cfma(f[1], jy, f[0] * jx));
+      sp[N2 * W] = cfma(f[4], jz, cfma(f[3], jy, f[1] * jx));
+      sp[2 * N2 * W] = cfma(f[5], jz, cfma(f[4], jy, f[2] * jx));
       if (kz != 0 && kz != NH) {
         sp = spec + (N2 - kz) * W + cc;
         jx = sp[0];
         jy = sp[N2 * W];
         jz = sp[2 * N2 * W];
-        sp[0] = f[0] * jx + f[1] * jy - f[2] * jz;
-        sp[N2 * W] = f[1] * jx + f[3] * jy - f[4] * jz;
-        sp[2 * N2 * W] = -(f[2] * jx) - f[4] * jy + f[5] * jz;
+        sp[0] = cfma(-f[2], jz, cfma(f[1], jy, f[0] * jx));
+        sp[N2 * W] = cfma(-f[4], jz, cfma(f[3], jy, f[1] * jx));
+        sp[2 * N2 * W] = cfma(f[5], jz, cfma(-f[4], jy, (-f[2]) * jx));
       }
     }
   }
@@ -1310,9 +1310,9 @@ __global__ __launch_bounds__(1024) void k_yz(const cplx<T>* __restrict__ twy, co
           const cplx<T>* f = fcur[j];
           cplx<T>* s3 = spec + kz * WZ + cc;
           const cplx<T> jx = s3[0], jy = s3[N2 * WZ], jz = s3[2 * N2 * WZ];
-          s3[0] = f[0] * jx + f[1] * jy + f[2] * jz;
-          s3[N2 * WZ] = f[1] * jx + f[3] * jy + f[4] * jz;
-          s3[2 * N2 * WZ] = f[2] * jx + f[4] * jy + f[5] * jz;
+          s3[0] = cfma(f[2], jz, cfma(f[1], jy, f[0] * jx));
+          s3[N2 * WZ] = cfma(f[4], jz, cfma(f[3], jy, f[1] * jx));
+          s3[2 * N2 * WZ] = cfma(f[5], jz, cfma(f[4], jy, f[2] * jx));
         }
       }
       __syncthreads();
@@ -1475,9 +1475,9 @@ __global__ __launch_bounds__(512, HALF && sizeof(T) == 4 ? 4 : 2) void k_yzq(con
         fN.load6(kx, 2 * jj + py, 2 * m + pz, f);
         cplx<T>* s3 = sp + jj * NLP + m;
         const cplx<T> jx = s3[0], jy = s3[N], jz = s3[2 * N];
-        s3[0] = f[0] * jx + f[1] * jy + f[2] * jz;
-        s3[N] = f[1] * jx + f[3] * jy + f[4] * jz;
-        s3[2 * N] = f[2] * jx + f[4] * jy + f[5] * jz;
+        s3[0] = cfma(f[2], jz, cfma(f[1], jy, f[0] * jx));
+        s3[N] = cfma(f[4], jz, cfma(f[3], jy, f[1] * jx));
+        s3[2 * N] = cfma(f[5], jz, cfma(f[4], jy, f[2] * jx));
       }
     }
   };
@@ -1489,9 +1489,9 @@ __global__ __launch_bounds__(512, HALF && sizeof(T) == 4 ? 4 : 2) void k_yzq(con
         const int jj = e % M, m = e / M;
         cplx<T>* s3 = sp + jj * NLP + m;
         const cplx<T> jx = s3[0], jy = s3[N], jz = s3[2 * N];
-        s3[0] = f[i][0] * jx + f[i][1] * jy + f[i][2] * jz;
-        s3[N] = f[i][1] * jx + f[i][3] * jy + f[i][4] * jz;
-        s3[2 * N] = f[i][2] * jx + f[i][4] * jy + f[i][5] * jz;
+        s3[0] = cfma(f[i][2], jz, cfma(f[i][1], jy, f[i][0] * jx));
+        s3[N] = cfma(f[i][4], jz, cfma(f[i][3], jy, f[i][1] * jx));
+        s3[2 * N] = cfma(f[i][5], jz, cfma(f[i][4], jy, f[i][2] * jx));
       }
     }
   };

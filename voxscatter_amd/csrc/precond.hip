@@ -714,32 +714,41 @@ __global__ __launch_bounds__(512) void k_blockmv2m(const cplx<T>* __restrict__ B
 #pragma unroll
     for (int c = 0; c < NV; ++c) acc[c] = mk<T>(0, 0);
     const int t0 = SYM ? r * (2 * n - r + 1) / 2 : r * n;
-    const bool rz = r >= nz0;
-    for (int s0 = 0; s0 < n; s0 += BU) {
-      cplx<T> b[BU];
-      T sg[BU];
+    const int rz = r >= nz0;
+    // batch of BU entries s0 .. s0 + BU - 1 of row r (sign folded in, zero past n)
+    auto load_batch = [&](int s0, cplx<T> (&b)[BU], T (&sg)[BU]) {
 #pragma unroll
       for (int j = 0; j < BU; ++j) {
         const int s = min(s0 + j, n - 1);
         int t;
         T g = (T)1;
         if (SYM) {
-          // s < r: transposed entry (s, r) with the z sign pattern; s >= r: row r of the upper triangle
-          const bool lo = s < r;
-          t = lo ? s * (2 * n - s + 1) / 2 + (r - s) : t0 + (s - r);
-          if (lo && rz != (s >= nz0)) g = (T)-1;
+          // s < r: transposed entry (s, r), t = s (2n - s + 1) / 2 + (r - s), negated when exactly one of r, s
+          // is a z unknown; s >= r: row r of the upper triangle, t = t0 + (s - r).  Branch-free: the terms in
+          // s are uniform over the workgroup, the choice is a mask (a branch here costs an exec-mask round
+          // per entry)
+          const int lo = s < r ? -1 : 0;
+          const int tlo = s * (2 * n - s + 1) / 2 - s + r, thi = t0 - r + s;
+          t = thi + ((tlo - thi) & lo);
+          g = (T)(1 - 2 * (lo & (rz ^ (s >= nz0 ? 1 : 0)) & 1));
         } else {
           t = t0 + s;
         }
-        b[j] = Bk[(int64_t)t * Wt + w];
+        // offsets within one layout tile stay below 2^31 for n < ~16000
+        b[j] = Bk[t * Wt + w];
         sg[j] = s0 + j < n ? g : (T)0;
       }
+    };
+    for (int s0 = 0; s0 < n; s0 += BU) {
+      cplx<T> b[BU];
+      T sg[BU];
+      load_batch(s0, b, sg);
 #pragma unroll
       for (int j = 0; j < BU; ++j) {
         const int s = min(s0 + j, n - 1);
         const cplx<T> bj = sg[j] * b[j];
 #pragma unroll
-        for (int c = 0; c < NV; ++c) acc[c] = acc[c] + bj * u[(c * n + s) * W + w];
+        for (int c = 0; c < NV; ++c) acc[c] = cfma(bj, u[(c * n + s) * W + w], acc[c]);
       }
     }
     const int kx = kx0 + w;

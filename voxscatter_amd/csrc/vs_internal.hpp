@@ -42,6 +42,15 @@ template <typename T> __host__ __device__ __forceinline__ cplx<T> operator*(cplx
 }
 template <typename T> __host__ __device__ __forceinline__ cplx<T> operator*(T s, cplx<T> a) { return mk<T>(s * a.x, s * a.y); }
 template <typename T> __host__ __device__ __forceinline__ cplx<T> cconj(cplx<T> a) { return mk<T>(a.x, -a.y); }
+// acc + a * b as four fused multiply-adds (acc + (a * b) written with the operators rounds the product
+// first and costs a multiply, a fused multiply-add and an add per component)
+template <typename T> __host__ __device__ __forceinline__ cplx<T> cfma(cplx<T> a, cplx<T> b, cplx<T> acc) {
+  acc.x = fma(a.x, b.x, acc.x);
+  acc.x = fma(-a.y, b.y, acc.x);
+  acc.y = fma(a.x, b.y, acc.y);
+  acc.y = fma(a.y, b.x, acc.y);
+  return acc;
+}
 // a * conj(b)
 template <typename T> __host__ __device__ __forceinline__ cplx<T> mulconj(cplx<T> a, cplx<T> b) {
   return mk<T>(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
