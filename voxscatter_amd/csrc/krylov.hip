@@ -83,13 +83,13 @@ static double eps_of() {
 // read back on its own.
 template <typename T>
 SolveResult solve_bicgstab(Engine<T>& E, const LinOp<T>& op, const cplx<T>* b, cplx<T>* x, bool has_x0, double tol,
-                           int maxit, bool verify) {
+                           int maxit, bool verify, double bnorm) {
   using cv = cplx<T>;
   const int64_t n = E.n;
   vs_ctx* ctx = E.ctx;
   SolveResult R;
   const double eps = eps_of<T>();
-  const double n2b = E.norm(b);
+  const double n2b = bnorm > 0 ? bnorm : E.norm(b);
   if (n2b == 0) {
     VS_HIP(hipMemsetAsync(x, 0, sizeof(cv) * n, ctx->stream));
     R.flag = 0;
@@ -98,7 +98,6 @@ SolveResult solve_bicgstab(Engine<T>& E, const LinOp<T>& op, const cplx<T>* b, c
     R.resvec = {0.0};
     return R;
   }
-  if (!has_x0) VS_HIP(hipMemsetAsync(x, 0, sizeof(cv) * n, ctx->stream));
   Vecs<T> W(ctx, vs_ctx::WS_KRYLOV, n, 14);
   cv *r = W[0], *rt = W[1], *ph = W[2], *sh = W[3], *t = W[4], *xmin = W[5];
   // p, v, xhalf, s of iteration i live in slot i & 1
@@ -107,36 +106,9 @@ SolveResult solve_bicgstab(Engine<T>& E, const LinOp<T>& op, const cplx<T>* b, c
   auto XH = [&](int i) { return W[10 + (i & 1)]; };
   auto S = [&](int i) { return W[12 + (i & 1)]; };
   int flag = 1;
-  if (verify) E.copy(x, xmin);  // the minimum-residual iterate is kept only by verified solves
   double imin = 0;
   const double tolb = tol * n2b;
   double normr;
-  if (has_x0) {
-    op.A(x, r);
-    normr = E.update(UPD_SUB_, b, nullptr, nullptr, nullptr, r, nullptr, C(0), C(0), true);
-  } else {
-    E.copy(b, r);  // x0 = 0: r = b exactly, no matvec
-    normr = n2b;
-  }
-  double normr_act = normr;
-  if (normr <= tolb) {
-    R.flag = 0;
-    R.relres = normr / n2b;
-    R.iter = 0;
-    R.resvec = {normr};
-    return R;
-  }
-  E.copy(r, rt);
-  std::vector<double> resvec(2 * (size_t)maxit + 1, 0.0);
-  resvec[0] = normr;
-  int stag = 0, moresteps = 0;
-  const int64_t maxmsteps = std::min<int64_t>(std::min<int64_t>(n / 50, 10), n - maxit);
-  const int maxstagsteps = 3;
-  double normrmin = normr;
-  int64_t nres = -1;
-  double iter = 0;
-  int ii = 0;
-
   BiScal* sc = static_cast<BiScal*>(ctx->workspace(vs_ctx::WS_KRYLOV_AUX, sizeof(BiScal)));
   BiScal* hs = reinterpret_cast<BiScal*>(ctx->pinned);  // two read-back slots (iteration parity)
   BiScal h{};
@@ -147,18 +119,49 @@ SolveResult solve_bicgstab(Engine<T>& E, const LinOp<T>& op, const cplx<T>* b, c
   h.beta = C(0);
   h.first = 1;
   h.has_pc = op.M ? 1 : 0;
-  bi_set_scalars(ctx, sc, h);
-  bi_stage_init<T>(E, rt, r, sc);
-  EventPair ev;
   // Fused halves (operator and preconditioner permitting) on the fast path only: a first half that must
   // finish its own half step (moresteps > 0) and a second half recomputed from a replaced s use the
   // separate stages, so they never depend on the fused passes' vectors.
   const bool fused = op.F1 && op.F2 && op.M;
   BiParts bp;
-  if (fused) {
-    bi_parts_take<T>(E, bp);
-    bi_stage_xnorm<T>(E, x, bp);
+  if (fused) bi_parts_take<T>(E, bp);
+  if (has_x0) {
+    if (verify) E.copy(x, xmin);  // the minimum-residual iterate is kept only by verified solves
+    op.A(x, r);
+    normr = E.update(UPD_SUB_, b, nullptr, nullptr, nullptr, r, nullptr, C(0), C(0), true);
+    E.copy(r, rt);
+  } else {
+    // x0 = 0: r = b exactly, no matvec; x = 0, rt = r (and the minimum-residual iterate) in one pass that
+    // also leaves the partials of conj(rt) . r = ||b||^2 (and of ||x||^2 = 0 for the fused halves)
+    bi_start_dev<T>(E, b, x, r, rt, verify ? xmin : nullptr, fused ? &bp : nullptr);
+    normr = n2b;
   }
+  double normr_act = normr;
+  if (normr <= tolb) {
+    R.flag = 0;
+    R.relres = normr / n2b;
+    R.iter = 0;
+    R.resvec = {normr};
+    return R;
+  }
+  std::vector<double> resvec(2 * (size_t)maxit + 1, 0.0);
+  resvec[0] = normr;
+  int stag = 0, moresteps = 0;
+  const int64_t maxmsteps = std::min<int64_t>(std::min<int64_t>(n / 50, 10), n - maxit);
+  const int maxstagsteps = 3;
+  double normrmin = normr;
+  int64_t nres = -1;
+  double iter = 0;
+  int ii = 0;
+
+  if (has_x0) {
+    bi_set_scalars(ctx, sc, h);
+    bi_stage_init<T>(E, rt, r, sc);
+    if (fused) bi_stage_xnorm<T>(E, x, bp);
+  } else {
+    bi_set_init_scalars<T>(E, sc, h);  // the scalars set and rho reduced from the start pass's partials
+  }
+  EventPair ev;
 
   auto first_half = [&](int i, bool finish_half) {
     cv* pi = P(i);
@@ -960,9 +963,9 @@ template SolveResult solve_pgmresdr<float>(Engine<float>&, const LinOp<float>&, 
                                            double, int, int, Recycle*);
 
 template SolveResult solve_bicgstab<double>(Engine<double>&, const LinOp<double>&, const c128*, c128*, bool, double, int,
-                                            bool);
+                                            bool, double);
 template SolveResult solve_bicgstab<float>(Engine<float>&, const LinOp<float>&, const c64*, c64*, bool, double, int,
-                                           bool);
+                                           bool, double);
 template SolveResult solve_gmres<double>(Engine<double>&, const LinOp<double>&, const c128*, c128*, bool, int, double,
                                          int);
 template SolveResult solve_gmres<float>(Engine<float>&, const LinOp<float>&, const c64*, c64*, bool, int, double, int);
@@ -985,25 +988,25 @@ SolveResult solve_refine(Engine<double>& E64, Engine<float>& E32, const LinOp<do
   SolveResult R;
   vs_ctx* ctx = E64.ctx;
   const double n2b = E64.norm(b);
-  if (!has_x0) VS_HIP(hipMemsetAsync(x, 0, sizeof(c128) * n, ctx->stream));
   if (n2b == 0) {
+    if (!has_x0) VS_HIP(hipMemsetAsync(x, 0, sizeof(c128) * n, ctx->stream));
     R.flag = 0;
     R.resvec = {0.0};
     return R;
   }
   Carve ws;
-  const size_t i64a = ws.add(sizeof(c128) * n), i32a = ws.add(sizeof(c64) * n), i32b = ws.add(sizeof(c64) * n),
-               i64b = ws.add(sizeof(c128) * n);
+  const size_t i64a = ws.add(sizeof(c128) * n), i32a = ws.add(sizeof(c64) * n), i32b = ws.add(sizeof(c64) * n);
   ws.take(ctx, vs_ctx::WS_REFINE);
-  const Span r64 = ws.at(i64a), r32 = ws.at(i32a), d32 = ws.at(i32b), d64 = ws.at(i64b);
+  const Span r64 = ws.at(i64a), r32 = ws.at(i32a), d32 = ws.at(i32b);
+  bool x_set = has_x0;  // x holds an iterate (x0, or the first update from x0 = 0)
   int tot = 0;
   double relres = 0;
   R.flag = 1;
   int outer = 0;
   for (outer = 0; outer <= max_outer; ++outer) {
     double nr;
-    if (outer == 0 && !has_x0) {
-      E64.copy(b, r64.as<c128>());  // x0 = 0: r = b exactly, no matvec
+    const bool from_b = outer == 0 && !has_x0;  // x0 = 0: r = b exactly, no matvec
+    if (from_b) {
       nr = n2b;
     } else {
       A64.A(x, r64.as<c128>());
@@ -1021,20 +1024,21 @@ SolveResult solve_refine(Engine<double>& E64, Engine<float>& E32, const LinOp<do
       break;
     }
     if (outer == max_outer) break;
-    // scale the residual to unit norm before rounding to FP32
-    E64.update(UPD_SCALE_, r64.as<c128>(), nullptr, nullptr, nullptr, r64.as<c128>(), nullptr, mk<double>(1.0 / nr, 0),
-               mk<double>(0, 0), false);
-    convert_dev(ctx, r64.p, VS_FP64, r32.p, VS_FP32, n);
+    // scale the residual to unit norm and round it to FP32 in one pass
+    scale_to32_dev(ctx, from_b ? b : r64.as<c128>(), r32.as<c64>(), n, 1.0 / nr);
     SolveResult in;
     if (inner_solver == 0)
-      in = solve_bicgstab<float>(E32, A32, r32.as<c64>(), d32.as<c64>(), false, inner_tol, inner_maxit, false);
+      // the right-hand side was scaled to unit norm before rounding: its norm is 1 to FP32 rounding, which
+      // is all the inner tolerance needs (no reduction and host round trip for it)
+      in = solve_bicgstab<float>(E32, A32, r32.as<c64>(), d32.as<c64>(), false, inner_tol, inner_maxit, false, 1.0);
     else
       in = solve_gmres<float>(E32, A32, r32.as<c64>(), d32.as<c64>(), false, restart, inner_tol, inner_maxit);
     tot += inner_solver == 0 ? (int)(in.resvec.size() - 1) : (int)(in.resvec.size() - 1);
-    convert_dev(ctx, d32.p, VS_FP32, d64.p, VS_FP64, n);
-    E64.update(UPD_AXPBY_, d64.as<c128>(), nullptr, nullptr, nullptr, x, nullptr, mk<double>(nr, 0), mk<double>(1, 0),
-               false);
+    // x += nr d (x = nr d on the first step from x0 = 0, which skips zeroing x)
+    axpy_from32_dev(ctx, d32.as<c64>(), x, n, nr, from_b);
+    x_set = true;
   }
+  if (!x_set) VS_HIP(hipMemsetAsync(x, 0, sizeof(c128) * n, ctx->stream));
   R.relres = relres;
   R.outer = outer;
   R.iter = outer;

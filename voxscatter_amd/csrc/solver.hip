@@ -68,12 +68,27 @@ __global__ __launch_bounds__(RED_THREADS) void k_dots(DotArgs<T> args, int64_t n
   double re[K], im[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) re[k] = im[k] = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * RED_THREADS) {
+  constexpr int U = K <= 2 ? 4 : 2;  // elements per thread whose loads are issued together
+  for (int64_t i0 = (int64_t)blockIdx.x * RED_THREADS * U + threadIdx.x; i0 < n;
+       i0 += (int64_t)gridDim.x * RED_THREADS * U) {
+    cplx<T> a[U][K], b[U][K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const cplx<T> a = args.a[k][i], b = args.b[k][i];
-      re[k] += (double)a.x * b.x + (double)a.y * b.y;
-      im[k] += (double)a.x * b.y - (double)a.y * b.x;
+    for (int u = 0; u < U; ++u) {
+      const int64_t i = min(i0 + u * RED_THREADS, n - 1);
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        a[u][k] = args.a[k][i];
+        b[u][k] = args.b[k][i];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (i0 + u * RED_THREADS >= n) break;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        re[k] += (double)a[u][k].x * b[u][k].x + (double)a[u][k].y * b[u][k].y;
+        im[k] += (double)a[u][k].x * b[u][k].y - (double)a[u][k].y * b[u][k].x;
+      }
     }
   }
   c128 tot[K];
@@ -145,22 +160,45 @@ struct UpdArgs {
 template <typename T>
 __global__ __launch_bounds__(RED_THREADS) void k_update(int op, UpdArgs<T> a, int64_t n, int want_norm, c128* partial) {
   double re[1] = {0.0}, im[1] = {0.0};
-  for (int64_t i = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * RED_THREADS) {
-    cplx<T> out;
-    switch (op) {
-      case UPD_AXPBY: out = a.al * a.x0[i] + a.be * a.y[i]; a.y[i] = out; break;
-      case UPD_BICG_P: out = a.x0[i] + a.be * (a.y[i] - a.al * a.x1[i]); a.y[i] = out; break;
-      case UPD_HALF: {
-        a.y[i] = a.x0[i] + a.al * a.x1[i];
-        out = a.x2[i] - a.al * a.x3[i];
-        a.z[i] = out;
-        break;
+  // the operands of U elements per thread are loaded together (each element is read and written by one
+  // thread only, so loading a group before any of its stores is safe even for in-place operands)
+  constexpr int U = 4;
+  const bool r0 = op != UPD_COPY && op != UPD_SCALE && op != UPD_SUB;  // reads x1
+  const bool ry = op == UPD_AXPBY || op == UPD_BICG_P || op == UPD_SUB;
+  for (int64_t i0 = (int64_t)blockIdx.x * RED_THREADS * U + threadIdx.x; i0 < n;
+       i0 += (int64_t)gridDim.x * RED_THREADS * U) {
+    cplx<T> q0[U], q1[U], q2[U], q3[U], qy[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t i = min(i0 + u * RED_THREADS, n - 1);
+      q0[u] = a.x0[i];
+      if (r0 && op != UPD_AXPBY) q1[u] = a.x1[i];
+      if (op == UPD_HALF) {
+        q2[u] = a.x2[i];
+        q3[u] = a.x3[i];
       }
-      case UPD_SUB: out = a.x0[i] - a.y[i]; a.y[i] = out; break;
-      case UPD_COPY: out = a.x0[i]; a.y[i] = out; break;
-      default: out = a.al * a.x0[i]; a.y[i] = out; break;
+      if (ry) qy[u] = a.y[i];
     }
-    re[0] += (double)out.x * out.x + (double)out.y * out.y;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t i = i0 + u * RED_THREADS;
+      if (i >= n) break;
+      cplx<T> out;
+      switch (op) {
+        case UPD_AXPBY: out = a.al * q0[u] + a.be * qy[u]; a.y[i] = out; break;
+        case UPD_BICG_P: out = q0[u] + a.be * (qy[u] - a.al * q1[u]); a.y[i] = out; break;
+        case UPD_HALF: {
+          a.y[i] = q0[u] + a.al * q1[u];
+          out = q2[u] - a.al * q3[u];
+          a.z[i] = out;
+          break;
+        }
+        case UPD_SUB: out = q0[u] - qy[u]; a.y[i] = out; break;
+        case UPD_COPY: out = q0[u]; a.y[i] = out; break;
+        default: out = a.al * q0[u]; a.y[i] = out; break;
+      }
+      re[0] += (double)out.x * out.x + (double)out.y * out.y;
+    }
   }
   if (want_norm) {
     c128 tot[1];
@@ -532,12 +570,27 @@ __global__ __launch_bounds__(RED_THREADS) void k_bi_dots(DotArgs<T> args, int64_
   double re[K], im[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) re[k] = im[k] = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * RED_THREADS) {
+  constexpr int U = K <= 2 ? 4 : 2;  // elements per thread whose loads are issued together
+  for (int64_t i0 = (int64_t)blockIdx.x * RED_THREADS * U + threadIdx.x; i0 < n;
+       i0 += (int64_t)gridDim.x * RED_THREADS * U) {
+    cplx<T> a[U][K], b[U][K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const cplx<T> a = args.a[k][i], b = args.b[k][i];
-      re[k] += (double)a.x * b.x + (double)a.y * b.y;
-      im[k] += (double)a.x * b.y - (double)a.y * b.x;
+    for (int u = 0; u < U; ++u) {
+      const int64_t i = min(i0 + u * RED_THREADS, n - 1);
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        a[u][k] = args.a[k][i];
+        b[u][k] = args.b[k][i];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (i0 + u * RED_THREADS >= n) break;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        re[k] += (double)a[u][k].x * b[u][k].x + (double)a[u][k].y * b[u][k].y;
+        im[k] += (double)a[u][k].x * b[u][k].y - (double)a[u][k].y * b[u][k].x;
+      }
     }
   }
   c128 tot[K];
@@ -573,18 +626,35 @@ __global__ __launch_bounds__(RED_THREADS) void k_bi_axpy2(const cplx<T>* x0, con
   double re[3] = {0.0, 0.0, 0.0}, im[3] = {0.0, 0.0, 0.0};
   if (!sc->flag) {
     const cplx<T> c = ccast<T>(full ? sc->omega : sc->alpha);
-    for (int64_t i = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * RED_THREADS) {
-      const cplx<T> yv = x0[i] + c * x1[i];
-      y[i] = yv;
-      const cplx<T> o = x2[i] - c * x3[i];
-      z[i] = o;
-      re[0] += (double)o.x * o.x + (double)o.y * o.y;
-      if (full) {
-        const cplx<T> a = rt[i];
-        re[1] += (double)a.x * o.x + (double)a.y * o.y;
-        im[1] += (double)a.x * o.y - (double)a.y * o.x;
+    constexpr int U = 4;  // elements per thread whose loads are issued together
+    for (int64_t i0 = (int64_t)blockIdx.x * RED_THREADS * U + threadIdx.x; i0 < n;
+         i0 += (int64_t)gridDim.x * RED_THREADS * U) {
+      cplx<T> q0[U], q1[U], q2[U], q3[U], qr[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int64_t i = min(i0 + u * RED_THREADS, n - 1);
+        q0[u] = x0[i];
+        q1[u] = x1[i];
+        q2[u] = x2[i];
+        q3[u] = x3[i];
+        if (full) qr[u] = rt[i];
       }
-      if (partial_y) re[2] += (double)yv.x * yv.x + (double)yv.y * yv.y;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int64_t i = i0 + u * RED_THREADS;
+        if (i >= n) break;
+        const cplx<T> yv = q0[u] + c * q1[u];
+        y[i] = yv;
+        const cplx<T> o = q2[u] - c * q3[u];
+        z[i] = o;
+        re[0] += (double)o.x * o.x + (double)o.y * o.y;
+        if (full) {
+          const cplx<T> a = qr[u];
+          re[1] += (double)a.x * o.x + (double)a.y * o.y;
+          im[1] += (double)a.x * o.y - (double)a.y * o.x;
+        }
+        if (partial_y) re[2] += (double)yv.x * yv.x + (double)yv.y * yv.y;
+      }
     }
   }
   c128 tot[3];
@@ -660,6 +730,67 @@ void bi_stage_init(Engine<T>& E, const cplx<T>* rt, const cplx<T>* r, BiScal* sc
   launch_scalar(E.ctx, part_main(E.ctx), E.grid, 1, nullptr, (int)BI_INIT, sc);
   VS_HIP(hipGetLastError());
 }
+
+// start of a BiCGStab solve from x0 = 0: x = 0, r = rt = b (and the minimum-residual iterate zeroed), with
+// per-workgroup partials of conj(rt) . r = ||b||^2 and (px nullable) zero partials of ||x||^2
+template <typename T>
+__global__ __launch_bounds__(RED_THREADS) void k_bi_start(const cplx<T>* __restrict__ b, cplx<T>* __restrict__ x,
+                                                          cplx<T>* __restrict__ r, cplx<T>* __restrict__ rt,
+                                                          cplx<T>* __restrict__ xmin, int64_t n, c128* part,
+                                                          c128* px) {
+  double re[1] = {0.0}, im[1] = {0.0};
+  for (int64_t i = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * RED_THREADS) {
+    const cplx<T> v = b[i];
+    x[i] = mk<T>(0, 0);
+    r[i] = v;
+    rt[i] = v;
+    if (xmin) xmin[i] = mk<T>(0, 0);
+    re[0] += (double)v.x * v.x + (double)v.y * v.y;
+  }
+  c128 tot[1];
+  block_sum<1>(re, im, tot);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = tot[0];
+    if (px) px[blockIdx.x] = mk<double>(0, 0);
+  }
+}
+
+template <typename T>
+void bi_start_dev(Engine<T>& E, const cplx<T>* b, cplx<T>* x, cplx<T>* r, cplx<T>* rt, cplx<T>* xmin, BiParts* P) {
+  hipLaunchKernelGGL((k_bi_start<T>), dim3(E.grid), dim3(RED_THREADS), 0, E.ctx->stream, b, x, r, rt, xmin, E.n,
+                     part_main(E.ctx), P ? P->x : (c128*)nullptr);
+  if (P) P->n_x = E.grid;
+  VS_HIP(hipGetLastError());
+}
+
+// *sc = h, then rho_n from the start pass's partials (the reduction of k_bi_scalar, one wavefront)
+__global__ __launch_bounds__(64) void k_bi_set_init(const c128* partial, int nblk, BiScal h, BiScal* sc) {
+  __shared__ c128 tot[1];
+  const int lane = threadIdx.x & 63;
+  double a = 0, b = 0;
+  lane_sum(partial, nblk, 1, 0, lane, a, b);
+  for (int off = 32; off >= 1; off >>= 1) {
+    a += __shfl_xor(a, off);
+    b += __shfl_xor(b, off);
+  }
+  if (lane == 0) {
+    tot[0] = mk<double>(a, b);
+    BiScal S = h;
+    bi_logic(BI_INIT, tot, S);
+    *sc = S;
+  }
+}
+
+template <typename T>
+void bi_set_init_scalars(Engine<T>& E, BiScal* sc, const BiScal& h) {
+  hipLaunchKernelGGL(k_bi_set_init, dim3(1), dim3(64), 0, E.ctx->stream, part_main(E.ctx), E.grid, h, sc);
+  VS_HIP(hipGetLastError());
+}
+
+template void bi_start_dev<float>(Engine<float>&, const c64*, c64*, c64*, c64*, c64*, BiParts*);
+template void bi_start_dev<double>(Engine<double>&, const c128*, c128*, c128*, c128*, c128*, BiParts*);
+template void bi_set_init_scalars<float>(Engine<float>&, BiScal*, const BiScal&);
+template void bi_set_init_scalars<double>(Engine<double>&, BiScal*, const BiScal&);
 
 template <typename T>
 void bi_stage_rho(Engine<T>& E, const cplx<T>* r, const cplx<T>* v0, const cplx<T>* p0, cplx<T>* p, BiScal* sc) {
@@ -1018,6 +1149,35 @@ template void batch_sub<double>(Engine<double>&, int, const c128*, c128*);
 
 template struct Engine<float>;
 template struct Engine<double>;
+
+// out = (FP32) (s * in), in FP64 (the refinement's first scaled residual, r = b)
+__global__ __launch_bounds__(256) void k_scale_to32(const c128* __restrict__ in, c64* __restrict__ out, int64_t n,
+                                                    double s) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const c128 v = in[i];
+    out[i] = mk<float>((float)(s * v.x), (float)(s * v.y));
+  }
+}
+
+// x = a d (first) or x += a d, d in FP32 (the refinement's update with the inner correction)
+__global__ __launch_bounds__(256) void k_axpy_from32(const c64* __restrict__ d, c128* __restrict__ x, int64_t n,
+                                                     double a, int first) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const c64 v = d[i];
+    const c128 dv = mk<double>(a * (double)v.x, a * (double)v.y);
+    x[i] = first ? dv : x[i] + dv;
+  }
+}
+
+void scale_to32_dev(vs_ctx* ctx, const c128* in, c64* out, int64_t n, double s) {
+  hipLaunchKernelGGL(k_scale_to32, dim3(1024), dim3(256), 0, ctx->stream, in, out, n, s);
+  VS_HIP(hipGetLastError());
+}
+
+void axpy_from32_dev(vs_ctx* ctx, const c64* d, c128* x, int64_t n, double a, bool first) {
+  hipLaunchKernelGGL(k_axpy_from32, dim3(1024), dim3(256), 0, ctx->stream, d, x, n, a, first ? 1 : 0);
+  VS_HIP(hipGetLastError());
+}
 
 void convert_dev(vs_ctx* ctx, const void* in, int pin, void* out, int pout, int64_t n) {
   if (pin == pout) {

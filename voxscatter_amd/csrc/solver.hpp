@@ -160,8 +160,9 @@ template <typename T>
 // verify = false (inner solves of the refinement, whose result the FP64 residual checks): a recursive
 // residual norm <= tol * ||b|| ends the solve without MATLAB's true-residual check, and an unconverged
 // solve returns its last iterate (no minimum-residual iterate is kept)
+// bnorm > 0: ||b|| known to the caller (no reduction and host round trip for it)
 SolveResult solve_bicgstab(Engine<T>& E, const LinOp<T>& op, const cplx<T>* b, cplx<T>* x, bool has_x0, double tol,
-                           int maxit, bool verify = true);
+                           int maxit, bool verify = true, double bnorm = -1.0);
 template <typename T>
 SolveResult solve_gmres(Engine<T>& E, const LinOp<T>& op, const cplx<T>* b, cplx<T>* x, bool has_x0, int restart,
                         double tol, int maxit);
@@ -195,6 +196,17 @@ BatchResult solve_refine_batch(Engine<double>& E64, Engine<float>& E32, const Li
                                const c128* diag = nullptr);
 
 void convert_dev(vs_ctx* ctx, const void* in, int pin, void* out, int pout, int64_t n);
+// out = (FP32) (s * in)
+void scale_to32_dev(vs_ctx* ctx, const c128* in, c64* out, int64_t n, double s);
+// x = a d (first) or x += a d, d in FP32
+void axpy_from32_dev(vs_ctx* ctx, const c64* d, c128* x, int64_t n, double a, bool first);
+// x = 0, r = rt = b, xmin = 0 (nullable), with the partials of conj(rt) . r = ||b||^2 for
+// bi_set_init_scalars and, when P is given, zero partials of ||x||^2 for the fused halves
+template <typename T>
+void bi_start_dev(Engine<T>& E, const cplx<T>* b, cplx<T>* x, cplx<T>* r, cplx<T>* rt, cplx<T>* xmin, BiParts* P);
+// the scalars set to h and rho_n reduced from bi_start_dev's partials (one kernel)
+template <typename T>
+void bi_set_init_scalars(Engine<T>& E, BiScal* sc, const BiScal& h);
 void mvp_dev(vs_op* op, const void* x, void* y, int transp, int precision, int nrhs);
 // the same matvec with a per-right-hand-side diagonal arhs[rhs] (device, nrhs complex doubles) in place of
 // the operator's Gram Mr (and Gram conj(Mr) in the transp branch)
