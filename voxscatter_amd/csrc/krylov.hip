@@ -214,7 +214,10 @@ SolveResult solve_bicgstab(Engine<T>& E, const LinOp<T>& op, const cplx<T>* b, c
       second_half(ii, true);
       snapshot(ii);
       spec = false;
-      if (ii < maxit) {
+      // the next first half is launched ahead unless this iteration is likely the last: a residual
+      // already within 10 tolb converges in about one iteration here, and the half step launched ahead of
+      // a converged one is wasted work (skipping it costs one host round trip instead)
+      if (ii < maxit && !(normr <= 10 * tolb)) {
         first_half(ii + 1, false);
         spec = true;
       }
