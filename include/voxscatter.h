@@ -222,6 +222,15 @@ int vs_pgmresdr_rec(vs_op* op, vs_pc* pc, int side, const double* b, double* x, 
                     int maxit, int ritz, const double* x0, vs_recycle* rec, int* flag, double* relres,
                     int iter[2], double* resvec, int* nres);
 
+/* Fused BiCGStab halves (no counterpart in the reference, whose bicgstab calls MVP and the preconditioner
+ * as separate functions): for one right-hand side with an FFT operator whose matvec is slab-major and a
+ * circulant preconditioner built from that operator's domain, the Krylov updates and inner products run
+ * inside the FFT passes (same vectors and iteration counts; inner products summed in another order).
+ * vs_set_fused_krylov(0) turns the fused path off for the whole process (1: on, the default);
+ * vs_fused_krylov_available reports in *on whether solves with (op, pc) in a precision take it. */
+int vs_set_fused_krylov(int on);
+int vs_fused_krylov_available(vs_op* op, vs_pc* pc, int precision, int* on);
+
 /* Device-pointer forms: b, x are FP64 device vectors of vs_op_nunknowns() values. */
 int vs_bicgstab_dev(vs_op* op, vs_pc* pc, const void* b, void* x, double tol, int maxit, int* flag,
                     double* relres, double* iter, int* nres);

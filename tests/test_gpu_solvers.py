@@ -93,6 +93,41 @@ def test_gmres_parity(c1, levels):
     assert _rel(x, xr) <= 1e-10
 
 
+@pytest.mark.parametrize("levels", [1, 2])
+def test_fused_halves_match_separate_stages(ctx, levels):
+    """The fused BiCGStab halves (Krylov updates and inner products formed inside the FFT passes) are the path
+    these solves take; solved again with them switched off (separate update, dot and scalar kernels), the
+    FP64 solve has the same flag, iteration count and resvec length, its residual history and solution agree
+    to rounding (the inner products are summed in another order), and the FP32 + FP64-refinement solve takes
+    the same outer steps and reaches the FP64 tolerance either way."""
+    from voxscatter_amd.capi import VS_FP32, VS_FP64
+    print(f"[fused halves] levels={levels}", flush=True)
+    sysd = DDASystem.build(G.hex_plate(), precisions=VS_FP64 | VS_FP32, ctx=ctx)
+    pc = sysd.preconditioner(levels, precisions=VS_FP64 | VS_FP32)
+    b = sysd.b
+    nb = np.linalg.norm(b)
+    assert api.fused_krylov_available(sysd.op, pc, VS_FP64)
+    assert api.fused_krylov_available(sysd.op, pc, VS_FP32)
+    x1, f1, r1, it1, rv1 = api.bicgstab(sysd.op, b, TOL, 2000, pc)
+    xr1, fr1, rr1, o1, in1, _ = api.refine(sysd.op, b, pc, tol=TOL, inner="bicgstab", inner_tol=TOL / 3)
+    api.set_fused_krylov(False, ctx)
+    try:
+        assert not api.fused_krylov_available(sysd.op, pc, VS_FP64)
+        x0, f0, r0, it0, rv0 = api.bicgstab(sysd.op, b, TOL, 2000, pc)
+        xr0, fr0, rr0, o0, in0, _ = api.refine(sysd.op, b, pc, tol=TOL, inner="bicgstab", inner_tol=TOL / 3)
+    finally:
+        api.set_fused_krylov(True, ctx)
+    assert api.fused_krylov_available(sysd.op, pc, VS_FP64)
+    print(f"  FP64: fused iter={it1} separate iter={it0}; refined: fused {o1}/{in1} separate {o0}/{in0}", flush=True)
+    assert f1 == f0 == 0 and it1 == it0 and len(rv1) == len(rv0)
+    assert np.max(np.abs(np.asarray(rv1) - np.asarray(rv0)) / np.asarray(rv0)) <= 1e-9
+    assert _rel(x1, x0) <= 1e-11
+    assert fr1 == fr0 == 0 and o1 == o0 and abs(in1 - in0) <= 2
+    for x in (xr1, xr0):
+        assert np.linalg.norm(sysd.op.mvp(x) - b) / nb <= TOL
+    assert _rel(xr1, xr0) <= 1e-5
+
+
 @pytest.mark.parametrize("side", ["L1", "R1"])
 def test_pgmres_parity(c1, side):
     print(f"[pgmres] side={side}", flush=True)

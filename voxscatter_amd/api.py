@@ -722,6 +722,20 @@ def refine(A, b, M1=None, tol=1e-6, max_outer=10, inner="bicgstab", inner_tol=1e
     return x, flag.value, relres.value, outer.value, tot.value, resvec[:nres.value].copy()
 
 
+def set_fused_krylov(on: bool, ctx=None) -> None:
+    """Turn the fused BiCGStab halves (Krylov updates and inner products inside the FFT passes) on or off for
+    the whole process; they are on by default."""
+    lib = (ctx or get_context()).lib
+    check(lib.vs_set_fused_krylov(1 if on else 0))
+
+
+def fused_krylov_available(A: Operator, M1: Preconditioner, precision: int) -> bool:
+    """Whether single-right-hand-side BiCGStab solves with A and M1 in `precision` take the fused halves."""
+    on = C.c_int()
+    check(A.ctx.lib.vs_fused_krylov_available(A.h, M1.h, int(precision), C.byref(on)))
+    return bool(on.value)
+
+
 def bicgstab_dev(A: Operator, b_ptr: int, x_ptr: int, tol=1e-6, maxit=2000, M1=None):
     flag, relres, it, nres = C.c_int(), C.c_double(), C.c_double(), C.c_int()
     check(A.ctx.lib.vs_bicgstab_dev(A.h, None if M1 is None else M1.h, C.c_void_p(b_ptr), C.c_void_p(x_ptr),

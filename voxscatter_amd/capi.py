@@ -66,6 +66,8 @@ SIGNATURES = {
     "vs_galerkin_near_table": (C.c_int, [C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "vs_galerkin_singular_directfn": (C.c_int, [C.c_double, C.c_double, C.c_int, D]),
     "vs_op_matvec_variant": (C.c_int, [P, C.c_int, C.POINTER(C.c_int)]),
+    "vs_set_fused_krylov": (C.c_int, [C.c_int]),
+    "vs_fused_krylov_available": (C.c_int, [P, P, C.c_int, C.POINTER(C.c_int)]),
     "vs_mvp": (INT, [P, D, D, INT]),
     "vs_mvp_dev": (INT, [P, P, P, INT, INT, INT]),
     "vs_mvp_dev_noalloc": (INT, [P, P, P, INT, INT, INT]),

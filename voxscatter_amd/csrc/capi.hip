@@ -484,6 +484,18 @@ int vs_op_set_domain(vs_op* op, const int64_t* idx, int64_t nidx, const double* 
 
 int64_t vs_op_nunknowns(const vs_op* op) { return op ? op->nunk : -1; }
 
+int vs_set_fused_krylov(int on) {
+  return guarded([&] { bi_fused_set(on != 0); });
+}
+
+int vs_fused_krylov_available(vs_op* op, vs_pc* pc, int precision, int* on) {
+  return guarded([&] {
+    VS_CHECK(op && pc && on, VS_ERR_ARG, "null pointer");
+    VS_CHECK(precision == VS_FP64 || precision == VS_FP32, VS_ERR_ARG, "precision must be VS_FP64 or VS_FP32");
+    *on = (precision == VS_FP64 ? bi_fused_available<double>(op, pc) : bi_fused_available<float>(op, pc)) ? 1 : 0;
+  });
+}
+
 int vs_op_matvec_variant(const vs_op* op, int precision, int* variant) {
   return guarded([&] {
     VS_CHECK(op && variant, VS_ERR_ARG, "null pointer");

@@ -20,6 +20,7 @@
 #include "handles.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cstdlib>
 #include <cstring>
 #include <type_traits>
@@ -2317,13 +2318,20 @@ static bool bi_len_compiled(int n) {
   }
 }
 
+// process-wide switch of the fused halves (VS_BIFUSED=0 in the environment starts with it off)
+static std::atomic<int> g_bi_fused{-1};
 static bool bi_fused_env() {
-  static const bool on = [] {
+  int v = g_bi_fused.load();
+  if (v < 0) {
     const char* e = std::getenv("VS_BIFUSED");
-    return !(e && e[0] == '0');
-  }();
-  return on;
+    int expect = -1;
+    g_bi_fused.compare_exchange_strong(expect, (e && e[0] == '0') ? 0 : 1);
+    v = g_bi_fused.load();
+  }
+  return v != 0;
 }
+
+void bi_fused_set(bool on) { g_bi_fused.store(on ? 1 : 0); }
 
 template <typename T>
 bool bi_fused_available(vs_op* op, vs_pc* pc) {

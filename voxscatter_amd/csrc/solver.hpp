@@ -226,6 +226,7 @@ void* pc_work_dev(vs_pc* pc, int precision);
 // fused BiCGStab halves of an FFT operator and a circulant preconditioner (see LinOp::F1 / F2)
 template <typename T>
 bool bi_fused_available(vs_op* op, vs_pc* pc);
+void bi_fused_set(bool on);
 template <typename T>
 void bi_fused_h1(vs_op* op, vs_pc* pc, const cplx<T>* r, const cplx<T>* p0, const cplx<T>* v0, cplx<T>* p, cplx<T>* ph,
                  cplx<T>* v, const cplx<T>* rt, const BiScal* sc, BiParts& P);
