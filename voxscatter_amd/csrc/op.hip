@@ -528,19 +528,6 @@ __global__ __launch_bounds__(256) void k_row2(const cplx<T>* __restrict__ tw, IO
   const int rhs = blockIdx.y;
   const bool live = g < nlines;
   constexpr int NI = N / TPL;  // staged elements per thread (THREADS = TPL * LPW, TPL divides N)
-  // a small Pre (the unknown map alone) is requested for all of a thread's outputs before the transform
-  constexpr bool EARLY = !TOUT && sizeof(typename IO::Pre) <= 4;
-  typename IO::Pre pre0[EARLY ? R1 : 1];
-  if constexpr (EARLY) {
-    if (s < R2) {
-#pragma unroll
-      for (int k1 = 0; k1 < R1; ++k1) {
-        const int k = s + R2 * k1;
-        const bool ok = live && k < n_out;
-        pre0[k1] = io.pre(ok ? g : 0, ok ? k : 0, rhs);
-      }
-    }
-  }
   if (TIN) {
     cplx<T> st[NI];
 #pragma unroll
@@ -586,12 +573,6 @@ __global__ __launch_bounds__(256) void k_row2(const cplx<T>* __restrict__ tw, IO
     fft2_step2<T, N, INV>(u);
     if (TOUT) {
       // staged below, once every thread has read the exchange
-    } else if constexpr (EARLY) {
-#pragma unroll
-      for (int k1 = 0; k1 < R1; ++k1) {
-        const int k = s + R2 * k1;
-        if (live && k < n_out) io.store(g, k, rhs, u[k1], pre0[k1]);
-      }
     } else {
       // the loads the stores depend on, PB outputs at a time (bounded register use)
       constexpr int PB = sizeof(T) == 8 ? 4 : 8;
