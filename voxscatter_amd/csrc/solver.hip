@@ -208,13 +208,15 @@ __global__ __launch_bounds__(RED_THREADS) void k_update(int op, UpdArgs<T> a, in
 }
 
 // GMRES helpers: partial[blk*(k) + j] = sum_i conj(V_j[i]) w[i]  for j < k
-// Each lane keeps up to IPT of its elements of w in registers while the basis streams past four vectors
-// at a time, so w is read once (per chunk of IPT grid strides) instead of once per four basis vectors;
-// with more chunks the per-vector sums accumulate in LDS.
+// Each lane keeps IPT of its elements of w in registers (the grid of red_grid gives a lane IPT = 4
+// elements up to n = 2^20) while the basis streams past four vectors at a time, so w is read once (per
+// chunk of IPT grid strides) instead of once per four basis vectors; with more chunks the per-vector sums
+// accumulate in LDS.  The loads are branch-free (indices past n clamped to n - 1, their w set to zero), so
+// the IPT x JB loads of a group are all in flight together; a bounds branch per element serialises them.
 template <typename T>
 __global__ __launch_bounds__(RED_THREADS) void k_mdot(const cplx<T>* V, int64_t ldv, int k, const cplx<T>* w, int64_t n,
                                                       c128* partial) {
-  constexpr int JB = 4, IPT = 8;
+  constexpr int JB = 4, IPT = 4;
   __shared__ double sre[JB][RED_THREADS / 64], sim[JB][RED_THREADS / 64];
   __shared__ c128 sacc[512];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -223,10 +225,14 @@ __global__ __launch_bounds__(RED_THREADS) void k_mdot(const cplx<T>* V, int64_t 
   __syncthreads();
   for (int64_t i0 = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; i0 - threadIdx.x < n; i0 += IPT * stride) {
     cplx<T> wv[IPT];
+    int64_t ic[IPT];
 #pragma unroll
     for (int t = 0; t < IPT; ++t) {
       const int64_t i = i0 + t * stride;
-      wv[t] = i < n ? w[i] : mk<T>(0, 0);
+      const bool ok = i < n;
+      ic[t] = ok ? i : n - 1;
+      const cplx<T> x = w[ic[t]];
+      wv[t] = mk<T>(ok ? x.x : (T)0, ok ? x.y : (T)0);
     }
     for (int j0 = 0; j0 < k; j0 += JB) {
       double re[JB], im[JB];
@@ -236,17 +242,18 @@ __global__ __launch_bounds__(RED_THREADS) void k_mdot(const cplx<T>* V, int64_t 
         re[q] = im[q] = 0.0;
         v[q] = V + (int64_t)min(j0 + q, k - 1) * ldv;
       }
+      cplx<T> a[IPT][JB];
+#pragma unroll
+      for (int t = 0; t < IPT; ++t)
+#pragma unroll
+        for (int q = 0; q < JB; ++q) a[t][q] = v[q][ic[t]];
 #pragma unroll
       for (int t = 0; t < IPT; ++t) {
-        const int64_t i = i0 + t * stride;
-        if (i < n) {
-          const cplx<T> b = wv[t];
+        const cplx<T> b = wv[t];
 #pragma unroll
-          for (int q = 0; q < JB; ++q) {
-            const cplx<T> a = v[q][i];
-            re[q] += (double)a.x * b.x + (double)a.y * b.y;
-            im[q] += (double)a.x * b.y - (double)a.y * b.x;
-          }
+        for (int q = 0; q < JB; ++q) {
+          re[q] += (double)a[t][q].x * b.x + (double)a[t][q].y * b.y;
+          im[q] += (double)a[t][q].x * b.y - (double)a[t][q].y * b.x;
         }
       }
 #pragma unroll
