@@ -89,6 +89,10 @@ struct vs_pc {
   int Lb = 0, Mb = 0;    // 2-level stored block grid (L x M, or L/2+1 x M/2+1 when mirrored)
   vs::DevBuf inv64, inv32;  // apply layout (2-level: [ky][r][s][kx]; 1-level: [kx][r][s])
   vs::DevBuf invq32;     // mirrored 2-level FP32 blocks for the matrix-core product: full, column-major, block kx + Lb ky
+  // mirrored 2-level: the last rem = Lb mod Wt stored columns of each precision (a layout tile that is mostly
+  // padding, rem from mirror_rem) again as full column-major blocks, block (Lb - rem + w, ky) at
+  // (ky * rem + w) * n^2; empty when not used
+  vs::DevBuf invr64, invr32;
   vs::DevBuf map3;       // own copy of the unknown map
   int64_t nunk = 0;
   uint64_t dom_id = 0;   // the operator domain the map was copied from (0: a caller's unknown set)

@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <type_traits>
 
 namespace vs {
 
@@ -649,26 +650,52 @@ __global__ __launch_bounds__(256) void k_blockmv2_sym(const cplx<T>* __restrict_
 // The rows of a tile are split over RS sibling workgroups (out of place: U -> U2); siblings get
 // consecutive logical ids, and logical ids are laid out so that consecutive ones land on the same XCD
 // (hardware dispatch sends workgroup i to XCD i mod 8), so the siblings' reads of U share that XCD's L2.
+// With rem > 0 the last rem stored columns are not taken from their (mostly padding) layout tile but from
+// Br, full column-major blocks: the workgroups after the tiles' take one such column each (W = 1, lanes
+// over rows, RSr siblings), reading consecutive rows of a block column per load.
 // Index arithmetic is 32-bit within a block and the gather runs without integer division.
 template <typename T, bool SYM, int RC>
 __global__ __launch_bounds__(512) void k_blockmv2m(const cplx<T>* __restrict__ B, const cplx<T>* __restrict__ U,
                                                    cplx<T>* __restrict__ U2, int L, int M, int Lb, int Mb, int n,
-                                                   int Wt, int lw, int RS, int64_t rhs_stride, int nrhs) {
+                                                   int Wt, int lw, int RS, int64_t rhs_stride, int nrhs,
+                                                   const cplx<T>* __restrict__ Br, int rem, int RSr) {
   constexpr int NV = 4 * RC;
   constexpr int BU = 8;
   cplx<T>* u = reinterpret_cast<cplx<T>*>(vs_smem);
-  const int W = 1 << lw;
-  const int tiles = (Lb + Wt - 1) / Wt, subs = Wt / W;
+  const int Wm = 1 << lw;
+  const int tiles = rem > 0 ? Lb / Wt : (Lb + Wt - 1) / Wt, subs = Wt / Wm;
   const int64_t G = (int64_t)tiles * subs * Mb * RS;
-  const int64_t per_xcd = (G + 7) / 8;
+  const int64_t GA = G + (int64_t)rem * Mb * RSr;
+  const int64_t per_xcd = (GA + 7) / 8;
   const int64_t lid = (int64_t)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
-  if (lid >= G) return;
-  const int rsi = (int)(lid % RS);
-  int64_t rest = lid / RS;
-  const int sub = (int)(rest % subs);
-  rest /= subs;
-  const int tile = (int)(rest % tiles), kyb = (int)(rest / tiles);
-  const int kx0 = tile * Wt + sub * W;
+  if (lid >= GA) return;
+  const int NT = SYM ? n * (n + 1) / 2 : n * n;
+  const bool rmd = lid >= G;  // a remainder column
+  int lwe, rsi, kyb, kx0, nrs;
+  const cplx<T>* Bk;
+  if (!rmd) {
+    rsi = (int)(lid % RS);
+    int64_t rest = lid / RS;
+    const int sub = (int)(rest % subs);
+    rest /= subs;
+    const int tile = (int)(rest % tiles);
+    kyb = (int)(rest / tiles);
+    kx0 = tile * Wt + sub * Wm;
+    lwe = lw;
+    nrs = RS;
+    Bk = B + ((int64_t)tile * Mb + kyb) * NT * Wt + sub * Wm;
+  } else {
+    const int64_t l2 = lid - G;
+    rsi = (int)(l2 % RSr);
+    const int64_t rest = l2 / RSr;
+    const int wr = (int)(rest % rem);
+    kyb = (int)(rest / rem);
+    kx0 = Lb - rem + wr;
+    lwe = 0;
+    nrs = RSr;
+    Bk = Br + ((int64_t)kyb * rem + wr) * n * n;
+  }
+  const int W = 1 << lwe;
   const int wv = max(0, min(W, Lb - kx0));
   const int r0 = blockIdx.y * RC;
   const int nr = min(RC, nrhs - r0);
@@ -684,7 +711,7 @@ __global__ __launch_bounds__(512) void k_blockmv2m(const cplx<T>* __restrict__ B
     const int kx = kx0 + w;
     const bool wl = w < wv;
     const bool xl = kx != 0 && 2 * kx != L;
-    for (int s = threadIdx.x >> lw; s < n; s += blockDim.x >> lw) {
+    for (int s = threadIdx.x >> lwe; s < n; s += blockDim.x >> lwe) {
       const int comp = (s >= N) + (s >= 2 * N);
       cplx<T> val[NV];
       T sg[NV];
@@ -701,74 +728,88 @@ __global__ __launch_bounds__(512) void k_blockmv2m(const cplx<T>* __restrict__ B
     }
   }
   __syncthreads();
-  const int NT = SYM ? n * (n + 1) / 2 : n * n;
   const int nz0 = 2 * N;
-  const cplx<T>* Bk = B + ((int64_t)tile * Mb + kyb) * NT * Wt + sub * W;
-  const int rows = (n + RS - 1) / RS;
+  const int rows = (n + nrs - 1) / nrs;
   const int rlo = rsi * rows, rhi = min(n, rlo + rows);
-  for (int e = threadIdx.x; e < (rhi - rlo) * W; e += blockDim.x) {
-    const int w = e & (W - 1);
-    const int r = rlo + (e >> lw);
-    if (w >= wv) continue;
-    cplx<T> acc[NV];
+  // RM: a remainder column (entry (r, s) at Bk[s n + r], no sign folding); else the tiled layout
+  auto product = [&](auto rm_tag) {
+    constexpr bool RM = decltype(rm_tag)::value;
+    for (int e = threadIdx.x; e < (rhi - rlo) * W; e += blockDim.x) {
+      const int w = e & (W - 1);
+      const int r = rlo + (e >> lwe);
+      if (w >= wv) continue;
+      cplx<T> acc[NV];
 #pragma unroll
-    for (int c = 0; c < NV; ++c) acc[c] = mk<T>(0, 0);
-    const int t0 = SYM ? r * (2 * n - r + 1) / 2 : r * n;
-    const int rz = r >= nz0;
-    // batch of BU entries s0 .. s0 + BU - 1 of row r (sign folded in, zero past n)
-    auto load_batch = [&](int s0, cplx<T> (&b)[BU], T (&sg)[BU]) {
+      for (int c = 0; c < NV; ++c) acc[c] = mk<T>(0, 0);
+      const int t0 = SYM ? r * (2 * n - r + 1) / 2 : r * n;
+      const int rz = r >= nz0;
+      // batch of BU entries s0 .. s0 + BU - 1 of row r (sign folded in, zero past n)
+      auto load_batch = [&](int s0, cplx<T> (&b)[BU], T (&sg)[BU]) {
 #pragma unroll
-      for (int j = 0; j < BU; ++j) {
-        const int s = min(s0 + j, n - 1);
-        int t;
-        T g = (T)1;
-        if (SYM && RC == 1) {
-          // s < r: transposed entry (s, r), t = s (2n - s + 1) / 2 + (r - s), negated when exactly one of r, s
-          // is a z unknown; s >= r: row r of the upper triangle, t = t0 + (s - r).  Branch-free: the terms in
-          // s are uniform over the workgroup, the choice is a mask (a branch here costs an exec-mask round
-          // per entry; measured 20.7 -> 19.4 us on the Koch prism's FP32 blocks)
-          const int lo = s < r ? -1 : 0;
-          const int tlo = s * (2 * n - s + 1) / 2 - s + r, thi = t0 - r + s;
-          t = thi + ((tlo - thi) & lo);
-          g = (T)(1 - 2 * (lo & (rz ^ (s >= nz0 ? 1 : 0)) & 1));
-        } else if (SYM) {
-          // several right-hand sides: the same choice as a branch, which the 64 multiply-adds per entry hide
-          // and which keeps the loads in the compiler's original order (the masked form measured 221 -> 260 us
-          // for 16 right-hand sides)
-          const bool lo = s < r;
-          t = lo ? s * (2 * n - s + 1) / 2 + (r - s) : t0 + (s - r);
-          if (lo && rz != (s >= nz0 ? 1 : 0)) g = (T)-1;
-        } else {
-          t = t0 + s;
+        for (int j = 0; j < BU; ++j) {
+          const int s = min(s0 + j, n - 1);
+          int t;
+          T g = (T)1;
+          if (RM) {
+            t = s * n + r;
+          } else if (SYM && RC == 1) {
+            // s < r: transposed entry (s, r), t = s (2n - s + 1) / 2 + (r - s), negated when exactly one of r, s
+            // is a z unknown; s >= r: row r of the upper triangle, t = t0 + (s - r).  Branch-free: the terms in
+            // s are uniform over the workgroup, the choice is a mask (a branch here costs an exec-mask round
+            // per entry; measured 20.7 -> 19.4 us on the Koch prism's FP32 blocks)
+            const int lo = s < r ? -1 : 0;
+            const int tlo = s * (2 * n - s + 1) / 2 - s + r, thi = t0 - r + s;
+            t = thi + ((tlo - thi) & lo);
+            g = (T)(1 - 2 * (lo & (rz ^ (s >= nz0 ? 1 : 0)) & 1));
+          } else if (SYM) {
+            // several right-hand sides: the same choice as a branch, which the 64 multiply-adds per entry hide
+            // and which keeps the loads in the compiler's original order (the masked form measured 221 -> 260 us
+            // for 16 right-hand sides)
+            const bool lo = s < r;
+            t = lo ? s * (2 * n - s + 1) / 2 + (r - s) : t0 + (s - r);
+            if (lo && rz != (s >= nz0 ? 1 : 0)) g = (T)-1;
+          } else {
+            t = t0 + s;
+          }
+          // offsets within one layout tile stay below 2^31 for n < ~16000
+          b[j] = RM ? Bk[t] : Bk[t * Wt + w];
+          sg[j] = s0 + j < n ? g : (T)0;
         }
-        // offsets within one layout tile stay below 2^31 for n < ~16000
-        b[j] = Bk[t * Wt + w];
-        sg[j] = s0 + j < n ? g : (T)0;
+      };
+      for (int s0 = 0; s0 < n; s0 += BU) {
+        cplx<T> b[BU];
+        T sg[BU];
+        load_batch(s0, b, sg);
+#pragma unroll
+        for (int j = 0; j < BU; ++j) {
+          const int s = min(s0 + j, n - 1);
+          const cplx<T> bj = sg[j] * b[j];
+#pragma unroll
+          for (int c = 0; c < NV; ++c) acc[c] = cfma(bj, u[(c * n + s) * W + w], acc[c]);
+        }
       }
-    };
-    for (int s0 = 0; s0 < n; s0 += BU) {
-      cplx<T> b[BU];
-      T sg[BU];
-      load_batch(s0, b, sg);
+      const int kx = kx0 + w;
+      const int comp = r / N;
 #pragma unroll
-      for (int j = 0; j < BU; ++j) {
-        const int s = min(s0 + j, n - 1);
-        const cplx<T> bj = sg[j] * b[j];
-#pragma unroll
-        for (int c = 0; c < NV; ++c) acc[c] = cfma(bj, u[(c * n + s) * W + w], acc[c]);
+      for (int v = 0; v < NV; ++v) {
+        const int rc = v >> 2, xm = v & 1, ym = (v >> 1) & 1;
+        const bool live = rc < nr && (!xm || (kx != 0 && 2 * kx != L)) && (!ym || ymir);
+        if (!live) continue;
+        const int gx = xm ? L - kx : kx, gy = ym ? M - kyb : kyb;
+        const bool neg = (xm && comp == 0) || (ym && comp == 1);
+        Uo[rc * rhs_stride + r * sstride + gx + (int64_t)L * gy] = neg ? -acc[v] : acc[v];
       }
     }
-    const int kx = kx0 + w;
-    const int comp = r / N;
-#pragma unroll
-    for (int v = 0; v < NV; ++v) {
-      const int rc = v >> 2, xm = v & 1, ym = (v >> 1) & 1;
-      const bool live = rc < nr && (!xm || (kx != 0 && 2 * kx != L)) && (!ym || ymir);
-      if (!live) continue;
-      const int gx = xm ? L - kx : kx, gy = ym ? M - kyb : kyb;
-      const bool neg = (xm && comp == 0) || (ym && comp == 1);
-      Uo[rc * rhs_stride + r * sstride + gx + (int64_t)L * gy] = neg ? -acc[v] : acc[v];
-    }
+  };
+  // remainder columns exist for one right-hand side only (rem = 0 otherwise), so only that instantiation
+  // carries the second role
+  if constexpr (RC == 1) {
+    if (rmd)
+      product(std::true_type{});
+    else
+      product(std::false_type{});
+  } else {
+    product(std::false_type{});
   }
 }
 
@@ -1229,6 +1270,30 @@ static unsigned nblocks(int64_t n, int b = 256) { return (unsigned)((n + b - 1) 
 // column tile of the mirrored block layout: 128 bytes of complex values
 template <typename T> constexpr int mirror_tile() { return 128 / (int)sizeof(cplx<T>); }
 
+// Stored columns of the last, mostly padded layout tile that the single-right-hand-side product takes
+// from a compact copy (full column-major blocks) instead: rem = Lb mod Wt when it is at most a quarter of
+// the tile (even L gives Lb = L/2 + 1, so rem = 1 whenever L/2 is a multiple of Wt), else 0.  In the tiled
+// layout a workgroup on that tile keeps rem of its W lanes per row busy and, with W = 1, would read one
+// useful value per 128-byte line.
+template <typename T>
+static int mirror_rem(int Lb) {
+  if (std::getenv("VS_BMM_REM") && std::getenv("VS_BMM_REM")[0] == '0') return 0;
+  const int Wt = mirror_tile<T>(), rem = Lb % Wt;
+  return (Lb > Wt && rem > 0 && 4 * rem <= Wt) ? rem : 0;
+}
+
+// the last rem stored columns of the column-major FP64 blocks (block kx + Lb ky) -> full column-major
+// blocks (ky * rem + w) in To
+template <typename To>
+__global__ void k_blocks_rem(const c128* __restrict__ in, cplx<To>* __restrict__ out, int Lb, int rem, int64_t nn,
+                             int64_t tot) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= tot) return;
+  const int64_t b = e / nn, q = e - b * nn;
+  const int64_t w = b % rem, ky = b / rem;
+  out[e] = ccast<To>(in[(ky * Lb + Lb - rem + w) * nn + q]);
+}
+
 // Toep (device, FP64) -> circ_L (device): Chan in x + FFT along x
 void build_circ_L(vs_ctx* ctx, const c128* toep, c128* cL, int L, int M, int N) {
   const int64_t V = (int64_t)L * M * N;
@@ -1336,6 +1401,15 @@ void pc_store_inverse(vs_pc* pc, const c128* inv) {
         hipLaunchKernelGGL((k_relayout2_tiled<To, false>), dim3(nblocks(tot)), dim3(256), 0, ctx->stream, inv,
                            dst.as<cplx<To>>(), pc->Lb, pc->Mb, (int)n, Wt);
       VS_HIP(hipGetLastError());
+      const int rem = mirror_rem<To>(pc->Lb);
+      DevBuf& rdst = sizeof(To) == 8 ? pc->invr64 : pc->invr32;
+      if (rem > 0) {
+        const int64_t rtot = (int64_t)rem * pc->Mb * n * n;
+        rdst.ensure(sizeof(cplx<To>) * (size_t)rtot);
+        hipLaunchKernelGGL((k_blocks_rem<To>), dim3(nblocks(rtot)), dim3(256), 0, ctx->stream, inv,
+                           rdst.as<cplx<To>>(), pc->Lb, rem, n * n, rtot);
+        VS_HIP(hipGetLastError());
+      }
       return;
     }
     const int64_t stored = (pc->levels == 2 && pc->sym) ? nblk * n * (n + 1) / 2 : tot;
@@ -1379,6 +1453,15 @@ static bool bmm_tile_use(int n, int nrhs) {
 constexpr size_t kBmmLdsMax = 160 * 1024;
 
 // forced row split of the mirrored block product (a tuning knob; 0 = automatic)
+// VS_BMM_W: at most this many stored columns per workgroup of the mirrored product (default: the tile)
+static int bmm_w_max() {
+  static const int v = [] {
+    const char* e = std::getenv("VS_BMM_W");
+    return e ? std::max(1, std::atoi(e)) : 1 << 30;
+  }();
+  return v;
+}
+
 static int bmm_rows_split() {
   static const int v = [] {
     const char* e = std::getenv("VS_BMM_RS");
@@ -1491,9 +1574,12 @@ static cplx<T>* pc_apply_impl(vs_pc* pc, const cplx<T>* x, cplx<T>* y, int nrhs,
       // RS sibling workgroups of about 256 lanes (measured best on 48-row blocks in both precisions)
       const int Wt = mirror_tile<T>();
       int lw = 0;
-      while ((2 << lw) <= Wt && (2 << lw) <= (big ? 4 : Wt) && fits(RC, 2 << lw)) ++lw;
+      while ((2 << lw) <= Wt && (2 << lw) <= (big ? 4 : Wt) && (2 << lw) <= bmm_w_max() && fits(RC, 2 << lw)) ++lw;
       const int W = 1 << lw;
-      const int tiles = (Lb + Wt - 1) / Wt, subs = Wt / W;
+      // the last rem stored columns from their compact copy (single right-hand side), when there is one
+      const cplx<T>* Br = sizeof(T) == 8 ? pc->invr64.as<cplx<T>>() : pc->invr32.as<cplx<T>>();
+      const int rem = (RC == 1 && Br) ? mirror_rem<T>(Lb) : 0;
+      const int tiles = rem > 0 ? Lb / Wt : (Lb + Wt - 1) / Wt, subs = Wt / W;
       const int chunks = (nrhs + RC - 1) / RC;
       const int rows_wg = std::max(1, 256 / W);
       int RS = bmm_rows_split() > 0 ? std::min(bmm_rows_split(), n) : (n + rows_wg - 1) / rows_wg;
@@ -1501,7 +1587,9 @@ static cplx<T>* pc_apply_impl(vs_pc* pc, const cplx<T>* x, cplx<T>* y, int nrhs,
       RS = (n + rows - 1) / rows;
       const size_t lds = sizeof(cplx<T>) * (size_t)4 * RC * n * W;
       const int threads = (int)std::min<int64_t>(512, ((int64_t)rows * W + 63) / 64 * 64);
-      const int64_t G = (int64_t)tiles * subs * Mb * RS;
+      // a remainder column's rows over as few workgroups of the same size as hold them
+      const int RSr = rem > 0 ? (n + threads - 1) / threads : 1;
+      const int64_t G = (int64_t)tiles * subs * Mb * RS + (int64_t)rem * Mb * RSr;
       dim3 grid((unsigned)(((G + 7) / 8) * 8), (unsigned)chunks);
       pc->U2.ensure(sizeof(cplx<T>) * V3 * nrhs);
       cplx<T>* U2 = pc->U2.as<cplx<T>>();
@@ -1511,7 +1599,7 @@ static cplx<T>* pc_apply_impl(vs_pc* pc, const cplx<T>* x, cplx<T>* y, int nrhs,
       VS_HIP(hipFuncSetAttribute((const void*)k_blockmv2m<T, SYMV, RCV>,                                  \
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));                  \
     hipLaunchKernelGGL((k_blockmv2m<T, SYMV, RCV>), grid, dim3(threads), lds, ctx->stream, B, U, U2, L, M, Lb, \
-                       Mb, n, Wt, lw, RS, V3, nrhs);                                                      \
+                       Mb, n, Wt, lw, RS, V3, nrhs, Br, rem, RSr);                                        \
   } while (0)
       if (pc->sym) {
         if (RC == 4) VS_BMM(true, 4); else if (RC == 2) VS_BMM(true, 2); else VS_BMM(true, 1);
