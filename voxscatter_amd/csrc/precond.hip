@@ -12,7 +12,11 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <type_traits>
+#include <vector>
 
 namespace vs {
 
@@ -1453,6 +1457,70 @@ static bool bmm_tile_use(int n, int nrhs) {
 constexpr size_t kBmmLdsMax = 160 * 1024;
 
 // forced row split of the mirrored block product (a tuning knob; 0 = automatic)
+// Row split of the mirrored block product, picked by timing: the best number of sibling workgroups per
+// layout tile depends on the block size, the precision, the right-hand sides per pass and how the
+// workgroups fill the CUs (measured: 3 for the 48-row FP32 blocks of a 128 x 128 x 16 grid with one
+// right-hand side, 1 with four per pass, 5 instead of 6 or 3 for 192-row blocks).  Each configuration is
+// timed once per process, at its first launch outside a graph capture (candidates of 16 .. n rows per
+// workgroup, four launches each on the apply's own data), and the choice is kept for every later apply of
+// that shape (whatever the number of passes over right-hand sides, which shrinks as batched systems
+// finish).  VS_AUTOTUNE=0 keeps the default split.
+struct BmmKey {
+  bool fp64, sym;
+  int n, Lb, Mb, W, RC, rem;
+  bool operator<(const BmmKey& o) const {
+    return std::tie(fp64, sym, n, Lb, Mb, W, RC, rem) < std::tie(o.fp64, o.sym, o.n, o.Lb, o.Mb, o.W, o.RC, o.rem);
+  }
+};
+
+template <typename F>
+static int bmm_tuned_rs(vs_ctx* ctx, const BmmKey& key, int RS0, F&& launch) {
+  static std::mutex mu;
+  static std::map<BmmKey, int> tuned;
+  {
+    std::lock_guard<std::mutex> g(mu);
+    auto it = tuned.find(key);
+    if (it != tuned.end()) return it->second;
+  }
+  static const bool on = [] {
+    const char* e = std::getenv("VS_AUTOTUNE");
+    return !(e && e[0] == '0');
+  }();
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  if (!on || hipStreamIsCapturing(ctx->stream, &st) != hipSuccess || st != hipStreamCaptureStatusNone) return RS0;
+  const int n = key.n;
+  std::vector<int> cand{RS0};
+  for (int rows : {16, 24, 32, 40, 48, 64, 96, 128, n}) {
+    if (rows > n || (int64_t)rows * key.W > 512) continue;
+    const int rs = (n + rows - 1) / rows;
+    const int rs_n = (n + (n + rs - 1) / rs - 1) / ((n + rs - 1) / rs);
+    if (std::find(cand.begin(), cand.end(), rs_n) == cand.end()) cand.push_back(rs_n);
+  }
+  hipEvent_t e0, e1;
+  VS_HIP(hipEventCreate(&e0));
+  VS_HIP(hipEventCreate(&e1));
+  float best = 1e30f;
+  int best_rs = RS0;
+  for (int rs : cand) {
+    launch(rs);
+    VS_HIP(hipEventRecord(e0, ctx->stream));
+    for (int r = 0; r < 3; ++r) launch(rs);
+    VS_HIP(hipEventRecord(e1, ctx->stream));
+    VS_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    VS_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best) {
+      best = ms;
+      best_rs = rs;
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  std::lock_guard<std::mutex> g(mu);
+  tuned[key] = best_rs;
+  return best_rs;
+}
+
 // VS_BMM_W: at most this many stored columns per workgroup of the mirrored product (default: the tile)
 static int bmm_w_max() {
   static const int v = [] {
@@ -1581,33 +1649,45 @@ static cplx<T>* pc_apply_impl(vs_pc* pc, const cplx<T>* x, cplx<T>* y, int nrhs,
       const int rem = (RC == 1 && Br) ? mirror_rem<T>(Lb) : 0;
       const int tiles = rem > 0 ? Lb / Wt : (Lb + Wt - 1) / Wt, subs = Wt / W;
       const int chunks = (nrhs + RC - 1) / RC;
-      const int rows_wg = std::max(1, 256 / W);
-      int RS = bmm_rows_split() > 0 ? std::min(bmm_rows_split(), n) : (n + rows_wg - 1) / rows_wg;
-      const int rows = (n + RS - 1) / RS;
-      RS = (n + rows - 1) / rows;
       const size_t lds = sizeof(cplx<T>) * (size_t)4 * RC * n * W;
-      const int threads = (int)std::min<int64_t>(512, ((int64_t)rows * W + 63) / 64 * 64);
-      // a remainder column's rows over as few workgroups of the same size as hold them
-      const int RSr = rem > 0 ? (n + threads - 1) / threads : 1;
-      const int64_t G = (int64_t)tiles * subs * Mb * RS + (int64_t)rem * Mb * RSr;
-      dim3 grid((unsigned)(((G + 7) / 8) * 8), (unsigned)chunks);
       pc->U2.ensure(sizeof(cplx<T>) * V3 * nrhs);
       cplx<T>* U2 = pc->U2.as<cplx<T>>();
+      // one launch with the rows of a block split over RS sibling workgroups (every split computes each
+      // output row in the same order, so the results do not depend on it)
+      const cplx<T>* Uin = U;
+      auto launch = [&](int RS) {
+        const int rows = (n + RS - 1) / RS;
+        RS = (n + rows - 1) / rows;
+        const int threads = (int)std::min<int64_t>(512, ((int64_t)rows * W + 63) / 64 * 64);
+        // a remainder column's rows over as few workgroups of the same size as hold them
+        const int RSr = rem > 0 ? (n + threads - 1) / threads : 1;
+        const int64_t G = (int64_t)tiles * subs * Mb * RS + (int64_t)rem * Mb * RSr;
+        dim3 grid((unsigned)(((G + 7) / 8) * 8), (unsigned)chunks);
 #define VS_BMM(SYMV, RCV)                                                                                 \
   do {                                                                                                    \
     if (lds > 65536)                                                                                      \
       VS_HIP(hipFuncSetAttribute((const void*)k_blockmv2m<T, SYMV, RCV>,                                  \
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));                  \
-    hipLaunchKernelGGL((k_blockmv2m<T, SYMV, RCV>), grid, dim3(threads), lds, ctx->stream, B, U, U2, L, M, Lb, \
-                       Mb, n, Wt, lw, RS, V3, nrhs, Br, rem, RSr);                                        \
+    hipLaunchKernelGGL((k_blockmv2m<T, SYMV, RCV>), grid, dim3(threads), lds, ctx->stream, B, Uin, U2, L, M, \
+                       Lb, Mb, n, Wt, lw, RS, V3, nrhs, Br, rem, RSr);                                    \
   } while (0)
-      if (pc->sym) {
-        if (RC == 4) VS_BMM(true, 4); else if (RC == 2) VS_BMM(true, 2); else VS_BMM(true, 1);
-      } else {
-        if (RC == 4) VS_BMM(false, 4); else if (RC == 2) VS_BMM(false, 2); else VS_BMM(false, 1);
-      }
+        if (pc->sym) {
+          if (RC == 4) VS_BMM(true, 4); else if (RC == 2) VS_BMM(true, 2); else VS_BMM(true, 1);
+        } else {
+          if (RC == 4) VS_BMM(false, 4); else if (RC == 2) VS_BMM(false, 2); else VS_BMM(false, 1);
+        }
 #undef VS_BMM
-      VS_HIP(hipGetLastError());
+        VS_HIP(hipGetLastError());
+      };
+      const int rows_wg = std::max(1, 256 / W);
+      int RS = (n + rows_wg - 1) / rows_wg;
+      if (bmm_rows_split() > 0) {
+        RS = std::min(bmm_rows_split(), n);
+      } else {
+        const BmmKey key{sizeof(T) == 8, pc->sym, n, Lb, Mb, W, RC, rem};
+        RS = bmm_tuned_rs(ctx, key, RS, launch);
+      }
+      launch(RS);
       U = U2;
       }
     } else {
