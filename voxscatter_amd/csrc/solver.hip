@@ -20,6 +20,9 @@ namespace vs {
 
 constexpr int RED_THREADS = 256;
 constexpr int RED_MAX_K = 8;
+#ifndef VS_MDOT_JB
+#define VS_MDOT_JB 4
+#endif
 
 static int red_grid(int64_t n) {
   int64_t g = (n + RED_THREADS * 4 - 1) / (RED_THREADS * 4);
@@ -209,20 +212,20 @@ __global__ __launch_bounds__(RED_THREADS) void k_update(int op, UpdArgs<T> a, in
 
 // GMRES helpers: partial[blk*(k) + j] = sum_i conj(V_j[i]) w[i]  for j < k
 // Each lane keeps IPT of its elements of w in registers (the grid of red_grid gives a lane IPT = 4
-// elements up to n = 2^20) while the basis streams past four vectors at a time, so w is read once (per
-// chunk of IPT grid strides) instead of once per four basis vectors; with more chunks the per-vector sums
-// accumulate in LDS.  The loads are branch-free (indices past n clamped to n - 1, their w set to zero), so
-// the IPT x JB loads of a group are all in flight together; a bounds branch per element serialises them.
+// elements up to n = 2^20) while the basis streams past JB vectors at a time, so w is read once (per
+// chunk of IPT grid strides) instead of once per JB basis vectors.  The loads are branch-free (indices
+// past n clamped to n - 1, their w set to zero), so the IPT x JB loads of a group are all in flight
+// together; a bounds branch per element serialises them.  Each wavefront reduces its sums across lanes
+// and accumulates them in LDS slots of its own (no barrier between groups); the wavefronts' slots are
+// added in order at the end.
 template <typename T>
 __global__ __launch_bounds__(RED_THREADS) void k_mdot(const cplx<T>* V, int64_t ldv, int k, const cplx<T>* w, int64_t n,
                                                       c128* partial) {
-  constexpr int JB = 4, IPT = 4;
-  __shared__ double sre[JB][RED_THREADS / 64], sim[JB][RED_THREADS / 64];
-  __shared__ c128 sacc[512];
+  constexpr int JB = VS_MDOT_JB, IPT = 4, NW = RED_THREADS / 64;
+  __shared__ c128 sacc[NW][512];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int64_t stride = (int64_t)gridDim.x * RED_THREADS;
-  for (int j = threadIdx.x; j < k; j += RED_THREADS) sacc[j] = mk<double>(0, 0);
-  __syncthreads();
+  for (int j = lane; j < k; j += 64) sacc[wave][j] = mk<double>(0, 0);
   for (int64_t i0 = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; i0 - threadIdx.x < n; i0 += IPT * stride) {
     cplx<T> wv[IPT];
     int64_t ic[IPT];
@@ -262,25 +265,25 @@ __global__ __launch_bounds__(RED_THREADS) void k_mdot(const cplx<T>* V, int64_t 
           re[q] += __shfl_xor(re[q], off);
           im[q] += __shfl_xor(im[q], off);
         }
-        if (lane == 0) {
-          sre[q][wave] = re[q];
-          sim[q][wave] = im[q];
-        }
       }
-      __syncthreads();
-      if (threadIdx.x < JB && j0 + (int)threadIdx.x < k) {
-        const int q = threadIdx.x;
-        double a = 0, b = 0;
-        for (int r = 0; r < RED_THREADS / 64; ++r) {
-          a += sre[q][r];
-          b += sim[q][r];
+      // lane q < JB (j0 + q < k) adds sum q to this wavefront's slot; every lane holds every sum
+      double mr = re[0], mi = im[0];
+#pragma unroll
+      for (int q = 1; q < JB; ++q)
+        if (lane == q) {
+          mr = re[q];
+          mi = im[q];
         }
-        sacc[j0 + q] = sacc[j0 + q] + mk<double>(a, b);
-      }
-      __syncthreads();
+      if (lane < JB && j0 + lane < k) sacc[wave][j0 + lane] = sacc[wave][j0 + lane] + mk<double>(mr, mi);
     }
   }
-  for (int j = threadIdx.x; j < k; j += RED_THREADS) partial[(int64_t)blockIdx.x * k + j] = sacc[j];
+  __syncthreads();
+  for (int j = threadIdx.x; j < k; j += RED_THREADS) {
+    c128 a = sacc[0][j];
+#pragma unroll
+    for (int r = 1; r < NW; ++r) a = a + sacc[r][j];
+    partial[(int64_t)blockIdx.x * k + j] = a;
+  }
 }
 
 // w += sgn * sum_j V_j h_j  (h in FP64 on the device)
