@@ -1054,15 +1054,21 @@ __global__ __launch_bounds__((FzCfg<T, N2>::THREADS)) void k_fz2(const cplx<T>* 
   const int kx0 = tile * W;
   const bool live = kx0 + col < L2;
   cplx<T>* Sb = S + (int64_t)blockIdx.y * S_rhs;
-  const int64_t zs = (int64_t)L2 * M2, cs = zs * N;
-  cplx<T>* gcol = Sb + c * cs + (int64_t)ky * L2 + kx0 + col;
+  // element offsets within one right-hand side's volume (2L x 2M x N x 3 < 2^31) in 32 bits; N = N2 / 2,
+  // so with R1 | N the padding (n >= N) is the inputs n2 >= N / R1 of every thread, known at compile time
+  const unsigned zs = (unsigned)L2 * (unsigned)M2, cs = zs * (unsigned)NH;
+  cplx<T>* gcol = Sb + (c * cs + (unsigned)ky * (unsigned)L2 + (unsigned)(kx0 + col));
+  constexpr bool RZ = NH % R1 == 0;
   // forward transform of the three components
   if (s < R1) {
     cplx<T> v[R2];
 #pragma unroll
     for (int n2 = 0; n2 < R2; ++n2) {
       const int n = s + R1 * n2;
-      v[n2] = (live && n < N) ? gcol[n * zs] : mk<T>(0, 0);
+      if (RZ && R1 * n2 >= NH)
+        v[n2] = mk<T>(0, 0);
+      else
+        v[n2] = (live && n < NH) ? gcol[(unsigned)n * zs] : mk<T>(0, 0);
     }
     fft2_step1<T, N2, false>(v, s, tw);
 #pragma unroll
@@ -1126,7 +1132,8 @@ __global__ __launch_bounds__((FzCfg<T, N2>::THREADS)) void k_fz2(const cplx<T>* 
 #pragma unroll
     for (int k1 = 0; k1 < R1; ++k1) {
       const int k = s + R2 * k1;
-      if (live && k < N) gcol[k * zs] = u[k1];
+      if ((NH % R2 == 0 && R2 * k1 >= NH) || !(live && k < NH)) continue;
+      gcol[(unsigned)k * zs] = u[k1];
     }
   }
 }
