@@ -1075,8 +1075,8 @@ double time_on_stream(vs_ctx* ctx, int nrep, F&& body) {
   VS_HIP(hipEventSynchronize(e1));
   float ms = 0;
   VS_HIP(hipEventElapsedTime(&ms, e0, e1));
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   return (double)ms / nrep;
 }
 }  // namespace

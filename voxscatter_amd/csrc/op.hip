@@ -2516,8 +2516,8 @@ static void autotune_prec(vs_op* op) {
       best_v = v;
     }
   }
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   op->mv_variant[slot] = best_v;
 }
 

@@ -1514,8 +1514,8 @@ static int bmm_tuned_rs(vs_ctx* ctx, const BmmKey& key, int RS0, F&& launch) {
       best_rs = rs;
     }
   }
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   std::lock_guard<std::mutex> g(mu);
   tuned[key] = best_rs;
   return best_rs;

@@ -207,11 +207,11 @@ void ctx_release(vs_ctx* ctx);
 struct DeviceGuard {
   int prev = -1;
   explicit DeviceGuard(int dev) {
-    hipGetDevice(&prev);
-    if (prev != dev) hipSetDevice(dev);
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != dev) (void)hipSetDevice(dev);
   }
   ~DeviceGuard() {
-    if (prev >= 0) hipSetDevice(prev);
+    if (prev >= 0) (void)hipSetDevice(prev);
   }
 };
 
