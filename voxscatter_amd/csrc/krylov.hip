@@ -448,8 +448,10 @@ SolveResult solve_gmres(Engine<T>& E, const LinOp<T>& op, const cplx<T>* b, cplx
       c128* h2d = h1d + (k + 1);
       c128* nrd = h2d + (k + 1);
       E.mdot_dev(Vb[0], n, k + 1, w, h1d);
-      E.gemv_acc_dev(Vb[0], n, k + 1, h1d, -1.0, w);
-      E.mdot_dev(Vb[0], n, k + 1, w, h2d);
+      if (!E.gemv_mdot_dev(Vb[0], n, k + 1, h1d, -1.0, w, h2d)) {
+        E.gemv_acc_dev(Vb[0], n, k + 1, h1d, -1.0, w);
+        E.mdot_dev(Vb[0], n, k + 1, w, h2d);
+      }
       E.gemv_acc_dev(Vb[0], n, k + 1, h2d, -1.0, w);
       E.norm2_dev(w, nrd);
       E.scale_inv_norm_dev(w, Vb[k + 1], nrd);

@@ -300,6 +300,117 @@ __global__ __launch_bounds__(RED_THREADS) void k_gemv_acc(const cplx<T>* V, int6
   }
 }
 
+// The middle two passes of CGS2 in one read of the basis: w1 = w + sgn V h (h in FP64 on the device, cast
+// like k_gemv_acc), stored over w, and partial[blk*k + j] = sum_i conj(V_j[i]) w1[i] (the layout of
+// k_mdot).  A workgroup stages a tile of GE elements of all k vectors in LDS (read once from memory, row
+// pitch GE + 1), forms w1 of the tile (RED_THREADS / GE lanes per element over interleaved vectors, added
+// in a fixed order), then takes the dot products from LDS, two threads per vector, summed in FP64 over the
+// workgroup's tiles.  The next tile's loads (KB vectors' worth per lane, k <= KB, indices clamped) are
+// issued before the current tile is multiplied and land in LDS after it.
+extern __shared__ __align__(16) unsigned char vs_gm_smem[];
+
+template <typename T>
+struct GmCfg {
+  static constexpr int GE = sizeof(T) == 8 ? 32 : 64;  // elements per tile
+  static constexpr int KMAX = RED_THREADS / 2;
+  // dynamic LDS of a launch over k vectors (the tile and w1)
+  static constexpr size_t lds(int k) { return sizeof(cplx<T>) * ((size_t)k * (GE + 1) + GE + RED_THREADS); }
+};
+
+template <typename T, int KB>
+__global__ __launch_bounds__(RED_THREADS) void k_gemv_mdot(const cplx<T>* __restrict__ V, int64_t ldv, int k,
+                                                           const c128* __restrict__ h, double sgn, cplx<T>* w,
+                                                           int64_t n, c128* partial) {
+  using C = GmCfg<T>;
+  constexpr int GE = C::GE, GP = GE + 1, HALFE = GE / 2;
+  constexpr int PER = (KB * GE + RED_THREADS - 1) / RED_THREADS;  // staged values per lane and tile
+  static_assert(KB <= C::KMAX, "two threads per vector");
+  cplx<T>* tile = reinterpret_cast<cplx<T>*>(vs_gm_smem);  // [k][GP]
+  cplx<T>* sw = tile + (size_t)k * GP;                       // w1 of the tile
+  cplx<T>* sp = sw + GE;                                     // partial sums of w1, [RED_THREADS / GE][GE]
+  __shared__ cplx<T> hs[KB];
+  const int t = threadIdx.x;
+  for (int j = t; j < k; j += RED_THREADS) hs[j] = ccast<T>(sgn * h[j]);
+  const int jd = t >> 1, part = t & 1;  // dot phase: vector jd, elements part * GE/2 .. + GE/2
+  double are = 0.0, aim = 0.0;
+  const int64_t ntiles = (n + GE - 1) / GE;
+  // staging: GE divides RED_THREADS, so lane t always takes element t % GE of a tile, of vectors
+  // t / GE + q * (RED_THREADS / GE) (those past k clamped to k - 1)
+  constexpr int JS = RED_THREADS / GE;
+  const int es = t % GE, js = t / GE;
+  // the prefetched values as separate real and imaginary arrays (an array of complex structs carried
+  // across the loop is kept in scratch memory by the compiler in FP64)
+  T pvr[PER], pvi[PER];
+  cplx<T> pw = mk<T>(0, 0);
+  int64_t tl = blockIdx.x;
+  if (tl < ntiles) {
+    const cplx<T>* vb = V + min(tl * GE + es, n - 1);
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const cplx<T> x = vb[(int64_t)min(js + q * JS, k - 1) * ldv];
+      pvr[q] = x.x;
+      pvi[q] = x.y;
+    }
+    if (t < GE) pw = w[min(tl * GE + t, n - 1)];
+  }
+  for (; tl < ntiles; tl += gridDim.x) {
+    const int64_t e0 = tl * GE;
+    __syncthreads();  // the previous tile's dot phase has read the LDS
+    // rows past k - 1 were loaded from row k - 1 and are stored there again (the same value), so that the
+    // stores need no branch (a branch per store moves the prefetched values to scratch memory in FP64)
+#pragma unroll
+    for (int q = 0; q < PER; ++q) tile[min(js + q * JS, k - 1) * GP + es] = mk<T>(pvr[q], pvi[q]);
+    const cplx<T> wv = pw;
+    __syncthreads();
+    const int64_t tn = tl + gridDim.x;
+    if (tn < ntiles) {
+      const cplx<T>* vb = V + min(tn * GE + es, n - 1);
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+        const cplx<T> x = vb[(int64_t)min(js + q * JS, k - 1) * ldv];
+        pvr[q] = x.x;
+        pvi[q] = x.y;
+      }
+      if (t < GE) pw = w[min(tn * GE + t, n - 1)];
+    }
+    {
+      // w1: the RED_THREADS / GE lanes of element es take the vectors j = js (mod JS) in order; their sums
+      // are added in a fixed order
+      cplx<T> acc = mk<T>(0, 0);
+      for (int j = js; j < k; j += JS) acc = acc + tile[j * GP + es] * hs[j];
+      sp[js * GE + es] = acc;
+    }
+    __syncthreads();
+    if (t < GE) {
+      cplx<T> acc = wv;
+#pragma unroll
+      for (int q = 0; q < JS; ++q) acc = acc + sp[q * GE + t];
+      const bool ok = e0 + t < n;
+      if (ok) w[e0 + t] = acc;
+      sw[t] = ok ? acc : mk<T>(0, 0);
+    }
+    __syncthreads();
+    if (jd < k) {
+      const cplx<T>* row = tile + jd * GP + part * HALFE;
+      const cplx<T>* wr = sw + part * HALFE;
+      // four independent partial sums per lane (the additions of one chain wait on each other)
+      double pr[4] = {0, 0, 0, 0}, pi[4] = {0, 0, 0, 0};
+#pragma unroll
+      for (int e = 0; e < HALFE; ++e) {
+        const cplx<T> a = row[e], b = wr[e];
+        pr[e & 3] += (double)a.x * b.x + (double)a.y * b.y;
+        pi[e & 3] += (double)a.x * b.y - (double)a.y * b.x;
+      }
+      are += (pr[0] + pr[1]) + (pr[2] + pr[3]);
+      aim += (pi[0] + pi[1]) + (pi[2] + pi[3]);
+    }
+  }
+  // the two halves of each vector are neighbouring lanes
+  are += __shfl_xor(are, 1);
+  aim += __shfl_xor(aim, 1);
+  if (part == 0 && jd < k) partial[(int64_t)blockIdx.x * k + jd] = mk<double>(are, aim);
+}
+
 // w *= 1 / sqrt(nrm2[0].x) into y (Arnoldi normalisation with the norm left on the device)
 template <typename T>
 __global__ __launch_bounds__(RED_THREADS) void k_scale_inv_norm(const cplx<T>* w, cplx<T>* y, int64_t n,
@@ -397,6 +508,44 @@ void Engine<T>::mdot_dev(const cplx<T>* V, int64_t ldv, int k, const cplx<T>* w,
   hipLaunchKernelGGL(k_reduce_partials, dim3(reduce_grid(k)), dim3(RED_THREADS), 0, ctx->stream, ctx->red.as<c128>(),
                      grid, k, out_dev);
   VS_HIP(hipGetLastError());
+}
+
+template <typename T>
+bool Engine<T>::gemv_mdot_dev(const cplx<T>* V, int64_t ldv, int k, const c128* h_dev, double sgn, cplx<T>* w,
+                              c128* out_dev) {
+  static const bool on = [] {
+    const char* e = std::getenv("VS_GMDOT");
+    return !(e && e[0] == '0');
+  }();
+  if (!on || k > GmCfg<T>::KMAX) return false;
+  static const bool attr = [] {
+    const int l = (int)GmCfg<T>::lds(GmCfg<T>::KMAX);
+    return hipFuncSetAttribute((const void*)k_gemv_mdot<T, 32>, hipFuncAttributeMaxDynamicSharedMemorySize, l) ==
+               hipSuccess &&
+           hipFuncSetAttribute((const void*)k_gemv_mdot<T, 64>, hipFuncAttributeMaxDynamicSharedMemorySize, l) ==
+               hipSuccess &&
+           hipFuncSetAttribute((const void*)k_gemv_mdot<T, 128>, hipFuncAttributeMaxDynamicSharedMemorySize, l) ==
+               hipSuccess;
+  }();
+  VS_CHECK(attr, VS_ERR_HIP, "cannot raise the LDS limit of the fused Gram-Schmidt pass");
+  // workgroups: up to six resident per CU, each over several tiles; the partials (gm * k) stay within the
+  // reduction buffer (1024 x 512)
+  const int64_t ntiles = (n + GmCfg<T>::GE - 1) / GmCfg<T>::GE;
+  const int gm = (int)std::max<int64_t>(1, std::min<int64_t>({ntiles, 1536, (int64_t)1024 * 512 / k}));
+  const size_t lds = GmCfg<T>::lds(k);
+  c128* part = ctx->red.as<c128>();
+  if (k <= 32)
+    hipLaunchKernelGGL((k_gemv_mdot<T, 32>), dim3(gm), dim3(RED_THREADS), lds, ctx->stream, V, ldv, k, h_dev, sgn, w,
+                       n, part);
+  else if (k <= 64)
+    hipLaunchKernelGGL((k_gemv_mdot<T, 64>), dim3(gm), dim3(RED_THREADS), lds, ctx->stream, V, ldv, k, h_dev, sgn, w,
+                       n, part);
+  else
+    hipLaunchKernelGGL((k_gemv_mdot<T, 128>), dim3(gm), dim3(RED_THREADS), lds, ctx->stream, V, ldv, k, h_dev, sgn,
+                       w, n, part);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(reduce_grid(k)), dim3(RED_THREADS), 0, ctx->stream, part, gm, k, out_dev);
+  VS_HIP(hipGetLastError());
+  return true;
 }
 
 template <typename T>

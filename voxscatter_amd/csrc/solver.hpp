@@ -26,6 +26,10 @@ struct Engine {
   // variants that leave their results on the device (no host round trip)
   void mdot_dev(const cplx<T>* V, int64_t ldv, int k, const cplx<T>* w, c128* out_dev);
   void gemv_acc_dev(const cplx<T>* V, int64_t ldv, int k, const c128* h_dev, double sgn, cplx<T>* w);
+  // w += sgn V h and out_dev = V^H w (of the updated w) in one pass over V; false (nothing launched) when k
+  // exceeds the fused pass's limit or VS_GMDOT=0
+  bool gemv_mdot_dev(const cplx<T>* V, int64_t ldv, int k, const c128* h_dev, double sgn, cplx<T>* w,
+                     c128* out_dev);
   void norm2_dev(const cplx<T>* x, c128* out_dev);
   void scale_inv_norm_dev(const cplx<T>* w, cplx<T>* y, const c128* nrm2_dev);
   void read(const c128* dev, int K, c128* host);
