@@ -452,8 +452,7 @@ SolveResult solve_gmres(Engine<T>& E, const LinOp<T>& op, const cplx<T>* b, cplx
         E.gemv_acc_dev(Vb[0], n, k + 1, h1d, -1.0, w);
         E.mdot_dev(Vb[0], n, k + 1, w, h2d);
       }
-      E.gemv_acc_dev(Vb[0], n, k + 1, h2d, -1.0, w);
-      E.norm2_dev(w, nrd);
+      E.gemv_acc_norm2_dev(Vb[0], n, k + 1, h2d, -1.0, w, nrd);
       E.scale_inv_norm_dev(w, Vb[k + 1], nrd);
       std::vector<c128> hh(2 * (k + 1) + 1);
       E.read(h1d, 2 * (k + 1) + 1, hh.data());

@@ -286,17 +286,25 @@ __global__ __launch_bounds__(RED_THREADS) void k_mdot(const cplx<T>* V, int64_t 
   }
 }
 
-// w += sgn * sum_j V_j h_j  (h in FP64 on the device)
+// w += sgn * sum_j V_j h_j  (h in FP64 on the device); with partial != nullptr also one partial of
+// ||w||^2 (of the updated w) per workgroup
 template <typename T>
 __global__ __launch_bounds__(RED_THREADS) void k_gemv_acc(const cplx<T>* V, int64_t ldv, int k, const c128* h, double sgn,
-                                                          cplx<T>* w, int64_t n) {
+                                                          cplx<T>* w, int64_t n, c128* partial = nullptr) {
   __shared__ cplx<T> hs[512];
   for (int j = threadIdx.x; j < k; j += RED_THREADS) hs[j] = ccast<T>(sgn * h[j]);
   __syncthreads();
+  double re[1] = {0.0}, im[1] = {0.0};
   for (int64_t i = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * RED_THREADS) {
     cplx<T> acc = w[i];
     for (int j = 0; j < k; ++j) acc = acc + V[(int64_t)j * ldv + i] * hs[j];
     w[i] = acc;
+    re[0] += (double)acc.x * acc.x + (double)acc.y * acc.y;
+  }
+  if (partial) {
+    c128 tot[1];
+    block_sum<1>(re, im, tot);
+    if (threadIdx.x == 0) partial[blockIdx.x] = tot[0];
   }
 }
 
@@ -305,13 +313,14 @@ __global__ __launch_bounds__(RED_THREADS) void k_gemv_acc(const cplx<T>* V, int6
 // k_mdot).  A workgroup stages a tile of GE elements of all k vectors in LDS (read once from memory, row
 // pitch GE + 1), forms w1 of the tile (RED_THREADS / GE lanes per element over interleaved vectors, added
 // in a fixed order), then takes the dot products from LDS, two threads per vector, summed in FP64 over the
-// workgroup's tiles.  The next tile's loads (KB vectors' worth per lane, k <= KB, indices clamped) are
-// issued before the current tile is multiplied and land in LDS after it.
+// workgroup's tiles.  The next tile's loads (KB vectors' worth per lane, k <= KB, indices clamped; KB in
+// 32 / 64 / 96 / 128 bounds the registers they take) are issued before the current tile is multiplied and
+// land in LDS after it.
 extern __shared__ __align__(16) unsigned char vs_gm_smem[];
 
 template <typename T>
 struct GmCfg {
-  static constexpr int GE = sizeof(T) == 8 ? 32 : 64;  // elements per tile
+  static constexpr int GE = 32;  // elements per tile (64 in FP32 measured 3.7 % slower: more prefetch registers)
   static constexpr int KMAX = RED_THREADS / 2;
   // dynamic LDS of a launch over k vectors (the tile and w1)
   static constexpr size_t lds(int k) { return sizeof(cplx<T>) * ((size_t)k * (GE + 1) + GE + RED_THREADS); }
@@ -524,6 +533,8 @@ bool Engine<T>::gemv_mdot_dev(const cplx<T>* V, int64_t ldv, int k, const c128* 
                hipSuccess &&
            hipFuncSetAttribute((const void*)k_gemv_mdot<T, 64>, hipFuncAttributeMaxDynamicSharedMemorySize, l) ==
                hipSuccess &&
+           hipFuncSetAttribute((const void*)k_gemv_mdot<T, 96>, hipFuncAttributeMaxDynamicSharedMemorySize, l) ==
+               hipSuccess &&
            hipFuncSetAttribute((const void*)k_gemv_mdot<T, 128>, hipFuncAttributeMaxDynamicSharedMemorySize, l) ==
                hipSuccess;
   }();
@@ -540,6 +551,9 @@ bool Engine<T>::gemv_mdot_dev(const cplx<T>* V, int64_t ldv, int k, const c128* 
   else if (k <= 64)
     hipLaunchKernelGGL((k_gemv_mdot<T, 64>), dim3(gm), dim3(RED_THREADS), lds, ctx->stream, V, ldv, k, h_dev, sgn, w,
                        n, part);
+  else if (k <= 96)
+    hipLaunchKernelGGL((k_gemv_mdot<T, 96>), dim3(gm), dim3(RED_THREADS), lds, ctx->stream, V, ldv, k, h_dev, sgn, w,
+                       n, part);
   else
     hipLaunchKernelGGL((k_gemv_mdot<T, 128>), dim3(gm), dim3(RED_THREADS), lds, ctx->stream, V, ldv, k, h_dev, sgn,
                        w, n, part);
@@ -551,6 +565,16 @@ bool Engine<T>::gemv_mdot_dev(const cplx<T>* V, int64_t ldv, int k, const c128* 
 template <typename T>
 void Engine<T>::gemv_acc_dev(const cplx<T>* V, int64_t ldv, int k, const c128* h_dev, double sgn, cplx<T>* w) {
   hipLaunchKernelGGL((k_gemv_acc<T>), dim3(grid), dim3(RED_THREADS), 0, ctx->stream, V, ldv, k, h_dev, sgn, w, n);
+  VS_HIP(hipGetLastError());
+}
+
+template <typename T>
+void Engine<T>::gemv_acc_norm2_dev(const cplx<T>* V, int64_t ldv, int k, const c128* h_dev, double sgn, cplx<T>* w,
+                                   c128* nrm2_dev) {
+  hipLaunchKernelGGL((k_gemv_acc<T>), dim3(grid), dim3(RED_THREADS), 0, ctx->stream, V, ldv, k, h_dev, sgn, w, n,
+                     ctx->red.as<c128>());
+  hipLaunchKernelGGL(k_reduce_partials, dim3(reduce_grid(1)), dim3(RED_THREADS), 0, ctx->stream, ctx->red.as<c128>(),
+                     grid, 1, nrm2_dev);
   VS_HIP(hipGetLastError());
 }
 

@@ -31,6 +31,9 @@ struct Engine {
   bool gemv_mdot_dev(const cplx<T>* V, int64_t ldv, int k, const c128* h_dev, double sgn, cplx<T>* w,
                      c128* out_dev);
   void norm2_dev(const cplx<T>* x, c128* out_dev);
+  // gemv_acc_dev with ||w||^2 of the updated w in nrm2_dev (one pass)
+  void gemv_acc_norm2_dev(const cplx<T>* V, int64_t ldv, int k, const c128* h_dev, double sgn, cplx<T>* w,
+                          c128* nrm2_dev);
   void scale_inv_norm_dev(const cplx<T>* w, cplx<T>* y, const c128* nrm2_dev);
   void read(const c128* dev, int K, c128* host);
   // Y(:, j) (+)= sum_i X(:, i) P(i, j), P q x k column-major on the host, columns of X / Y at stride ldx / ldy
