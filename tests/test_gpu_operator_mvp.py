@@ -136,7 +136,7 @@ def test_mvp_full_size_linearity_and_adjoint(ctx):
     assert abs(np.vdot(y, Ax) - np.vdot(AHy, x)) <= 1e-12 * abs(np.vdot(y, Ax))
 
 
-@pytest.mark.parametrize("shape", [(24, 24, 24), (32, 32, 32), (64, 64, 16), (32, 32, 8), (5, 16, 4)])
+@pytest.mark.parametrize("shape", [(24, 24, 24), (32, 32, 32), (64, 64, 16), (32, 32, 8), (5, 16, 4), (64, 64, 64)])
 @pytest.mark.parametrize("transp", ["notransp", "transp"])
 def test_fused_slab_mvp_matches_oracle_and_unfused(ctx, shape, transp, monkeypatch):
     """Grids whose shape has fused slab kernels: every matvec variant (five passes, padded slab, parity

@@ -1390,13 +1390,15 @@ __global__ __launch_bounds__(1024) void k_yz(const cplx<T>* __restrict__ twy, co
 // where one slab's workgroup alone left the CU half idle); each writes its term of the cropped inverse
 // (IDFT_M(X_even), resp. conj(w_2M^y) IDFT_M(X_odd)) to a buffer of its own, and the inverse x pass adds
 // the two when it loads.
-#define VS_YZQ_SHAPES(X) X(128, 16) X(64, 16) X(128, 8) X(64, 8) X(32, 8) X(32, 16) X(16, 8) X(16, 4)
+#define VS_YZQ_SHAPES(X) X(128, 16) X(64, 16) X(128, 8) X(64, 8) X(32, 8) X(32, 16) X(16, 8) X(16, 4) X(64, 64)
 
 template <typename T, int M, int N>
 struct YzqCfg {
   static constexpr int NL = 3 * N, NLP = NL + 1;
   static constexpr int RY1 = Split<M>::R1, RY2 = Split<M>::R2;
-  static constexpr int TH = 512;
+  // long z columns (N > 16, the spectrum of a slab alone fills most of the LDS, one workgroup per CU): 1024
+  // lanes, four wavefronts per SIMD, and half the registers per lane for the spectrum copies
+  static constexpr int TH = N > 16 ? 1024 : 512;
   static constexpr int IY1 = (NL * RY1 + TH - 1) / TH, IY2 = (NL * RY2 + TH - 1) / TH;
   static constexpr int IS = (NL * M + TH - 1) / TH;  // slab values per thread
   static constexpr int IZ = (M * N + TH - 1) / TH;   // tensor-product points per thread
@@ -1418,7 +1420,7 @@ __device__ __forceinline__ int opaque(int v) {
 }
 
 template <typename T, int M, int N, bool HALF>
-__global__ __launch_bounds__(512, HALF && sizeof(T) == 4 ? 4 : 2) void k_yzq(const cplx<T>* __restrict__ twm, const cplx<T>* __restrict__ tw2m,
+__global__ __launch_bounds__((YzqCfg<T, M, N>::TH), (HALF && sizeof(T) == 4 && N <= 16 ? 4 : 2)) void k_yzq(const cplx<T>* __restrict__ twm, const cplx<T>* __restrict__ tw2m,
                                              const cplx<T>* __restrict__ tw2n, cplx<T>* S, TensorQ<T> fN,
                                              int64_t S_rhs, cplx<T>* __restrict__ Se, cplx<T>* __restrict__ So) {
   using C = YzqCfg<T, M, N>;
@@ -1540,7 +1542,7 @@ __global__ __launch_bounds__(512, HALF && sizeof(T) == 4 ? 4 : 2) void k_yzq(con
     }
     __syncthreads();
     // 3. z stage: thread (j, c) transforms its column both ways; the tensor product runs per kz parity
-    {
+    if constexpr (N <= 16) {
       const bool zt = t < 3 * M;
       const int j = t % M, c = t / M;
       cplx<T>* col = sp + j * NLP + c * N;
@@ -1589,6 +1591,84 @@ __global__ __launch_bounds__(512, HALF && sizeof(T) == 4 ? 4 : 2) void k_yzq(con
         Bfly<T, N>::run(o, true);
 #pragma unroll
         for (int z = 0; z < N; ++z) col[z] = a[z] + cconj(l2n[z]) * o[z];
+      }
+      __syncthreads();
+    }
+    else {
+      // long z columns (N = M): the 3M columns of N values stay in LDS and are transformed there (two-step
+      // DFT_N), each kz parity in turn; every thread keeps its IZS values of the untransformed spectrum (for
+      // the odd parity's twisted input) and of the first parity's result in registers.  Value e = (3 j + c) N + z sits at
+      // sp[j NLP + c N + z] = sp[e + e / (3 N)].
+      static_assert(N == M, "the DFT_N twiddles are the DFT_M table");
+      constexpr int NC = 3 * M, RZ1 = Split<N>::R1, RZ2 = Split<N>::R2;
+      constexpr int IZS = (NC * N + TH - 1) / TH;
+      cplx<T> orig[IZS], res[IZS];
+#pragma unroll
+      for (int i = 0; i < IZS; ++i) {
+        const int e = min(t + i * TH, NC * N - 1);
+        orig[i] = sp[e + e / (3 * N)];
+      }
+      __syncthreads();  // every copy is taken before the transforms overwrite the spectrum
+      // DFT_N of every column in place: the TPC lanes of a column are consecutive lanes of one wavefront,
+      // so its step-1 reads precede the exchange writes and those precede the step-2 reads in program
+      // order (no barrier); the columns go in rounds of TH / TPC
+      constexpr int TPC = cmax(RZ1, RZ2);
+      static_assert(64 % TPC == 0, "a column's lanes within one wavefront");
+      auto zdft = [&](auto inv_tag) {
+        constexpr bool INV = decltype(inv_tag)::value;
+#pragma unroll 1
+        for (int base = 0; base < NC * TPC; base += TH) {
+          const int it = base + t;
+          const int col = min(it / TPC, NC - 1), s = it % TPC;
+          const bool live = it < NC * TPC;
+          cplx<T>* cz = sp + (col / 3) * NLP + (col % 3) * N;
+          cplx<T> v[RZ2];
+          if (live && s < RZ1) {
+#pragma unroll
+            for (int n2 = 0; n2 < RZ2; ++n2) v[n2] = cz[s + RZ1 * n2];
+            fft2_step1<T, N, INV>(v, s, lm);
+#pragma unroll
+            for (int k2 = 0; k2 < RZ2; ++k2) cz[s * RZ2 + k2] = v[k2];
+          }
+          cplx<T> u[RZ1];
+          if (live && s < RZ2) {
+#pragma unroll
+            for (int n1 = 0; n1 < RZ1; ++n1) u[n1] = cz[n1 * RZ2 + s];
+            fft2_step2<T, N, INV>(u);
+#pragma unroll
+            for (int k1 = 0; k1 < RZ1; ++k1) cz[s + RZ2 * k1] = u[k1];
+          }
+        }
+        __syncthreads();
+      };
+#pragma unroll 1
+      for (int pz = 0; pz < 2; ++pz) {
+        if (pz == 1) {
+#pragma unroll
+          for (int i = 0; i < IZS; ++i) {
+            const int e = t + i * TH;
+            if (e < NC * N) sp[e + e / (3 * N)] = orig[i] * l2n[e % N];
+          }
+          __syncthreads();
+        }
+        zdft(std::false_type{});
+        tmul(py, pz);
+        __syncthreads();
+        zdft(std::true_type{});
+#pragma unroll
+        for (int i = 0; i < IZS; ++i) {
+          const int e = t + i * TH;
+          if (e < NC * N) {
+            const cplx<T> r = sp[e + e / (3 * N)];
+            res[i] = pz == 0 ? r : res[i] + cconj(l2n[e % N]) * r;
+          }
+        }
+        __syncthreads();
+      }
+#pragma unroll
+      for (int i = 0; i < IZS; ++i) {
+        const int e = t + i * TH;
+        if (e < NC * N) sp[e + e / (3 * N)] = res[i];
       }
       __syncthreads();
     }
