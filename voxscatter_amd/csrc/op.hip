@@ -57,7 +57,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_row_fft(FftDev<T> plan, IO io, 
     const int l = e / n, x = e - l * n;
     const int64_t g = g0 + l;
     cplx<T> v = mk<T>(0, 0);
-    if (x < n_in && g < nlines) v = io.load(g, x, rhs);
+    if (x < n_in && g < nlines) v = io_finish(io, g, x, rhs, io.load(g, x, rhs));
     buf[x * ld + l] = v;
   }
   __syncthreads();
@@ -122,6 +122,15 @@ struct IoDense {
 };
 
 // compressed vector -> dense volume (x FFT of the L x (lines) volume, stored with row pitch L)
+// v where ok, else zero, as a multiplication by 1 or 0: a select would let the compiler move the (clamped,
+// always valid) load that produced v under a branch on ok, and the loads of consecutive elements would then
+// wait on each other instead of being in flight together
+template <typename T>
+__device__ __forceinline__ cplx<T> masked(cplx<T> v, bool ok) {
+  const T k = ok ? (T)1 : (T)0;
+  return mk<T>(v.x * k, v.y * k);
+}
+
 template <typename T>
 struct IoGather {
   const cplx<T>* x;
@@ -133,8 +142,7 @@ struct IoGather {
   using Pre = NoPre;
   __device__ __forceinline__ cplx<T> load(int64_t g, int xp, int rhs) const {
     const int m = map3[g * L + xp];
-    const cplx<T> v = x[rhs * nunk + (m >= 0 ? m : 0)];
-    return m >= 0 ? v : mk<T>(0, 0);
+    return masked(x[rhs * nunk + (m >= 0 ? m : 0)], m >= 0);
   }
   __device__ __forceinline__ Pre pre(int64_t, int, int) const { return {}; }
   __device__ __forceinline__ void store(int64_t g, int k, int rhs, cplx<T> v, const Pre& = {}) const {
@@ -177,9 +185,15 @@ struct IoMvFx {
   __device__ __forceinline__ cplx<T> load(int64_t g, int xp, int rhs) const {
     const int64_t gi = g * L + xp;
     const int m = map3[gi];
-    cplx<T> J = x[rhs * nunk + (m >= 0 ? m : 0)];
-    if (transp) J = mc.at(gi % V) * cconj(J);
-    return m >= 0 ? J : mk<T>(0, 0);
+    return masked(x[rhs * nunk + (m >= 0 ? m : 0)], m >= 0);
+  }
+  // the transp pre-operation Mc conj(J), applied once every input of a thread is loaded (io_finish); the
+  // voxel of line g = y + M (z + N c) at x is (g mod MN) L + x
+  __device__ __forceinline__ cplx<T> finish(int64_t g, int xp, int, cplx<T> J) const {
+    if (!transp) return J;
+    const int64_t MN = V / L;
+    const int64_t gv = g - MN * ((g >= MN ? 1 : 0) + (g >= 2 * MN ? 1 : 0));
+    return mc.at(gv * L + xp) * cconj(J);
   }
   __device__ __forceinline__ Pre pre(int64_t, int, int) const { return {}; }
   __device__ __forceinline__ void store(int64_t g, int k, int rhs, cplx<T> v, const Pre& = {}) const {
@@ -243,6 +257,20 @@ struct IoMvFxSlab {
     in.S[rhs * in.S_rhs + k * SL + g] = v;
   }
 };
+
+// the IO's per-value step after its loads (identity but for the matvec's first pass)
+template <class IO, typename T>
+__device__ __forceinline__ cplx<T> io_finish(const IO&, int64_t, int, int, cplx<T> v) {
+  return v;
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> io_finish(const IoMvFx<T>& io, int64_t g, int xp, int rhs, cplx<T> v) {
+  return io.finish(g, xp, rhs, v);
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> io_finish(const IoMvFxSlab<T>& io, int64_t g, int xp, int rhs, cplx<T> v) {
+  return io.in.finish(g, xp, rhs, v);
+}
 
 template <typename T>
 struct IoMvIxSlab {
@@ -541,7 +569,11 @@ __global__ __launch_bounds__(256) void k_row2(const cplx<T>* __restrict__ tw, IO
 #pragma unroll
     for (int i = 0; i < NI; ++i) {
       const int e = t + i * C::THREADS;
-      lds[(e / LPW) * LP + e % LPW] = st[i];
+      const int l = e % LPW, n = e / LPW;
+      // positions past the input (and lines past the end) are zero: the step runs at position 0 of line 0,
+      // in bounds of every per-voxel table
+      const bool ok = g0 + l < nlines && n < n_in;
+      lds[(e / LPW) * LP + e % LPW] = io_finish(io, ok ? g0 + l : 0, ok ? n : 0, rhs, st[i]);
     }
     __syncthreads();
   }
@@ -556,6 +588,14 @@ __global__ __launch_bounds__(256) void k_row2(const cplx<T>* __restrict__ tw, IO
         const bool ok = live && n < n_in;
         v[n2] = io.load(ok ? g : 0, ok ? n : 0, rhs);
         if (!ok) v[n2] = mk<T>(0, 0);
+      }
+    }
+    if (!TIN) {
+#pragma unroll
+      for (int n2 = 0; n2 < R2; ++n2) {
+        const int n = s + R1 * n2;
+        const bool ok = live && n < n_in;  // zero inputs take the step at an in-bounds position
+        v[n2] = io_finish(io, ok ? g : 0, ok ? n : 0, rhs, v[n2]);
       }
     }
     fft2_step1<T, N, INV>(v, s, tw);
@@ -694,8 +734,13 @@ __global__ __launch_bounds__(512) void k_row2p(const cplx<T>* __restrict__ tw, c
 #pragma unroll
     for (int n2 = 0; n2 < R2; ++n2) {
       const int n = s + R1 * n2;
-      cplx<T> x = io.load(live ? g0 + l : 0, live ? n : 0, rhs);
-      if (!live) x = mk<T>(0, 0);
+      v[n2] = io.load(live ? g0 + l : 0, live ? n : 0, rhs);
+      if (!live) v[n2] = mk<T>(0, 0);
+    }
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) {
+      const int n = s + R1 * n2;
+      const cplx<T> x = io_finish(io, live ? g0 + l : 0, n, rhs, v[n2]);
       v[n2] = par ? x * tw2[n] : x;
     }
     fft2_step1<T, NH, false>(v, s, tw);
