@@ -1071,11 +1071,14 @@ __global__ __launch_bounds__(256) void k_blockmv2q(const c64* __restrict__ B, co
 // of 8).  Each wave computes 16 RT rows x 16 CT columns of its block; the loads of the next chunk of KC
 // block columns are issued before the products of the current one (two LDS buffers) and their results are
 // touched only after those products.
-template <int RT, int CT, int KC>
+template <int RT, int CT, int KC, int WB>
 __global__ __launch_bounds__(512) void k_blockmv2t(const c64* __restrict__ B, const c64* __restrict__ U,
                                                    c64* __restrict__ U2, int L, int M, int Lb, int Mb, int n,
                                                    int RG, int CG, int64_t rhs_stride, int nrhs) {
-  constexpr int WB = 8, KS = KC / 4;
+  // WB blocks per workgroup, WPB = 8 / WB wavefronts per block (row bands of 16 RT rows each) sharing the
+  // block's staged columns
+  constexpr int KS = KC / 4, WPB = 8 / WB;
+  static_assert(WB * WPB == 8, "eight wavefronts");
   constexpr int QW = 16 * CT;
   constexpr int PER = WB * KC * QW / 512;  // staged values per lane per chunk
   c64* lds = reinterpret_cast<c64*>(vs_smem);  // [2][WB][KC][QW + 1]
@@ -1092,9 +1095,9 @@ __global__ __launch_bounds__(512) void k_blockmv2t(const c64* __restrict__ B, co
   const int cg = (int)(rest % CG), rg = (int)(rest / CG);
   const int b0 = (int)tk * WB;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int bw = b0 + wave;
+  const int bw = b0 + wave / WPB;
   const int kx = bw % Lb, kyb = bw / Lb;
-  const int r0 = rg * 16 * RT, q0 = cg * QW;
+  const int r0 = (rg * WPB + wave % WPB) * 16 * RT, q0 = cg * QW;
   const int NQ = 4 * nrhs;
   const int N = n / 3;
   const int64_t sstride = (int64_t)L * M;
@@ -1180,7 +1183,7 @@ __global__ __launch_bounds__(512) void k_blockmv2t(const c64* __restrict__ B, co
       stage_load(s0 + KC, sv);
       load_a(s0 + KC, anxt);
     }
-    const c64* ub = lds + (size_t)(buf * WB + wave) * KC * QP;
+    const c64* ub = lds + (size_t)(buf * WB + wave / WPB) * KC * QP;
 #pragma unroll
     for (int j = 0; j < KS; ++j) {
       c64 b[CT];
@@ -1521,6 +1524,19 @@ static int bmm_tuned_rs(vs_ctx* ctx, const BmmKey& key, int RS0, F&& launch) {
   return best_rs;
 }
 
+// Stored blocks per workgroup of the matrix-core product: 4 (two wavefronts per block, row bands of 96
+// sharing the staged columns, so they are read half as often) when the block size is a multiple of 96,
+// else 8 (one wavefront per block, 48-row bands).  96^3 particle, 64 right-hand sides: 5.27 -> 5.03 ms.
+// VS_BMT_WB=4|8 forces one.
+static int bmm_tile_blocks(int n) {
+  static const int forced = [] {
+    const char* e = std::getenv("VS_BMT_WB");
+    return e ? std::atoi(e) : 0;
+  }();
+  if (forced == 4 || forced == 8) return forced;
+  return n % 96 == 0 ? 4 : 8;
+}
+
 // VS_BMM_W: at most this many stored columns per workgroup of the mirrored product (default: the tile)
 static int bmm_w_max() {
   static const int v = [] {
@@ -1585,15 +1601,23 @@ static cplx<T>* pc_apply_impl(vs_pc* pc, const cplx<T>* x, cplx<T>* y, int nrhs,
           // tiles of 8 blocks: 48-row groups, 64-column groups (fewer columns: 32 / 16); chunks of 8 block
           // columns (16 measured 4% slower: 256 VGPRs)
           const int CTt = NQ >= 64 ? 4 : (NQ >= 32 ? 2 : 1);
-          const int RGt = (n + 47) / 48, CGt = (NQ + 16 * CTt - 1) / (16 * CTt);
-          const int64_t Gt = (int64_t)((Lb * Mb + 7) / 8) * RGt * CGt;
-          const size_t ldst = sizeof(c64) * 2 * 8 * 8 * (16 * CTt + 1);
+          const int WBt = bmm_tile_blocks(n);  // blocks per workgroup (8 / WBt wavefronts per block)
+          const int RGt = (n + 48 * (8 / WBt) - 1) / (48 * (8 / WBt)), CGt = (NQ + 16 * CTt - 1) / (16 * CTt);
+          const int64_t Gt = (int64_t)((Lb * Mb + WBt - 1) / WBt) * RGt * CGt;
+          const size_t ldst = sizeof(c64) * 2 * WBt * 8 * (16 * CTt + 1);
           dim3 gridt((unsigned)(((Gt + 7) / 8) * 8));
-          switch (CTt) {
-            case 4: hipLaunchKernelGGL((k_blockmv2t<3, 4, 8>), gridt, dim3(512), ldst, ctx->stream, Bq, Ui, Uo, L, M, Lb, Mb, n, RGt, CGt, V3, nrhs); break;
-            case 2: hipLaunchKernelGGL((k_blockmv2t<3, 2, 8>), gridt, dim3(512), ldst, ctx->stream, Bq, Ui, Uo, L, M, Lb, Mb, n, RGt, CGt, V3, nrhs); break;
-            default: hipLaunchKernelGGL((k_blockmv2t<3, 1, 8>), gridt, dim3(512), ldst, ctx->stream, Bq, Ui, Uo, L, M, Lb, Mb, n, RGt, CGt, V3, nrhs); break;
+#define VS_BMT(WBV)                                                                                             \
+  switch (CTt) {                                                                                                \
+    case 4: hipLaunchKernelGGL((k_blockmv2t<3, 4, 8, WBV>), gridt, dim3(512), ldst, ctx->stream, Bq, Ui, Uo, L, M, Lb, Mb, n, RGt, CGt, V3, nrhs); break; \
+    case 2: hipLaunchKernelGGL((k_blockmv2t<3, 2, 8, WBV>), gridt, dim3(512), ldst, ctx->stream, Bq, Ui, Uo, L, M, Lb, Mb, n, RGt, CGt, V3, nrhs); break; \
+    default: hipLaunchKernelGGL((k_blockmv2t<3, 1, 8, WBV>), gridt, dim3(512), ldst, ctx->stream, Bq, Ui, Uo, L, M, Lb, Mb, n, RGt, CGt, V3, nrhs); break; \
+  }
+          if (WBt == 4) {
+            VS_BMT(4)
+          } else {
+            VS_BMT(8)
           }
+#undef VS_BMT
         } else switch (CT) {
           case 1: hipLaunchKernelGGL((k_blockmv2q<1>), grid, dim3(64 * NW), lds, ctx->stream, Bq, Ui, Uo, L, M, Lb, Mb, n, RG, CG, V3, nrhs); break;
           case 2: hipLaunchKernelGGL((k_blockmv2q<2>), grid, dim3(64 * NW), lds, ctx->stream, Bq, Ui, Uo, L, M, Lb, Mb, n, RG, CG, V3, nrhs); break;
