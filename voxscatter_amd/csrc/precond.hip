@@ -1071,16 +1071,15 @@ __global__ __launch_bounds__(256) void k_blockmv2q(const c64* __restrict__ B, co
 // of 8).  Each wave computes 16 RT rows x 16 CT columns of its block; the loads of the next chunk of KC
 // block columns are issued before the products of the current one (two LDS buffers) and their results are
 // touched only after those products.
-template <int RT, int CT, int KC, int WB>
-__global__ __launch_bounds__(512) void k_blockmv2t(const c64* __restrict__ B, const c64* __restrict__ U,
+template <int RT, int CT, int KC, int WB, int WPB, int MINW>
+__global__ __launch_bounds__(64 * WB * WPB, MINW) void k_blockmv2t(const c64* __restrict__ B, const c64* __restrict__ U,
                                                    c64* __restrict__ U2, int L, int M, int Lb, int Mb, int n,
                                                    int RG, int CG, int64_t rhs_stride, int nrhs) {
-  // WB blocks per workgroup, WPB = 8 / WB wavefronts per block (row bands of 16 RT rows each) sharing the
-  // block's staged columns
-  constexpr int KS = KC / 4, WPB = 8 / WB;
-  static_assert(WB * WPB == 8, "eight wavefronts");
+  // WB blocks per workgroup, WPB wavefronts per block (row bands of 16 RT rows each) sharing the block's
+  // staged columns; MINW wavefronts per SIMD asked of the register allocation
+  constexpr int KS = KC / 4, TH = 64 * WB * WPB;
   constexpr int QW = 16 * CT;
-  constexpr int PER = WB * KC * QW / 512;  // staged values per lane per chunk
+  constexpr int PER = WB * KC * QW / TH;  // staged values per lane per chunk
   c64* lds = reinterpret_cast<c64*>(vs_smem);  // [2][WB][KC][QW + 1]
   constexpr int QP = QW + 1;                    // padded column stride: rows of a block on other banks
   const int nblk = Lb * Mb;  // stored blocks, kx fastest; a tile may wrap to the next ky
@@ -1101,11 +1100,11 @@ __global__ __launch_bounds__(512) void k_blockmv2t(const c64* __restrict__ B, co
   const int NQ = 4 * nrhs;
   const int N = n / 3;
   const int64_t sstride = (int64_t)L * M;
-  // staging: element e = j * 512 + thread = ((sl * QW + ql) * WB + w): consecutive lanes take consecutive
-  // blocks; a lane keeps one (block w, column ql) for every chunk and element (WB * QW divides 512), so that
+  // staging: element e = j * TH + thread = ((sl * QW + ql) * WB + w): consecutive lanes take consecutive
+  // blocks; a lane keeps one (block w, column ql) for every chunk and element (WB * QW divides TH), so that
   // its base offset, liveness and mirror flags are computed once
-  static_assert(512 % (WB * QW) == 0 && KC % (512 / (WB * QW)) == 0, "staging layout");
-  constexpr int SPJ = 512 / (WB * QW);  // staged rows per element index j
+  static_assert(TH % (WB * QW) == 0 && KC % (TH / (WB * QW)) == 0, "staging layout");
+  constexpr int SPJ = TH / (WB * QW);  // staged rows per element index j
   const int w_l = (int)threadIdx.x % WB, ql_l = ((int)threadIdx.x / WB) % QW, sl_l = (int)threadIdx.x / (WB * QW);
   int64_t base_l = 0;
   bool live_l, xm_l, ym_l;
@@ -1524,17 +1523,18 @@ static int bmm_tuned_rs(vs_ctx* ctx, const BmmKey& key, int RS0, F&& launch) {
   return best_rs;
 }
 
-// Stored blocks per workgroup of the matrix-core product: 4 (two wavefronts per block, row bands of 96
-// sharing the staged columns, so they are read half as often) when the block size is a multiple of 96,
-// else 8 (one wavefront per block, 48-row bands).  96^3 particle, 64 right-hand sides: 5.27 -> 5.03 ms.
-// VS_BMT_WB=4|8 forces one.
-static int bmm_tile_blocks(int n) {
+// Tiling of the matrix-core product: 0 = 8 blocks per 512-lane workgroup, one wavefront each (48-row
+// bands); 1 = 4 blocks with two wavefronts each (row bands of 96 sharing the block's staged columns, which
+// are then read half as often), taken when the block size is a multiple of 96 (96^3 particle, 64
+// right-hand sides: 5.27 -> 5.03 ms).  256-lane workgroups (2 x 2 or 4 x 1, registers capped for three per
+// CU) measured 6.06 / 6.22 ms there (spills, shorter gather runs).  VS_BMT_CFG=0|1 forces one.
+static int bmm_tile_cfg(int n) {
   static const int forced = [] {
-    const char* e = std::getenv("VS_BMT_WB");
-    return e ? std::atoi(e) : 0;
+    const char* e = std::getenv("VS_BMT_CFG");
+    return e ? std::atoi(e) : -1;
   }();
-  if (forced == 4 || forced == 8) return forced;
-  return n % 96 == 0 ? 4 : 8;
+  if (forced == 0 || forced == 1) return forced;
+  return n % 96 == 0 ? 1 : 0;
 }
 
 // VS_BMM_W: at most this many stored columns per workgroup of the mirrored product (default: the tile)
@@ -1601,21 +1601,23 @@ static cplx<T>* pc_apply_impl(vs_pc* pc, const cplx<T>* x, cplx<T>* y, int nrhs,
           // tiles of 8 blocks: 48-row groups, 64-column groups (fewer columns: 32 / 16); chunks of 8 block
           // columns (16 measured 4% slower: 256 VGPRs)
           const int CTt = NQ >= 64 ? 4 : (NQ >= 32 ? 2 : 1);
-          const int WBt = bmm_tile_blocks(n);  // blocks per workgroup (8 / WBt wavefronts per block)
-          const int RGt = (n + 48 * (8 / WBt) - 1) / (48 * (8 / WBt)), CGt = (NQ + 16 * CTt - 1) / (16 * CTt);
+          const int cfg = bmm_tile_cfg(n);  // 0: 8 blocks x 1 wavefront, 1: 4 blocks x 2 wavefronts
+          const int WBt = cfg == 0 ? 8 : 4, WPBt = cfg == 1 ? 2 : 1;
+          const int RGt = (n + 48 * WPBt - 1) / (48 * WPBt), CGt = (NQ + 16 * CTt - 1) / (16 * CTt);
           const int64_t Gt = (int64_t)((Lb * Mb + WBt - 1) / WBt) * RGt * CGt;
           const size_t ldst = sizeof(c64) * 2 * WBt * 8 * (16 * CTt + 1);
           dim3 gridt((unsigned)(((Gt + 7) / 8) * 8));
-#define VS_BMT(WBV)                                                                                             \
+          const dim3 tht(64 * WBt * WPBt);
+#define VS_BMT(WBV, WPBV, MINWV)                                                                                \
   switch (CTt) {                                                                                                \
-    case 4: hipLaunchKernelGGL((k_blockmv2t<3, 4, 8, WBV>), gridt, dim3(512), ldst, ctx->stream, Bq, Ui, Uo, L, M, Lb, Mb, n, RGt, CGt, V3, nrhs); break; \
-    case 2: hipLaunchKernelGGL((k_blockmv2t<3, 2, 8, WBV>), gridt, dim3(512), ldst, ctx->stream, Bq, Ui, Uo, L, M, Lb, Mb, n, RGt, CGt, V3, nrhs); break; \
-    default: hipLaunchKernelGGL((k_blockmv2t<3, 1, 8, WBV>), gridt, dim3(512), ldst, ctx->stream, Bq, Ui, Uo, L, M, Lb, Mb, n, RGt, CGt, V3, nrhs); break; \
+    case 4: hipLaunchKernelGGL((k_blockmv2t<3, 4, 8, WBV, WPBV, MINWV>), gridt, tht, ldst, ctx->stream, Bq, Ui, Uo, L, M, Lb, Mb, n, RGt, CGt, V3, nrhs); break; \
+    case 2: hipLaunchKernelGGL((k_blockmv2t<3, 2, 8, WBV, WPBV, MINWV>), gridt, tht, ldst, ctx->stream, Bq, Ui, Uo, L, M, Lb, Mb, n, RGt, CGt, V3, nrhs); break; \
+    default: hipLaunchKernelGGL((k_blockmv2t<3, 1, 8, WBV, WPBV, MINWV>), gridt, tht, ldst, ctx->stream, Bq, Ui, Uo, L, M, Lb, Mb, n, RGt, CGt, V3, nrhs); break; \
   }
-          if (WBt == 4) {
-            VS_BMT(4)
+          if (cfg == 1) {
+            VS_BMT(4, 2, 1)
           } else {
-            VS_BMT(8)
+            VS_BMT(8, 1, 1)
           }
 #undef VS_BMT
         } else switch (CT) {
