@@ -172,8 +172,9 @@ def particles_roofline(api, parts, ctx, ndir):
           file=sys.stderr, flush=True)
     return {"bound": "mfma", "achieved": round(tf, 2), "peak": MFMA_FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(tf / MFMA_FP32_PEAK_TFLOPS, 4), "traffic": None,
-            "kernel": f"FP32 2-level block product on the matrix cores (k_blockmv2t, tiles of 8 stored blocks), "
-                      f"{nrhs} right-hand sides of "
+            "kernel": f"FP32 2-level block product on the matrix cores (k_blockmv2t; "
+                      f"{'4 stored blocks x 2 wavefronts' if n % 96 == 0 else '8 stored blocks x 1 wavefront'} "
+                      f"per workgroup), {nrhs} right-hand sides of "
                       f"particle {p.name} ({L}x{M}x{N}, blocks {n}x{n})",
             "flops_per_launch": flops, "ms_per_launch": round(ms, 4)}
 

@@ -201,19 +201,23 @@ def test_fp32_matrix_core_apply_matches_oracle(ctx, shape, monkeypatch):
     # the matrix-core copy of the blocks is built for small blocks only when that product is asked for
     monkeypatch.setenv("VS_BMM_MFMA_MIN", "1")
     pc = api.Preconditioner.from_operator(op, 2, 1 / alpha, wscalar=d ** 3, precisions=VS_FP32 | VS_FP64)
-    for nrhs, tile in ((1, "1"), (3, "1"), (5, "0"), (5, "1"), (17, "1"), (33, "0"), (33, "1")):
+    # tiles: 8 blocks x one wavefront (cfg 0) and 4 blocks x two wavefronts (cfg 1, the default for block
+    # sizes that are multiples of 96), both forced on every shape
+    for nrhs, tile, cfg in ((1, "1", "0"), (3, "1", "1"), (5, "0", "0"), (5, "1", "0"), (5, "1", "1"),
+                            (17, "1", "0"), (17, "1", "1"), (33, "0", "0"), (33, "1", "0"), (33, "1", "1")):
         X = rng.normal(size=(nrhs, idx3.size)) + 1j * rng.normal(size=(nrhs, idx3.size))
         Yref = np.stack([Ch.mvp_circ_2_level(inv, x, L, M, N, idx3) for x in X])
         Xd = api.DeviceArray.from_numpy(X, np.complex64, ctx)
         Yd = api.DeviceArray(X.shape, np.complex64, ctx)
         monkeypatch.setenv("VS_BMM_MFMA_MIN", "1")
         monkeypatch.setenv("VS_BMM_TILE", tile)
+        monkeypatch.setenv("VS_BMT_CFG", cfg)
         pc.apply_dev(Xd.ptr, Yd.ptr, VS_FP32, nrhs)
         ym = Yd.numpy().astype(np.complex128)
         monkeypatch.setenv("VS_BMM_MFMA_MIN", "100000")
         pc.apply_dev(Xd.ptr, Yd.ptr, VS_FP32, nrhs)
         yv = Yd.numpy().astype(np.complex128)
         em, ev = _rel(ym, Yref), _rel(ym, yv)
-        print(f"[mfma apply] nrhs={nrhs} tile={tile}: vs oracle {em:.2e}, vs vector-unit FP32 {ev:.2e}",
-              flush=True)
+        print(f"[mfma apply] nrhs={nrhs} tile={tile} cfg={cfg}: vs oracle {em:.2e}, vs vector-unit FP32 "
+              f"{ev:.2e}", flush=True)
         assert em <= 2e-5 and ev <= 2e-5

@@ -1529,10 +1529,8 @@ static int bmm_tuned_rs(vs_ctx* ctx, const BmmKey& key, int RS0, F&& launch) {
 // right-hand sides: 5.27 -> 5.03 ms).  256-lane workgroups (2 x 2 or 4 x 1, registers capped for three per
 // CU) measured 6.06 / 6.22 ms there (spills, shorter gather runs).  VS_BMT_CFG=0|1 forces one.
 static int bmm_tile_cfg(int n) {
-  static const int forced = [] {
-    const char* e = std::getenv("VS_BMT_CFG");
-    return e ? std::atoi(e) : -1;
-  }();
+  const char* e = std::getenv("VS_BMT_CFG");  // read per call, like VS_BMM_TILE
+  const int forced = e ? std::atoi(e) : -1;
   if (forced == 0 || forced == 1) return forced;
   return n % 96 == 0 ? 1 : 0;
 }
