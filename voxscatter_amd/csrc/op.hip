@@ -1104,6 +1104,15 @@ __global__ __launch_bounds__((FzCfg<T, N2>::THREADS)) void k_fz2(const cplx<T>* 
   const unsigned zs = (unsigned)L2 * (unsigned)M2, cs = zs * (unsigned)NH;
   cplx<T>* gcol = Sb + (c * cs + (unsigned)ky * (unsigned)L2 + (unsigned)(kx0 + col));
   constexpr bool RZ = NH % R1 == 0;
+  // FP32: the tensor entries of this thread's first product point (e = t below) are requested here, so that
+  // their round trip overlaps the forward transforms (12 registers, within the two workgroups per CU that
+  // the LDS allows)
+  constexpr bool PF = sizeof(T) == 4;
+  cplx<T> f0[PF ? 6 : 1];
+  if constexpr (PF) {
+    const int e = min(t, W * (NH + 1) - 1);
+    fN.load6(min(kx0 + e % W, L2 - 1), ky, e / W, f0);
+  }
   // forward transform of the three components
   if (s < R1) {
     cplx<T> v[R2];
@@ -1133,25 +1142,33 @@ __global__ __launch_bounds__((FzCfg<T, N2>::THREADS)) void k_fz2(const cplx<T>* 
   }
   __syncthreads();
   // tensor product, one (kx, kz <= N) per iteration, applied at kz and at 2N - kz
-  for (int e = t; e < W * (NH + 1); e += C::THREADS) {
+  auto product = [&](int cc, int kz, const cplx<T> (&f)[6]) {
+    cplx<T>* sp = spec + kz * W + cc;
+    cplx<T> jx = sp[0], jy = sp[N2 * W], jz = sp[2 * N2 * W];
+    sp[0] = cfma(f[2], jz, cfma(f[1], jy, f[0] * jx));
+    sp[N2 * W] = cfma(f[4], jz, cfma(f[3], jy, f[1] * jx));
+    sp[2 * N2 * W] = cfma(f[5], jz, cfma(f[4], jy, f[2] * jx));
+    if (kz != 0 && kz != NH) {
+      sp = spec + (N2 - kz) * W + cc;
+      jx = sp[0];
+      jy = sp[N2 * W];
+      jz = sp[2 * N2 * W];
+      sp[0] = cfma(-f[2], jz, cfma(f[1], jy, f[0] * jx));
+      sp[N2 * W] = cfma(-f[4], jz, cfma(f[3], jy, f[1] * jx));
+      sp[2 * N2 * W] = cfma(f[5], jz, cfma(-f[4], jy, (-f[2]) * jx));
+    }
+  };
+  int e = t;
+  if constexpr (PF) {
+    if (e < W * (NH + 1) && kx0 + e % W < L2) product(e % W, e / W, f0);
+    e += C::THREADS;
+  }
+  for (; e < W * (NH + 1); e += C::THREADS) {
     const int cc = e % W, kz = e / W;
     if (kx0 + cc < L2) {
       cplx<T> f[6];
       fN.load6(kx0 + cc, ky, kz, f);
-      cplx<T>* sp = spec + kz * W + cc;
-      cplx<T> jx = sp[0], jy = sp[N2 * W], jz = sp[2 * N2 * W];
-      sp[0] = cfma(f[2], jz, cfma(f[1], jy, f[0] * jx));
-      sp[N2 * W] = cfma(f[4], jz, cfma(f[3], jy, f[1] * jx));
-      sp[2 * N2 * W] = cfma(f[5], jz, cfma(f[4], jy, f[2] * jx));
-      if (kz != 0 && kz != NH) {
-        sp = spec + (N2 - kz) * W + cc;
-        jx = sp[0];
-        jy = sp[N2 * W];
-        jz = sp[2 * N2 * W];
-        sp[0] = cfma(-f[2], jz, cfma(f[1], jy, f[0] * jx));
-        sp[N2 * W] = cfma(-f[4], jz, cfma(f[3], jy, f[1] * jx));
-        sp[2 * N2 * W] = cfma(f[5], jz, cfma(-f[4], jy, (-f[2]) * jx));
-      }
+      product(cc, kz, f);
     }
   }
   __syncthreads();
