@@ -1075,7 +1075,7 @@ struct FzCfg {
 // and writes after it.  The tensor stage takes kz and 2N - kz together: they share one octant entry (the
 // xz and yz components change sign), so each entry is read once per (kx, ky) instead of twice.
 template <typename T, int N2>
-__global__ __launch_bounds__((FzCfg<T, N2>::THREADS)) void k_fz2(const cplx<T>* __restrict__ tw, cplx<T>* S,
+__global__ __launch_bounds__((FzCfg<T, N2>::THREADS), (FzCfg<T, N2>::THREADS / 128)) void k_fz2(const cplx<T>* __restrict__ tw, cplx<T>* S,
                                                                 TensorC<T> fN, int L2, int M2, int N, int64_t S_rhs) {
   using C = FzCfg<T, N2>;
   constexpr int R1 = C::R1, R2 = C::R2, W = C::W, NH = N2 / 2;
@@ -1104,14 +1104,16 @@ __global__ __launch_bounds__((FzCfg<T, N2>::THREADS)) void k_fz2(const cplx<T>* 
   const unsigned zs = (unsigned)L2 * (unsigned)M2, cs = zs * (unsigned)NH;
   cplx<T>* gcol = Sb + (c * cs + (unsigned)ky * (unsigned)L2 + (unsigned)(kx0 + col));
   constexpr bool RZ = NH % R1 == 0;
-  // FP32: the tensor entries of this thread's first product point (e = t below) are requested here, so that
-  // their round trip overlaps the forward transforms (12 registers, within the two workgroups per CU that
-  // the LDS allows)
-  constexpr bool PF = sizeof(T) == 4;
-  cplx<T> f0[PF ? 6 : 1];
-  if constexpr (PF) {
-    const int e = min(t, W * (NH + 1) - 1);
-    fN.load6(min(kx0 + e % W, L2 - 1), ky, e / W, f0);
+  // FP32: the tensor entries of this thread's first product points (e = t, t + THREADS below: two for the
+  // long lines, where every thread has at least two, one otherwise) are requested here, so that their round
+  // trips overlap the forward transforms (12 registers each, within the two workgroups per CU that the LDS
+  // allows)
+  constexpr int NPF = sizeof(T) == 4 ? (N2 >= 192 ? 2 : 1) : 0;
+  cplx<T> f0[NPF > 0 ? NPF : 1][6];
+#pragma unroll
+  for (int i = 0; i < NPF; ++i) {
+    const int e = min(t + i * C::THREADS, W * (NH + 1) - 1);
+    fN.load6(min(kx0 + e % W, L2 - 1), ky, e / W, f0[i]);
   }
   // forward transform of the three components
   if (s < R1) {
@@ -1159,10 +1161,9 @@ __global__ __launch_bounds__((FzCfg<T, N2>::THREADS)) void k_fz2(const cplx<T>* 
     }
   };
   int e = t;
-  if constexpr (PF) {
-    if (e < W * (NH + 1) && kx0 + e % W < L2) product(e % W, e / W, f0);
-    e += C::THREADS;
-  }
+#pragma unroll
+  for (int i = 0; i < NPF; ++i, e += C::THREADS)
+    if (e < W * (NH + 1) && kx0 + e % W < L2) product(e % W, e / W, f0[i]);
   for (; e < W * (NH + 1); e += C::THREADS) {
     const int cc = e % W, kz = e / W;
     if (kx0 + cc < L2) {
