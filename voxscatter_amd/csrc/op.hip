@@ -1068,6 +1068,8 @@ struct FzCfg {
   static constexpr int W0 = pow2_floor(WA < WB ? WA : WB);
   static constexpr int W = W0 > 32 ? 32 : (W0 < 1 ? 1 : W0);
   static constexpr int THREADS = 3 * TPL * W;
+  // waves per SIMD of two resident workgroups (the LDS allows two): the register cap of the launch bound
+  static constexpr int MINW = THREADS / 128 > 0 ? THREADS / 128 : 1;
 };
 
 // z pass of the matvec: zero-pad z to 2N, FFT, 3x3 symmetric tensor product, inverse FFT, crop to N.
@@ -1075,7 +1077,7 @@ struct FzCfg {
 // and writes after it.  The tensor stage takes kz and 2N - kz together: they share one octant entry (the
 // xz and yz components change sign), so each entry is read once per (kx, ky) instead of twice.
 template <typename T, int N2>
-__global__ __launch_bounds__((FzCfg<T, N2>::THREADS), (FzCfg<T, N2>::THREADS / 128)) void k_fz2(const cplx<T>* __restrict__ tw, cplx<T>* S,
+__global__ __launch_bounds__((FzCfg<T, N2>::THREADS), (FzCfg<T, N2>::MINW)) void k_fz2(const cplx<T>* __restrict__ tw, cplx<T>* S,
                                                                 TensorC<T> fN, int L2, int M2, int N, int64_t S_rhs) {
   using C = FzCfg<T, N2>;
   constexpr int R1 = C::R1, R2 = C::R2, W = C::W, NH = N2 / 2;
