@@ -614,10 +614,13 @@ __global__ __launch_bounds__(256) void k_row2(const cplx<T>* __restrict__ tw, IO
     if (TOUT) {
       // staged below, once every thread has read the exchange
     } else {
-      // the loads the stores depend on, PB outputs at a time (bounded register use)
+      // the loads the stores depend on, PB outputs at a time (bounded register use); outputs grow with k0, so
+      // once a batch's first one is cropped (k >= n_out: the padded inverse keeps half) no later one is
+      // stored and their loads are not issued
       constexpr int PB = sizeof(T) == 8 ? 4 : 8;
 #pragma unroll
       for (int k0 = 0; k0 < R1; k0 += PB) {
+        if (s + R2 * k0 >= n_out) break;
         typename IO::Pre pr[PB];
 #pragma unroll
         for (int j = 0; j < PB; ++j) {
