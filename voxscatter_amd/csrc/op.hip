@@ -542,7 +542,9 @@ struct RowCfg {
 // lines along the contiguous axis; IO supplies load(line, pos, rhs) / store(line, k, rhs, v).
 // TIN / TOUT stage the load / store through LDS so that consecutive threads touch consecutive lines at
 // one position (for line-contiguous layouts such as the slab-major matvec volume).
-template <typename T, int N, bool INV, class IO, bool TIN = false, bool TOUT = false>
+// PADIN: the forward transform of a line zero-padded to twice its length (n_in = N / 2); with R1 | N / 2 the
+// padding is the inputs n2 >= N / (2 R1) of every thread, known at compile time, so no load is issued for it
+template <typename T, int N, bool INV, class IO, bool TIN = false, bool TOUT = false, bool PADIN = false>
 __global__ __launch_bounds__(256) void k_row2(const cplx<T>* __restrict__ tw, IO io, int64_t nlines, int n_in,
                                               int n_out) {
   using C = RowCfg<T, N>;
@@ -556,6 +558,7 @@ __global__ __launch_bounds__(256) void k_row2(const cplx<T>* __restrict__ tw, IO
   const int rhs = blockIdx.y;
   const bool live = g < nlines;
   constexpr int NI = N / TPL;  // staged elements per thread (THREADS = TPL * LPW, TPL divides N)
+  constexpr bool SZ = PADIN && !INV && (N / 2) % R1 == 0;
   if (TIN) {
     cplx<T> st[NI];
 #pragma unroll
@@ -584,6 +587,8 @@ __global__ __launch_bounds__(256) void k_row2(const cplx<T>* __restrict__ tw, IO
       const int n = s + R1 * n2;
       if (TIN) {
         v[n2] = lds[n * LP + ll];
+      } else if (SZ && R1 * n2 >= N / 2) {
+        v[n2] = mk<T>(0, 0);
       } else {
         const bool ok = live && n < n_in;
         v[n2] = io.load(ok ? g : 0, ok ? n : 0, rhs);
@@ -593,6 +598,7 @@ __global__ __launch_bounds__(256) void k_row2(const cplx<T>* __restrict__ tw, IO
     if (!TIN) {
 #pragma unroll
       for (int n2 = 0; n2 < R2; ++n2) {
+        if (SZ && R1 * n2 >= N / 2) continue;  // static padding: zero before and after the step
         const int n = s + R1 * n2;
         const bool ok = live && n < n_in;  // zero inputs take the step at an in-bounds position
         v[n2] = io_finish(io, ok ? g : 0, ok ? n : 0, rhs, v[n2]);
@@ -1922,8 +1928,9 @@ static int col_tile(int n, int64_t fast, int64_t nbatch, size_t eb) {
   return W;
 }
 
-template <typename T, bool INV, class IO, bool TIN = false, bool TOUT = false>
+template <typename T, bool INV, class IO, bool TIN = false, bool TOUT = false, bool PADIN = false>
 static void run_rows(vs_ctx* ctx, const IO& io, int n, int64_t nlines, int n_in, int n_out, int nrhs) {
+  VS_CHECK(!PADIN || 2 * n_in == n, VS_ERR_INTERNAL, "padded row FFT takes half-length inputs");
   const FftPlanHost& ph = ctx->plan(n);
   const FftDev<T> pd = ph.dev<T>();
   switch (n) {
@@ -1931,8 +1938,8 @@ static void run_rows(vs_ctx* ctx, const IO& io, int n, int64_t nlines, int n_in,
   case NN: {                                                                                             \
     using Cf = RowCfg<T, NN>;                                                                            \
     dim3 grid((unsigned)((nlines + Cf::LPW - 1) / Cf::LPW), (unsigned)nrhs);                            \
-    hipLaunchKernelGGL((k_row2<T, NN, INV, IO, TIN, TOUT>), grid, dim3(Cf::THREADS), 0, ctx->stream,    \
-                       pd.tw, io, nlines, n_in, n_out);                                                  \
+    hipLaunchKernelGGL((k_row2<T, NN, INV, IO, TIN, TOUT, PADIN>), grid, dim3(Cf::THREADS), 0,           \
+                       ctx->stream, pd.tw, io, nlines, n_in, n_out);                                     \
     VS_HIP(hipGetLastError());                                                                           \
     return;                                                                                              \
   }
@@ -2357,7 +2364,7 @@ static void mvp_impl(vs_op* op, const cplx<T>* x, cplx<T>* y, int transp, int nr
   }
   // 1. gather + x FFT
   IoMvFx<T> fx{x, map3, op->nunk, op->V, L, M, transp, coef_dev<T>(op->b), S, S_rhs};
-  run_rows<T, false>(ctx, fx, L2, lines, L, L2, nrhs);
+  run_rows<T, false, IoMvFx<T>, false, false, true>(ctx, fx, L2, lines, L, L2, nrhs);
   // 2. y FFT: columns (kx), axis y (stride 2L), batch (z, c) stride 2L*2M, input y < M
   run_cols<T, false>(ctx, S, L2, L2, (int64_t)N * 3, (int64_t)L2 * M2, M, M2, M2, 1.0, nrhs, S_rhs);
   // 3. fused z FFT + tensor product + inverse z FFT
